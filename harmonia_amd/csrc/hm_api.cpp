@@ -1,0 +1,955 @@
+// C ABI of libharmonia_amd: tree ownership, device buffers, epochs and the host side of every
+// pipeline (sort -> search, range, staged upsert).  The reference keeps every device buffer inside
+// one driver function per variant (allocate, run, free; H/src/full-multi-prefix-sort-8thread.cu:329-495);
+// here a tree stays resident on the device and is searched any number of times from any stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/harmonia.h"
+#include "hm_internal.h"
+
+namespace hm {
+hipError_t range_offsets_tail(const uint32_t* len, uint64_t* offsets, size_t n, hipStream_t s);
+}
+
+using namespace hm;
+
+// ---------------------------------------------------------------------------------------------
+// errors
+namespace {
+thread_local std::string g_err;
+
+int set_err(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HM_TRY(expr)                                                                                       \
+  do {                                                                                                     \
+    hipError_t e_ = (expr);                                                                                \
+    if (e_ != hipSuccess) {                                                                                \
+      int code_ = (e_ == hipErrorOutOfMemory) ? HM_ENOMEM : HM_EDEVICE;                                    \
+      return set_err(code_, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__);    \
+    }                                                                                                      \
+  } while (0)
+
+size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+// launch configuration and profiling
+namespace hm {
+
+const LaunchCfg& launch_cfg() {
+  static LaunchCfg cfg;
+  static std::once_flag once;
+  std::call_once(once, [] {
+    int dev = 0;
+    hipDeviceProp_t p;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess) {
+      cfg.num_cus = p.multiProcessorCount > 0 ? p.multiProcessorCount : 256;
+      cfg.max_lds = (int)(p.maxSharedMemoryPerMultiProcessor ? p.maxSharedMemoryPerMultiProcessor : 65536);
+    }
+  });
+  return cfg;
+}
+
+namespace {
+struct ProfRec {
+  int stage;
+  hipEvent_t a, b;
+};
+std::mutex g_prof_mu;
+bool g_prof_on = false;
+std::vector<std::string> g_stage_names;
+std::vector<ProfRec> g_recs;
+std::vector<hipEvent_t> g_ev_pool;
+
+hipEvent_t prof_event() {
+  if (!g_ev_pool.empty()) {
+    hipEvent_t e = g_ev_pool.back();
+    g_ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+}  // namespace
+
+ProfScope::ProfScope(const char* stage, hipStream_t s) : slot(-1), stream(s) {
+  if (!g_prof_on) return;
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  int id = -1;
+  for (size_t i = 0; i < g_stage_names.size(); ++i)
+    if (g_stage_names[i] == stage) id = (int)i;
+  if (id < 0) {
+    g_stage_names.emplace_back(stage);
+    id = (int)g_stage_names.size() - 1;
+  }
+  ProfRec r{id, prof_event(), prof_event()};
+  if (!r.a || !r.b) return;
+  if (hipEventRecord(r.a, s) != hipSuccess) return;
+  g_recs.push_back(r);
+  slot = (int)g_recs.size() - 1;
+}
+
+ProfScope::~ProfScope() {
+  if (slot < 0) return;
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  if (slot < (int)g_recs.size()) (void)hipEventRecord(g_recs[slot].b, stream);
+}
+
+}  // namespace hm
+
+// ---------------------------------------------------------------------------------------------
+// tree object
+namespace {
+
+struct Workspace {
+  void* buf = nullptr;
+  size_t bytes = 0;
+  hipEvent_t last_use = nullptr;  // the latest search/range issued on this stream
+};
+
+}  // namespace
+
+struct hm_tree {
+  hm_params p{};
+  int F = 64, fill = 8, NS = 72;
+  int device = 0;
+  uint64_t n_keys = 0;
+  uint64_t n_leaves = 0, leaf_cap = 0;
+  uint32_t n_inner = 0, inner_cap = 0;
+  int height = 1;
+  uint32_t level_start[MAX_LEVELS + 1] = {0};
+  uint64_t* d_inner = nullptr;
+  uint32_t* d_prefix = nullptr;
+  Leaf* d_leaves = nullptr;
+  int lds_levels = 0;
+  int group = 4;
+  std::mutex mu;
+  std::unordered_map<hipStream_t, Workspace> ws;
+  hipStream_t own = nullptr;
+  hipEvent_t publish_ev = nullptr;
+  bool publish_valid = false;
+  uint64_t epoch = 0;
+  uint64_t* pend_k = nullptr;
+  uint64_t* pend_v = nullptr;
+  size_t pend_n = 0, pend_cap = 0;
+
+  TreeView view() const {
+    TreeView tv{};
+    tv.inner = d_inner;
+    tv.prefix = d_prefix;
+    tv.leaves = d_leaves;
+    tv.n_inner = n_inner;
+    tv.n_leaves = (uint32_t)n_leaves;
+    tv.height = height;
+    tv.fanout = F;
+    for (int i = 0; i <= MAX_LEVELS; ++i) tv.level_start[i] = level_start[i];
+    return tv;
+  }
+  uint64_t device_bytes() const {
+    return (uint64_t)inner_cap * NS * 8 + (uint64_t)inner_cap * 4 + leaf_cap * sizeof(Leaf) + pend_cap * 16;
+  }
+};
+
+namespace {
+
+bool is_device_ptr(const void* p) {
+  if (!p) return false;
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+int ensure_ws(hm_tree* t, hipStream_t s, size_t bytes, Workspace** out) {
+  Workspace& w = t->ws[s];
+  if (!w.last_use) HM_TRY(hipEventCreateWithFlags(&w.last_use, hipEventDisableTiming));
+  if (w.bytes < bytes) {
+    if (w.buf) {
+      HM_TRY(hipStreamSynchronize(s));
+      HM_TRY(hipFree(w.buf));
+      w.buf = nullptr;
+      w.bytes = 0;
+    }
+    size_t want = std::max(bytes, (size_t)1 << 20);
+    HM_TRY(hipMalloc(&w.buf, want));
+    w.bytes = want;
+  }
+  *out = &w;
+  return HM_OK;
+}
+
+int choose_lds_levels(const hm_tree* t, int req) {
+  const TreeView tv = t->view();
+  int best = 0;
+  const int maxl = std::min(3, t->height);
+  // an explicit request is honoured up to 64 KiB of LDS per block; the default keeps it under 48 KiB
+  // so several blocks share a CU
+  const size_t cap = req >= 0 ? 64 * 1024 : 48 * 1024;
+  const int top = req >= 0 ? std::min(req, maxl) : maxl;
+  for (int l = 1; l <= top; ++l)
+    if (lds_bytes_for(tv, l) <= cap) best = l;
+  return best;
+}
+
+// Level sizes for n_leaves leaves; returns height.
+int plan_levels(uint64_t n_leaves, int F, uint32_t* level_start, uint32_t* n_inner) {
+  std::vector<uint64_t> sizes;
+  uint64_t m = n_leaves;
+  do {
+    m = (m + F - 1) / F;
+    sizes.push_back(m);
+  } while (m > 1);
+  std::reverse(sizes.begin(), sizes.end());
+  if ((int)sizes.size() > MAX_LEVELS) return -1;
+  uint64_t acc = 0;
+  for (size_t l = 0; l < sizes.size(); ++l) {
+    level_start[l] = (uint32_t)acc;
+    acc += sizes[l];
+  }
+  for (size_t l = sizes.size(); l <= MAX_LEVELS; ++l) level_start[l] = (uint32_t)acc;
+  *n_inner = (uint32_t)acc;
+  return (int)sizes.size();
+}
+
+// Rebuild every inner level and the prefix array from the current leaves (device only).
+int rebuild_inner(hm_tree* t, hipStream_t s) {
+  uint32_t ls[MAX_LEVELS + 1];
+  uint32_t n_inner = 0;
+  const int h = plan_levels(t->n_leaves, t->F, ls, &n_inner);
+  if (h < 0) return set_err(HM_EINVAL, "tree would exceed %d levels", MAX_LEVELS);
+  if (n_inner > t->inner_cap) {
+    const uint64_t extra = (uint64_t)(n_inner - t->inner_cap) * (t->NS * 8 + 4);
+    if (t->p.mem_budget && t->device_bytes() + extra > t->p.mem_budget)
+      return set_err(HM_EBUDGET, "inner levels need %llu more bytes than the budget allows",
+                     (unsigned long long)extra);
+    HM_TRY(hipStreamSynchronize(s));
+    if (t->d_inner) HM_TRY(hipFree(t->d_inner));
+    if (t->d_prefix) HM_TRY(hipFree(t->d_prefix));
+    t->d_inner = nullptr;
+    t->d_prefix = nullptr;
+    const uint32_t cap = n_inner + n_inner / 8 + 8;
+    HM_TRY(hipMalloc(&t->d_inner, (size_t)cap * t->NS * 8));
+    HM_TRY(hipMalloc(&t->d_prefix, (size_t)cap * 4));
+    t->inner_cap = cap;
+  }
+  // temporaries: two min-key arrays and the child counts
+  uint64_t *minA = nullptr, *minB = nullptr;
+  uint32_t* nchild = nullptr;
+  void* scan_tmp = nullptr;
+  HM_TRY(hipMalloc(&minA, std::max<uint64_t>(t->n_leaves, 1) * 8));
+  HM_TRY(hipMalloc(&minB, std::max<uint64_t>((t->n_leaves + t->F - 1) / t->F, 1) * 8));
+  HM_TRY(hipMalloc(&nchild, (size_t)n_inner * 4));
+  HM_TRY(hipMalloc(&scan_tmp, scan_tmp_bytes(n_inner)));
+  HM_TRY(leaf_min_keys(t->d_leaves, t->n_leaves, minA, s));
+  uint64_t m_child = t->n_leaves;
+  uint64_t *cmin = minA, *nmin = minB;
+  for (int l = h - 1; l >= 0; --l) {
+    const uint64_t m = ls[l + 1] - ls[l];
+    HM_TRY(build_level(cmin, m_child, t->F, t->d_inner + (size_t)ls[l] * t->NS, nmin, nchild + ls[l], m, s));
+    std::swap(cmin, nmin);
+    m_child = m;
+  }
+  HM_TRY(exclusive_scan_u32(nchild, t->d_prefix, n_inner, 1u, scan_tmp, s));
+  HM_TRY(hipStreamSynchronize(s));
+  HM_TRY(hipFree(minA));
+  HM_TRY(hipFree(minB));
+  HM_TRY(hipFree(nchild));
+  HM_TRY(hipFree(scan_tmp));
+  t->height = h;
+  t->n_inner = n_inner;
+  for (int i = 0; i <= MAX_LEVELS; ++i) t->level_start[i] = ls[i];
+  t->lds_levels = choose_lds_levels(t, t->p.lds_levels);
+  return HM_OK;
+}
+
+void free_tree(hm_tree* t) {
+  if (!t) return;
+  if (t->own) (void)hipStreamSynchronize(t->own);
+  for (auto& kv : t->ws) {
+    if (kv.second.buf) (void)hipFree(kv.second.buf);
+    if (kv.second.last_use) (void)hipEventDestroy(kv.second.last_use);
+  }
+  if (t->d_inner) (void)hipFree(t->d_inner);
+  if (t->d_prefix) (void)hipFree(t->d_prefix);
+  if (t->d_leaves) (void)hipFree(t->d_leaves);
+  if (t->pend_k) (void)hipFree(t->pend_k);
+  if (t->pend_v) (void)hipFree(t->pend_v);
+  if (t->publish_ev) (void)hipEventDestroy(t->publish_ev);
+  if (t->own) (void)hipStreamDestroy(t->own);
+  delete t;
+}
+
+int check_params(const hm_params* p) {
+  if (!p) return set_err(HM_EINVAL, "params is NULL");
+  if (p->fanout != 32 && p->fanout != 64 && p->fanout != 128)
+    return set_err(HM_EINVAL, "fanout %u not in {32, 64, 128}", p->fanout);
+  if (p->leaf_fill < 1 || p->leaf_fill > 8) return set_err(HM_EINVAL, "leaf_fill %u not in 1..8", p->leaf_fill);
+  if (p->lds_levels < -1 || p->lds_levels > 3) return set_err(HM_EINVAL, "lds_levels %d not in -1..3", p->lds_levels);
+  if (p->group_size != 0 && p->group_size != 1 && p->group_size != 2 && p->group_size != 4 && p->group_size != 8)
+    return set_err(HM_EINVAL, "group_size %u not in {0,1,2,4,8}", p->group_size);
+  if (p->psa_bits > 32) return set_err(HM_EINVAL, "psa_bits %u > 32", p->psa_bits);
+  return HM_OK;
+}
+
+constexpr uint64_t kMaxBatch = (1ull << 31) - 1;
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+extern "C" {
+
+int hm_params_init(hm_params* p) {
+  if (!p) return set_err(HM_EINVAL, "params is NULL");
+  std::memset(p, 0, sizeof(*p));
+  p->fanout = 64;
+  p->leaf_fill = 8;
+  p->lds_levels = -1;
+  p->group_size = 0;
+  p->psa_bits = 16;
+  p->sort_threshold = 1ull << 16;
+  p->mem_budget = 0;
+  p->flags = 0;
+  return HM_OK;
+}
+
+int hm_tree_build(const uint64_t* keys, const uint64_t* vals, uint64_t n, const hm_params* p, hm_tree** out) {
+  if (!out) return set_err(HM_EINVAL, "out is NULL");
+  *out = nullptr;
+  int rc = check_params(p);
+  if (rc) return rc;
+  if (n > 0 && (!keys || !vals)) return set_err(HM_EINVAL, "keys/vals is NULL with n = %llu", (unsigned long long)n);
+  if (n > (1ull << 34)) return set_err(HM_EINVAL, "n = %llu too large", (unsigned long long)n);
+
+  hm_tree* t = new hm_tree();
+  t->p = *p;
+  t->F = (int)p->fanout;
+  t->fill = (int)p->leaf_fill;
+  t->NS = node_slots(t->F);
+  t->group = p->group_size ? (int)p->group_size : 4;
+  auto fail = [&](int code) {
+    free_tree(t);
+    return code;
+  };
+  if (hipGetDevice(&t->device) != hipSuccess) return fail(set_err(HM_EDEVICE, "no HIP device"));
+  if (hipStreamCreateWithFlags(&t->own, hipStreamNonBlocking) != hipSuccess)
+    return fail(set_err(HM_EDEVICE, "hipStreamCreate failed"));
+  if (hipEventCreateWithFlags(&t->publish_ev, hipEventDisableTiming) != hipSuccess)
+    return fail(set_err(HM_EDEVICE, "hipEventCreate failed"));
+  hipStream_t s = t->own;
+
+  t->n_keys = n;
+  t->n_leaves = std::max<uint64_t>(1, (n + t->fill - 1) / t->fill);
+  t->leaf_cap = t->n_leaves;
+  {
+    uint32_t ls[MAX_LEVELS + 1], ni = 0;
+    if (plan_levels(t->n_leaves, t->F, ls, &ni) < 0) return fail(set_err(HM_EINVAL, "too many levels"));
+    const uint64_t need = (uint64_t)ni * (t->NS * 8 + 4) + t->leaf_cap * sizeof(Leaf);
+    if (p->mem_budget && need > p->mem_budget)
+      return fail(set_err(HM_EBUDGET, "tree needs %llu device bytes, budget is %llu", (unsigned long long)need,
+                          (unsigned long long)p->mem_budget));
+  }
+
+  // stage input, sort (key, value) pairs, validate
+  uint64_t *k_in = nullptr, *v_in = nullptr, *k0 = nullptr, *v0 = nullptr, *k1 = nullptr, *v1 = nullptr;
+  void* tmp = nullptr;
+  uint32_t* d_flags = nullptr;
+  uint64_t* d_orand = nullptr;
+  auto cleanup = [&]() {
+    (void)hipStreamSynchronize(s);
+    for (void* q : {(void*)k_in, (void*)v_in, (void*)k0, (void*)v0, (void*)k1, (void*)v1, tmp, (void*)d_flags,
+                    (void*)d_orand})
+      if (q) (void)hipFree(q);
+  };
+#define BUILD_TRY(expr)            \
+  do {                             \
+    int r_ = [&]() -> int {        \
+      HM_TRY(expr);                \
+      return HM_OK;                \
+    }();                           \
+    if (r_) {                      \
+      cleanup();                   \
+      return fail(r_);             \
+    }                              \
+  } while (0)
+
+  const size_t nb = std::max<uint64_t>(n, 1) * 8;
+  const uint64_t* dk = keys;
+  const uint64_t* dv = vals;
+  BUILD_TRY(hipMalloc(&d_flags, 256));
+  BUILD_TRY(hipMalloc(&d_orand, 256));
+  BUILD_TRY(hipMemsetAsync(d_flags, 0, 256, s));
+  if (n > 0 && !is_device_ptr(keys)) {
+    BUILD_TRY(hipMalloc(&k_in, nb));
+    BUILD_TRY(hipMemcpyAsync(k_in, keys, n * 8, hipMemcpyHostToDevice, s));
+    dk = k_in;
+  }
+  if (n > 0 && !is_device_ptr(vals)) {
+    BUILD_TRY(hipMalloc(&v_in, nb));
+    BUILD_TRY(hipMemcpyAsync(v_in, vals, n * 8, hipMemcpyHostToDevice, s));
+    dv = v_in;
+  }
+  if (n > 1 && !(p->flags & HM_FLAG_KEYS_SORTED)) {
+    uint64_t orand_init[2] = {0ull, ~0ull}, orand[2];
+    BUILD_TRY(hipMemcpyAsync(d_orand, orand_init, 16, hipMemcpyHostToDevice, s));
+    BUILD_TRY(key_bit_summary(dk, n, d_orand, s));
+    BUILD_TRY(hipMemcpyAsync(orand, d_orand, 16, hipMemcpyDeviceToHost, s));
+    BUILD_TRY(hipStreamSynchronize(s));
+    const uint64_t varying = orand[0] ^ orand[1];
+    if (varying) {
+      const int lo = (__builtin_ctzll(varying) / 8) * 8;
+      const int hi = 64 - __builtin_clzll(varying);
+      BUILD_TRY(hipMalloc(&k0, nb));
+      BUILD_TRY(hipMalloc(&v0, nb));
+      BUILD_TRY(hipMalloc(&k1, nb));
+      BUILD_TRY(hipMalloc(&v1, nb));
+      BUILD_TRY(hipMalloc(&tmp, radix_tmp_bytes(n)));
+      int which = 0;
+      BUILD_TRY(radix_sort_u64pay(dk, dv, n, lo, hi, k0, v0, k1, v1, tmp, s, &which));
+      dk = which ? k1 : k0;
+      dv = which ? v1 : v0;
+    }
+  }
+  uint32_t flags = 0;
+  BUILD_TRY(validate_sorted(dk, n, d_flags, s));
+  BUILD_TRY(hipMemcpyAsync(&flags, d_flags, 4, hipMemcpyDeviceToHost, s));
+  BUILD_TRY(hipStreamSynchronize(s));
+  if (flags & 1u) {
+    cleanup();
+    return fail(set_err(HM_EINVAL, "key 0xFFFFFFFFFFFFFFFF is reserved for padding"));
+  }
+  if (flags & 2u) {
+    cleanup();
+    return fail(set_err(HM_EINVAL, (p->flags & HM_FLAG_KEYS_SORTED) ? "keys flagged sorted are not strictly ascending"
+                                                                     : "duplicate keys in build input"));
+  }
+  BUILD_TRY(hipMalloc(&t->d_leaves, t->leaf_cap * sizeof(Leaf)));
+  BUILD_TRY(fill_leaves(dk, dv, n, t->fill, t->d_leaves, t->n_leaves, s));
+  {
+    int r = rebuild_inner(t, s);
+    if (r) {
+      cleanup();
+      return fail(r);
+    }
+  }
+  cleanup();
+#undef BUILD_TRY
+  *out = t;
+  return HM_OK;
+}
+
+int hm_tree_destroy(hm_tree* t) {
+  if (!t) return set_err(HM_EINVAL, "tree is NULL");
+  free_tree(t);
+  return HM_OK;
+}
+
+int hm_search_ex(hm_tree* t, const uint64_t* q, uint64_t n, uint64_t* out_vals, uint8_t* out_found,
+                 const hm_search_opts* o, void* stream) {
+  if (!t) return set_err(HM_EINVAL, "tree is NULL");
+  if (n == 0) return HM_OK;
+  if (!q || !out_vals || !out_found) return set_err(HM_EINVAL, "NULL buffer");
+  if (n > kMaxBatch) return set_err(HM_EINVAL, "batch of %llu queries exceeds 2^31-1", (unsigned long long)n);
+  hipStream_t s = (hipStream_t)stream;
+  std::lock_guard<std::mutex> g(t->mu);
+
+  SearchLaunch sl;
+  sl.group = (o && o->group_size) ? (int)o->group_size : t->group;
+  if (sl.group != 1 && sl.group != 2 && sl.group != 4 && sl.group != 8)
+    return set_err(HM_EINVAL, "group_size %d not in {1,2,4,8}", sl.group);
+  sl.lds_levels = (o && o->lds_levels >= 0) ? choose_lds_levels(t, o->lds_levels) : t->lds_levels;
+  sl.blocks_per_cu = (o && o->blocks_per_cu) ? (int)o->blocks_per_cu : 8;
+  sl.queries_per_group = (o && o->queries_per_group) ? (int)o->queries_per_group : 2;
+  if (sl.queries_per_group != 1 && sl.queries_per_group != 2 && sl.queries_per_group != 4)
+    return set_err(HM_EINVAL, "queries_per_group %d not in {1,2,4}", sl.queries_per_group);
+  int bits = (o && o->psa_bits >= 0) ? o->psa_bits : (int)t->p.psa_bits;
+  if (bits < 0 || bits > 32) return set_err(HM_EINVAL, "psa_bits %d not in 0..32", bits);
+  const bool psa = bits > 0 && (n >= t->p.sort_threshold || (o && o->psa_bits > 0));
+  const bool sorted_out = o && o->sorted_output;
+  if (sorted_out && (!psa || !o->out_pos)) return set_err(HM_EINVAL, "sorted_output needs PSA on and out_pos");
+
+  const bool hq = !is_device_ptr(q), hv = !is_device_ptr(out_vals), hf = !is_device_ptr(out_found);
+  const bool hp = sorted_out && !is_device_ptr(o->out_pos);
+  size_t need = 0;
+  const size_t kq = align256(n * 8), kp = align256(n * 4);
+  if (hq) need += kq;
+  if (hv) need += kq;
+  if (hf) need += align256(n);
+  if (psa) need += 2 * kq + 2 * kp + align256(radix_tmp_bytes(n));
+  Workspace* w = nullptr;
+  int rc = ensure_ws(t, s, need, &w);
+  if (rc) return rc;
+  if (t->publish_valid) HM_TRY(hipStreamWaitEvent(s, t->publish_ev, 0));
+  char* cur = (char*)w->buf;
+  const uint64_t* dq = q;
+  uint64_t* dvals = out_vals;
+  uint8_t* dfound = out_found;
+  if (hq) {
+    HM_TRY(hipMemcpyAsync(cur, q, n * 8, hipMemcpyHostToDevice, s));
+    dq = (const uint64_t*)cur;
+    cur += kq;
+  }
+  if (hv) {
+    dvals = (uint64_t*)cur;
+    cur += kq;
+  }
+  if (hf) {
+    dfound = (uint8_t*)cur;
+    cur += align256(n);
+  }
+  const TreeView tv = t->view();
+  uint32_t* dpos_res = nullptr;
+  if (psa) {
+    uint64_t* k0 = (uint64_t*)cur;
+    cur += kq;
+    uint64_t* k1 = (uint64_t*)cur;
+    cur += kq;
+    uint32_t* p0 = (uint32_t*)cur;
+    cur += kp;
+    uint32_t* p1 = (uint32_t*)cur;
+    cur += kp;
+    void* tmp = cur;
+    int which = 0;
+    HM_TRY(radix_sort_pos(dq, n, 64 - bits, 64, k0, p0, k1, p1, tmp, s, &which));
+    const uint64_t* sk = which ? k1 : k0;
+    const uint32_t* sp = which ? p1 : p0;
+    HM_TRY(launch_search(tv, sl, sk, sp, dvals, dfound, n, !sorted_out, s));
+    dpos_res = (uint32_t*)sp;
+  } else {
+    HM_TRY(launch_search(tv, sl, dq, nullptr, dvals, dfound, n, false, s));
+  }
+  if (sorted_out) {
+    HM_TRY(hipMemcpyAsync(o->out_pos, dpos_res, n * 4, hp ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, s));
+  }
+  if (hv) HM_TRY(hipMemcpyAsync(out_vals, dvals, n * 8, hipMemcpyDeviceToHost, s));
+  if (hf) HM_TRY(hipMemcpyAsync(out_found, dfound, n, hipMemcpyDeviceToHost, s));
+  HM_TRY(hipEventRecord(w->last_use, s));
+  if (hq || hv || hf || hp) HM_TRY(hipStreamSynchronize(s));
+  return HM_OK;
+}
+
+int hm_search(hm_tree* t, const uint64_t* q, uint64_t n, uint64_t* out_vals, uint8_t* out_found, void* stream) {
+  return hm_search_ex(t, q, n, out_vals, out_found, nullptr, stream);
+}
+
+static int range_common(hm_tree* t, const uint64_t* start, const uint64_t* hi, const uint32_t* len, uint32_t cap_const,
+                        uint64_t n, uint64_t* out_offsets, uint64_t* out_keys, uint64_t* out_vals,
+                        uint32_t* out_counts, uint64_t capacity, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  std::lock_guard<std::mutex> g(t->mu);
+  const bool hs = !is_device_ptr(start), hh = hi && !is_device_ptr(hi), hl = len && !is_device_ptr(len);
+  const bool ho = out_offsets && !is_device_ptr(out_offsets), hk = !is_device_ptr(out_keys),
+             hv = !is_device_ptr(out_vals), hc = !is_device_ptr(out_counts);
+  const size_t k8 = align256(n * 8), k8p = align256((n + 1) * 8), k4 = align256(n * 4);
+  const size_t cap_bytes = align256(capacity * 8);
+  size_t need = (hs ? k8 : 0) + (hh ? k8 : 0) + (hl ? k4 : 0) + (out_offsets ? k8p : 0) + (hk ? cap_bytes : 0) +
+                (hv ? cap_bytes : 0) + (hc ? k4 : 0) + align256(scan_tmp_bytes(n + 1));
+  Workspace* w = nullptr;
+  int rc = ensure_ws(t, s, need, &w);
+  if (rc) return rc;
+  if (t->publish_valid) HM_TRY(hipStreamWaitEvent(s, t->publish_ev, 0));
+  char* cur = (char*)w->buf;
+  auto take = [&](size_t b) {
+    char* r = cur;
+    cur += b;
+    return r;
+  };
+  const uint64_t* ds = start;
+  const uint64_t* dh = hi;
+  const uint32_t* dl = len;
+  if (hs) {
+    uint64_t* b = (uint64_t*)take(k8);
+    HM_TRY(hipMemcpyAsync(b, start, n * 8, hipMemcpyHostToDevice, s));
+    ds = b;
+  }
+  if (hh) {
+    uint64_t* b = (uint64_t*)take(k8);
+    HM_TRY(hipMemcpyAsync(b, hi, n * 8, hipMemcpyHostToDevice, s));
+    dh = b;
+  }
+  if (hl) {
+    uint32_t* b = (uint32_t*)take(k4);
+    HM_TRY(hipMemcpyAsync(b, len, n * 4, hipMemcpyHostToDevice, s));
+    dl = b;
+  }
+  uint64_t* doff = nullptr;
+  if (out_offsets) {
+    doff = ho ? (uint64_t*)take(k8p) : out_offsets;
+    void* scan_tmp = take(align256(scan_tmp_bytes(n + 1)));
+    HM_TRY(exclusive_scan_u32_to_u64(dl, doff, n, scan_tmp, s));
+    HM_TRY(range_offsets_tail(dl, doff, n, s));
+  }
+  uint64_t* dk = hk ? (uint64_t*)take(cap_bytes) : out_keys;
+  uint64_t* dv = hv ? (uint64_t*)take(cap_bytes) : out_vals;
+  uint32_t* dc = hc ? (uint32_t*)take(k4) : out_counts;
+  HM_TRY(launch_range(t->view(), ds, dh, dl, cap_const, n, doff, cap_const, dk, dv, dc, capacity, s));
+  if (ho) HM_TRY(hipMemcpyAsync(out_offsets, doff, (n + 1) * 8, hipMemcpyDeviceToHost, s));
+  if (hk) HM_TRY(hipMemcpyAsync(out_keys, dk, capacity * 8, hipMemcpyDeviceToHost, s));
+  if (hv) HM_TRY(hipMemcpyAsync(out_vals, dv, capacity * 8, hipMemcpyDeviceToHost, s));
+  if (hc) HM_TRY(hipMemcpyAsync(out_counts, dc, n * 4, hipMemcpyDeviceToHost, s));
+  HM_TRY(hipEventRecord(w->last_use, s));
+  if (hs || hh || hl || ho || hk || hv || hc) HM_TRY(hipStreamSynchronize(s));
+  return HM_OK;
+}
+
+int hm_range(hm_tree* t, const uint64_t* start, const uint32_t* len, uint64_t n, uint64_t* out_offsets,
+             uint64_t* out_keys, uint64_t* out_vals, uint32_t* out_counts, uint64_t capacity, void* stream) {
+  if (!t) return set_err(HM_EINVAL, "tree is NULL");
+  if (n == 0) {
+    if (out_offsets) {
+      uint64_t z = 0;
+      if (is_device_ptr(out_offsets))
+        HM_TRY(hipMemcpy(out_offsets, &z, 8, hipMemcpyHostToDevice));
+      else
+        out_offsets[0] = 0;
+    }
+    return HM_OK;
+  }
+  if (!start || !len || !out_offsets || !out_keys || !out_vals || !out_counts)
+    return set_err(HM_EINVAL, "NULL buffer");
+  if (n > kMaxBatch) return set_err(HM_EINVAL, "too many ranges");
+  return range_common(t, start, nullptr, len, 0, n, out_offsets, out_keys, out_vals, out_counts, capacity, stream);
+}
+
+int hm_range_between(hm_tree* t, const uint64_t* lo, const uint64_t* hi, uint64_t n, uint32_t cap, uint64_t* out_keys,
+                     uint64_t* out_vals, uint32_t* out_counts, void* stream) {
+  if (!t) return set_err(HM_EINVAL, "tree is NULL");
+  if (n == 0) return HM_OK;
+  if (!lo || !hi || !out_keys || !out_vals || !out_counts) return set_err(HM_EINVAL, "NULL buffer");
+  if (cap == 0) return set_err(HM_EINVAL, "cap must be > 0");
+  if (n > kMaxBatch) return set_err(HM_EINVAL, "too many ranges");
+  return range_common(t, lo, hi, nullptr, cap, n, nullptr, out_keys, out_vals, out_counts, n * (uint64_t)cap, stream);
+}
+
+int hm_update(hm_tree* t, const uint64_t* keys, const uint64_t* vals, uint64_t n, void* stream) {
+  if (!t) return set_err(HM_EINVAL, "tree is NULL");
+  if (n == 0) return HM_OK;
+  if (!keys || !vals) return set_err(HM_EINVAL, "NULL buffer");
+  if (n > kMaxBatch) return set_err(HM_EINVAL, "update batch too large");
+  hipStream_t s = (hipStream_t)stream;
+  std::lock_guard<std::mutex> g(t->mu);
+  if (t->pend_n + n > kMaxBatch) return set_err(HM_EINVAL, "too many staged updates; publish first");
+  if (t->pend_n + n > t->pend_cap) {
+    size_t cap = std::max<size_t>(t->pend_n + n, 2 * t->pend_cap);
+    if (cap < 4096) cap = 4096;
+    if (t->p.mem_budget && t->device_bytes() + (cap - t->pend_cap) * 16 > t->p.mem_budget)
+      return set_err(HM_EBUDGET, "staging %llu updates exceeds the device-memory budget", (unsigned long long)n);
+    uint64_t *nk = nullptr, *nv = nullptr;
+    HM_TRY(hipMalloc(&nk, cap * 8));
+    HM_TRY(hipMalloc(&nv, cap * 8));
+    if (t->pend_n) {
+      HM_TRY(hipMemcpyAsync(nk, t->pend_k, t->pend_n * 8, hipMemcpyDeviceToDevice, s));
+      HM_TRY(hipMemcpyAsync(nv, t->pend_v, t->pend_n * 8, hipMemcpyDeviceToDevice, s));
+    }
+    HM_TRY(hipStreamSynchronize(s));
+    if (t->pend_k) HM_TRY(hipFree(t->pend_k));
+    if (t->pend_v) HM_TRY(hipFree(t->pend_v));
+    t->pend_k = nk;
+    t->pend_v = nv;
+    t->pend_cap = cap;
+  }
+  const hipMemcpyKind kk = is_device_ptr(keys) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  const hipMemcpyKind kv = is_device_ptr(vals) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  HM_TRY(hipMemcpyAsync(t->pend_k + t->pend_n, keys, n * 8, kk, s));
+  HM_TRY(hipMemcpyAsync(t->pend_v + t->pend_n, vals, n * 8, kv, s));
+  // reject the reserved key before anything is staged
+  uint32_t* d_flags = nullptr;
+  uint32_t flags = 0;
+  HM_TRY(hipMalloc(&d_flags, 4));
+  HM_TRY(hipMemsetAsync(d_flags, 0, 4, s));
+  // validate_sorted flags bit 1 for KEY_MAX; bit 2 (order) is ignored here
+  HM_TRY(validate_sorted(t->pend_k + t->pend_n, n, d_flags, s));
+  HM_TRY(hipMemcpyAsync(&flags, d_flags, 4, hipMemcpyDeviceToHost, s));
+  HM_TRY(hipStreamSynchronize(s));
+  HM_TRY(hipFree(d_flags));
+  if (flags & 1u) return set_err(HM_EINVAL, "key 0xFFFFFFFFFFFFFFFF is reserved for padding");
+  t->pend_n += n;
+  return HM_OK;
+}
+
+static int apply_pending(hm_tree* t, hipStream_t s, int* rebuilt) {
+  const size_t n = t->pend_n;
+  if (!n) return HM_OK;
+  // sort the staged batch by key (stable: a later write of a key follows the earlier one)
+  std::vector<void*> tmps;
+  auto cleanup = [&]() {
+    (void)hipStreamSynchronize(s);
+    for (void* p : tmps) (void)hipFree(p);
+  };
+  auto alloc = [&](size_t b) -> void* {
+    void* p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(b, 256)) != hipSuccess) return nullptr;
+    tmps.push_back(p);
+    return p;
+  };
+#define UPD_TRY(expr)                                                                                            \
+  do {                                                                                                           \
+    hipError_t e_ = (expr);                                                                                      \
+    if (e_ != hipSuccess) {                                                                                      \
+      cleanup();                                                                                                 \
+      return set_err(e_ == hipErrorOutOfMemory ? HM_ENOMEM : HM_EDEVICE, "%s failed: %s", #expr,                  \
+                     hipGetErrorString(e_));                                                                     \
+    }                                                                                                            \
+  } while (0)
+#define UPD_ALLOC(var, type, bytes)                                  \
+  type var = (type)alloc(bytes);                                     \
+  if (!var) {                                                        \
+    cleanup();                                                       \
+    return set_err(HM_ENOMEM, "update scratch allocation failed");   \
+  }
+  UPD_ALLOC(k0, uint64_t*, n * 8);
+  UPD_ALLOC(v0, uint64_t*, n * 8);
+  UPD_ALLOC(k1, uint64_t*, n * 8);
+  UPD_ALLOC(v1, uint64_t*, n * 8);
+  UPD_ALLOC(rtmp, void*, radix_tmp_bytes(n));
+  int which = 0;
+  UPD_TRY(radix_sort_u64pay(t->pend_k, t->pend_v, n, 0, 64, k0, v0, k1, v1, rtmp, s, &which));
+  uint64_t* sk = which ? k1 : k0;
+  uint64_t* sv = which ? v1 : v0;
+  UPD_ALLOC(keep, uint32_t*, n * 4);
+  UPD_ALLOC(keep_scan, uint32_t*, (n + 1) * 4);
+  UPD_ALLOC(stmp, void*, scan_tmp_bytes(n + 1));
+  UPD_TRY(mark_last_of_runs(sk, n, keep, s));
+  UPD_TRY(exclusive_scan_u32(keep, keep_scan, n, 0u, stmp, s));
+  uint64_t* uk = which ? k0 : k1;  // the other ping-pong buffers are free now
+  uint64_t* uv = which ? v0 : v1;
+  UPD_TRY(compact_pairs(sk, sv, keep, keep_scan, n, uk, uv, s));
+  uint32_t last_scan = 0, last_keep = 0;
+  UPD_TRY(hipMemcpyAsync(&last_scan, keep_scan + n - 1, 4, hipMemcpyDeviceToHost, s));
+  UPD_TRY(hipMemcpyAsync(&last_keep, keep + n - 1, 4, hipMemcpyDeviceToHost, s));
+  UPD_TRY(hipStreamSynchronize(s));
+  const size_t m = last_scan + last_keep;
+
+  // locate every unique key
+  UPD_ALLOC(leaf, uint32_t*, m * 4);
+  UPD_ALLOC(slot, uint32_t*, m * 4);
+  UPD_ALLOC(found, uint8_t*, m);
+  const TreeView tv = t->view();
+  UPD_TRY(launch_locate(tv, uk, m, leaf, slot, found, s));
+  // inserts per leaf and the first batch index of each leaf's inserts
+  UPD_ALLOC(ins, uint32_t*, t->n_leaves * 4);
+  UPD_ALLOC(istart, uint32_t*, t->n_leaves * 4);
+  UPD_ALLOC(flags, uint32_t*, 256);
+  UPD_TRY(hipMemsetAsync(ins, 0, t->n_leaves * 4, s));
+  UPD_TRY(hipMemsetAsync(istart, 0xFF, t->n_leaves * 4, s));
+  UPD_TRY(hipMemsetAsync(flags, 0, 256, s));
+  UPD_TRY(count_inserts(leaf, found, m, ins, s));
+  UPD_TRY(insert_starts(leaf, found, m, istart, s));
+  UPD_TRY(overflow_check(t->d_leaves, ins, t->n_leaves, flags, s));
+  // number of inserted keys = m - found count; computed via a scan of found flags on the host side
+  std::vector<uint8_t> hfound(m);
+  uint32_t hflags = 0;
+  UPD_TRY(hipMemcpyAsync(hfound.data(), found, m, hipMemcpyDeviceToHost, s));
+  UPD_TRY(hipMemcpyAsync(&hflags, flags, 4, hipMemcpyDeviceToHost, s));
+  UPD_TRY(hipStreamSynchronize(s));
+  size_t n_new = 0;
+  for (uint8_t f : hfound) n_new += f ? 0 : 1;
+
+  if (!(hflags & 1u)) {
+    UPD_TRY(apply_value_updates(t->d_leaves, leaf, slot, found, uv, m, s));
+    if (n_new) UPD_TRY(merge_in_place(t->d_leaves, ins, istart, uk, uv, found, t->n_leaves, s));
+    UPD_TRY(hipStreamSynchronize(s));
+    t->n_keys += n_new;
+    if (rebuilt) *rebuilt = 0;
+    cleanup();
+    return HM_OK;
+  }
+  // overflow: re-lay out all leaves, then rebuild the inner levels and prefix array
+  UPD_ALLOC(cnt, uint32_t*, t->n_leaves * 4);
+  UPD_ALLOC(nstart, uint32_t*, (t->n_leaves + 1) * 4);
+  UPD_ALLOC(ntmp, void*, scan_tmp_bytes(t->n_leaves + 1));
+  UPD_TRY(new_leaf_counts(t->d_leaves, ins, t->n_leaves, t->fill, cnt, s));
+  UPD_TRY(exclusive_scan_u32(cnt, nstart, t->n_leaves, 0u, ntmp, s));
+  uint32_t ls = 0, lc = 0;
+  UPD_TRY(hipMemcpyAsync(&ls, nstart + t->n_leaves - 1, 4, hipMemcpyDeviceToHost, s));
+  UPD_TRY(hipMemcpyAsync(&lc, cnt + t->n_leaves - 1, 4, hipMemcpyDeviceToHost, s));
+  UPD_TRY(hipStreamSynchronize(s));
+  const uint64_t new_leaves = (uint64_t)ls + lc;
+  if (new_leaves >= (1ull << 32)) {
+    cleanup();
+    return set_err(HM_EINVAL, "leaf count would exceed 2^32");
+  }
+  uint64_t new_cap = new_leaves + new_leaves / 16 + 64;
+  {
+    uint32_t lsx[MAX_LEVELS + 1], ni = 0;
+    if (plan_levels(new_leaves, t->F, lsx, &ni) < 0) {
+      cleanup();
+      return set_err(HM_EINVAL, "tree would exceed %d levels", MAX_LEVELS);
+    }
+    const uint64_t inner_more = ni > t->inner_cap ? (uint64_t)(ni + ni / 8 + 8 - t->inner_cap) * (t->NS * 8 + 4) : 0;
+    // old and new leaves coexist while relaying out
+    if (t->p.mem_budget && t->device_bytes() + new_cap * sizeof(Leaf) + inner_more > t->p.mem_budget) {
+      cleanup();
+      return set_err(HM_EBUDGET, "re-layout to %llu leaves exceeds the device-memory budget",
+                     (unsigned long long)new_leaves);
+    }
+  }
+  Leaf* nl = nullptr;
+  UPD_TRY(hipMalloc(&nl, new_cap * sizeof(Leaf)));
+  UPD_TRY(apply_value_updates(t->d_leaves, leaf, slot, found, uv, m, s));
+  UPD_TRY(relayout_leaves(t->d_leaves, t->n_leaves, ins, istart, uk, uv, found, nstart, t->fill, nl, s));
+  UPD_TRY(hipStreamSynchronize(s));
+  UPD_TRY(hipFree(t->d_leaves));
+  t->d_leaves = nl;
+  t->n_leaves = new_leaves;
+  t->leaf_cap = new_cap;
+  t->n_keys += n_new;
+  cleanup();
+  int r = rebuild_inner(t, s);
+  if (r) return r;
+  if (rebuilt) *rebuilt = 1;
+  return HM_OK;
+#undef UPD_TRY
+#undef UPD_ALLOC
+}
+
+int hm_epoch_publish(hm_tree* t, void* stream, int* out_rebuilt) {
+  if (!t) return set_err(HM_EINVAL, "tree is NULL");
+  if (out_rebuilt) *out_rebuilt = 0;
+  hipStream_t s = (hipStream_t)stream;
+  std::lock_guard<std::mutex> g(t->mu);
+  for (auto& kv : t->ws)
+    if (kv.second.last_use) HM_TRY(hipStreamWaitEvent(s, kv.second.last_use, 0));
+  if (t->publish_valid) HM_TRY(hipStreamWaitEvent(s, t->publish_ev, 0));
+  int rc = apply_pending(t, s, out_rebuilt);
+  if (rc) return rc;
+  t->pend_n = 0;
+  HM_TRY(hipEventRecord(t->publish_ev, s));
+  t->publish_valid = true;
+  t->epoch++;
+  return HM_OK;
+}
+
+int hm_tree_stats(const hm_tree* t, hm_stats* st) {
+  if (!t || !st) return set_err(HM_EINVAL, "NULL argument");
+  std::memset(st, 0, sizeof(*st));
+  st->n_keys = t->n_keys;
+  st->n_leaves = t->n_leaves;
+  st->n_inner = t->n_inner;
+  st->height = (uint32_t)t->height;
+  st->fanout = (uint32_t)t->F;
+  st->node_slots = (uint32_t)t->NS;
+  st->lds_levels = (uint32_t)t->lds_levels;
+  for (int i = 0; i <= MAX_LEVELS; ++i) st->level_start[i] = t->level_start[i];
+  st->bytes_inner = (uint64_t)t->n_inner * t->NS * 8;
+  st->bytes_prefix = (uint64_t)t->n_inner * 4;
+  st->bytes_leaves = t->n_leaves * sizeof(Leaf);
+  st->bytes_device = t->device_bytes();
+  st->epoch = t->epoch;
+  return HM_OK;
+}
+
+int hm_tree_export(const hm_tree* t, uint64_t* inner, uint32_t* prefix, uint64_t* leaves) {
+  if (!t) return set_err(HM_EINVAL, "tree is NULL");
+  hm_tree* mt = const_cast<hm_tree*>(t);
+  std::lock_guard<std::mutex> g(mt->mu);
+  HM_TRY(hipStreamSynchronize(t->own));
+  HM_TRY(hipDeviceSynchronize());
+  if (inner) HM_TRY(hipMemcpy(inner, t->d_inner, (size_t)t->n_inner * t->NS * 8, hipMemcpyDeviceToHost));
+  if (prefix) HM_TRY(hipMemcpy(prefix, t->d_prefix, (size_t)t->n_inner * 4, hipMemcpyDeviceToHost));
+  if (leaves) HM_TRY(hipMemcpy(leaves, t->d_leaves, t->n_leaves * sizeof(Leaf), hipMemcpyDeviceToHost));
+  return HM_OK;
+}
+
+int hm_psa_sort(const uint64_t* keys, uint64_t n, int begin_bit, int end_bit, uint64_t* keys_out, uint32_t* pos_out,
+                void* stream) {
+  if (n == 0) return HM_OK;
+  if (!keys || !keys_out || !pos_out) return set_err(HM_EINVAL, "NULL buffer");
+  if (begin_bit < 0 || end_bit > 64 || begin_bit > end_bit) return set_err(HM_EINVAL, "bad bit range");
+  if (n > kMaxBatch) return set_err(HM_EINVAL, "too many keys");
+  hipStream_t s = (hipStream_t)stream;
+  const size_t kq = align256(n * 8), kp = align256(n * 4);
+  char* buf = nullptr;
+  HM_TRY(hipMalloc(&buf, 3 * kq + 2 * kp + align256(radix_tmp_bytes(n))));
+  char* cur = buf;
+  const uint64_t* dk = keys;
+  if (!is_device_ptr(keys)) {
+    HM_TRY(hipMemcpyAsync(cur, keys, n * 8, hipMemcpyHostToDevice, s));
+    dk = (const uint64_t*)cur;
+  }
+  cur += kq;
+  uint64_t* k0 = (uint64_t*)cur;
+  cur += kq;
+  uint64_t* k1 = (uint64_t*)cur;
+  cur += kq;
+  uint32_t* p0 = (uint32_t*)cur;
+  cur += kp;
+  uint32_t* p1 = (uint32_t*)cur;
+  cur += kp;
+  int which = 0;
+  HM_TRY(radix_sort_pos(dk, n, begin_bit, end_bit, k0, p0, k1, p1, cur, s, &which));
+  const bool dko = is_device_ptr(keys_out), dpo = is_device_ptr(pos_out);
+  HM_TRY(hipMemcpyAsync(keys_out, which ? k1 : k0, n * 8, dko ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+  HM_TRY(hipMemcpyAsync(pos_out, which ? p1 : p0, n * 4, dpo ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+  HM_TRY(hipStreamSynchronize(s));
+  HM_TRY(hipFree(buf));
+  return HM_OK;
+}
+
+int hm_profile_enable(int on) {
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  g_prof_on = on != 0;
+  return HM_OK;
+}
+
+int hm_profile_reset(void) {
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  for (auto& r : g_recs) {
+    (void)hipEventSynchronize(r.b);
+    g_ev_pool.push_back(r.a);
+    g_ev_pool.push_back(r.b);
+  }
+  g_recs.clear();
+  return HM_OK;
+}
+
+int hm_profile_get(const char* stage, double* total_ms, uint64_t* launches) {
+  if (!stage || !total_ms || !launches) return set_err(HM_EINVAL, "NULL argument");
+  std::lock_guard<std::mutex> g(g_prof_mu);
+  double tot = 0;
+  uint64_t cnt = 0;
+  for (auto& r : g_recs) {
+    if (g_stage_names[r.stage] != stage) continue;
+    HM_TRY(hipEventSynchronize(r.b));
+    float ms = 0;
+    HM_TRY(hipEventElapsedTime(&ms, r.a, r.b));
+    tot += ms;
+    ++cnt;
+  }
+  *total_ms = tot;
+  *launches = cnt;
+  return HM_OK;
+}
+
+int hm_device_synchronize(void) {
+  HM_TRY(hipDeviceSynchronize());
+  return HM_OK;
+}
+
+int hm_last_error(char* buf, uint64_t len) {
+  if (!buf || !len) return HM_EINVAL;
+  std::snprintf(buf, (size_t)len, "%s", g_err.c_str());
+  return HM_OK;
+}
+
+}  // extern "C"

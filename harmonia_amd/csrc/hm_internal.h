@@ -1,0 +1,140 @@
+// Internal declarations shared by the HIP translation units of libharmonia_amd.
+//
+// Layout (all device memory, built on the device):
+//   inner   : n_inner nodes in breadth-first order, node_slots(F) u64 each.
+//             slots [0, IS)      index granule: idx[j] = key[8j+7] for j < F/8, else KEY_MAX
+//             slots [IS, IS+F)   separators key[0..F-2] ascending, KEY_MAX padding; key[F-1] = KEY_MAX
+//   prefix  : n_inner u32, prefix[g] = BFS index of the first child of node g
+//             (= 1 + exclusive scan of child counts, H/regular_bpt/traversal.cpp:40,58).
+//             BFS indices >= n_inner name leaves: leaf = index - n_inner.
+//   leaves  : 128-byte records {key[8], val[8]}, keys ascending then KEY_MAX padding.
+// Child choice is upper_bound (number of separators <= target), leaf match is exact equality
+// (H/regular_bpt/prefix-sort.cu:147-173, 202-228; H/regular_bpt/regular_bpt.cpp:346-393).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+
+namespace hm {
+
+constexpr uint64_t KEY_MAX = ~0ull;
+constexpr int LEAF_KEYS = 8;
+constexpr int MAX_LEVELS = 16;
+
+struct __attribute__((aligned(128))) Leaf {
+  uint64_t key[8];
+  uint64_t val[8];
+};
+static_assert(sizeof(Leaf) == 128, "leaf record must be 128 bytes");
+
+__host__ __device__ constexpr int idx_slots(int F) { return F / 8 < 8 ? 8 : F / 8; }
+__host__ __device__ constexpr int node_slots(int F) { return idx_slots(F) + F; }
+
+// Everything a search kernel needs, passed by value.
+struct TreeView {
+  const uint64_t* inner;
+  const uint32_t* prefix;
+  const Leaf* leaves;
+  uint32_t n_inner;
+  uint32_t n_leaves;
+  int32_t height;  // inner levels, >= 1
+  int32_t fanout;
+  uint32_t level_start[MAX_LEVELS + 1];
+};
+
+// ---- launch helpers (host) -------------------------------------------------------------
+struct LaunchCfg {
+  int num_cus = 256;
+  int max_lds = 160 * 1024;
+};
+const LaunchCfg& launch_cfg();
+
+// Stage timing (HIP events on the launch stream); no-ops unless profiling is enabled.
+struct ProfScope {
+  ProfScope(const char* stage, hipStream_t s);
+  ~ProfScope();
+  int slot;
+  hipStream_t stream;
+};
+
+// ---- scan (hm_scan.hip) ----------------------------------------------------------------
+// Exclusive scan of n u32 / u64 values; out may alias in.  tmp must hold scan_tmp_bytes(n).
+size_t scan_tmp_bytes(size_t n);
+hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t init, void* tmp,
+                              hipStream_t s);
+hipError_t exclusive_scan_u64(const uint64_t* in, uint64_t* out, size_t n, uint64_t init, void* tmp,
+                              hipStream_t s);
+// u32 input widened into a u64 exclusive scan (range offsets).
+hipError_t exclusive_scan_u32_to_u64(const uint32_t* in, uint64_t* out, size_t n, void* tmp, hipStream_t s);
+
+// ---- radix sort (hm_sort.hip) ----------------------------------------------------------
+// Stable LSD radix sort of u64 keys with a payload over bits [begin_bit, end_bit).
+// Ping-pong between (k0,v0) and (k1,v1); returns which buffer (0/1) holds the result in *which.
+size_t radix_tmp_bytes(size_t n);
+// payload = position 0..n-1 generated on the first pass (vals_in unused)
+hipError_t radix_sort_pos(const uint64_t* keys_in, size_t n, int begin_bit, int end_bit, uint64_t* k0, uint32_t* v0,
+                          uint64_t* k1, uint32_t* v1, void* tmp, hipStream_t s, int* which);
+hipError_t radix_sort_u64pay(const uint64_t* keys_in, const uint64_t* vals_in, size_t n, int begin_bit, int end_bit,
+                             uint64_t* k0, uint64_t* v0, uint64_t* k1, uint64_t* v1, void* tmp, hipStream_t s,
+                             int* which);
+
+// ---- build (hm_build.hip) --------------------------------------------------------------
+// flags |= 1 if a key is KEY_MAX, |= 2 if keys are not strictly ascending.
+hipError_t validate_sorted(const uint64_t* keys, size_t n, uint32_t* d_flags, hipStream_t s);
+// OR / AND over keys (to skip radix passes whose digit is constant).
+hipError_t key_bit_summary(const uint64_t* keys, size_t n, uint64_t* d_orand /*2 u64*/, hipStream_t s);
+hipError_t fill_leaves(const uint64_t* keys, const uint64_t* vals, size_t n, int fill, Leaf* leaves, size_t n_leaves,
+                       hipStream_t s);
+// Leaf minimum keys (KEY_MAX for an empty leaf) -> child_min.
+hipError_t leaf_min_keys(const Leaf* leaves, size_t n_leaves, uint64_t* child_min, hipStream_t s);
+// One inner level: m nodes over m_child children whose minimum keys are child_min.
+hipError_t build_level(const uint64_t* child_min, size_t m_child, int F, uint64_t* nodes, uint64_t* node_min,
+                       uint32_t* nchild, size_t m, hipStream_t s);
+
+// ---- search (hm_search.hip) ------------------------------------------------------------
+struct SearchLaunch {
+  int group = 4;        // lanes per query
+  int lds_levels = 2;   // levels staged in LDS
+  int blocks_per_cu = 8;
+  int queries_per_group = 2;  // independent descents interleaved per group
+};
+hipError_t launch_search(const TreeView& tv, const SearchLaunch& sl, const uint64_t* q, const uint32_t* pos,
+                         uint64_t* out_vals, uint8_t* out_found, size_t n, bool scatter, hipStream_t s);
+// Bytes of LDS the top `levels` need.
+size_t lds_bytes_for(const TreeView& tv, int levels);
+// Locate (leaf, slot, found) per key (update path).
+hipError_t launch_locate(const TreeView& tv, const uint64_t* keys, size_t n, uint32_t* leaf_out, uint32_t* slot_out,
+                         uint8_t* found_out, hipStream_t s);
+
+// ---- range (hm_range.hip) --------------------------------------------------------------
+hipError_t launch_range(const TreeView& tv, const uint64_t* start, const uint64_t* hi, const uint32_t* len,
+                        uint32_t cap_const, size_t n, const uint64_t* offsets, uint64_t offset_stride,
+                        uint64_t* out_keys, uint64_t* out_vals, uint32_t* out_counts, uint64_t capacity, hipStream_t s);
+
+// ---- update (hm_update.hip) ------------------------------------------------------------
+// Marks run ends of a sorted key array (keep the last of equal keys): keep[i] = 1 if last.
+hipError_t mark_last_of_runs(const uint64_t* sorted_keys, size_t n, uint32_t* keep, hipStream_t s);
+hipError_t compact_pairs(const uint64_t* k, const uint64_t* v, const uint32_t* keep, const uint32_t* keep_scan,
+                         size_t n, uint64_t* ko, uint64_t* vo, hipStream_t s);
+hipError_t apply_value_updates(Leaf* leaves, const uint32_t* leaf, const uint32_t* slot, const uint8_t* found,
+                               const uint64_t* vals, size_t n, hipStream_t s);
+// Per-leaf insert counts for the not-found keys (keys sorted => leaf ids nondecreasing).
+hipError_t count_inserts(const uint32_t* leaf, const uint8_t* found, size_t n, uint32_t* ins_per_leaf,
+                         hipStream_t s);
+// ins_start[leaf] = first index (into the sorted batch) of the leaf's inserts (valid where count > 0).
+hipError_t insert_starts(const uint32_t* leaf, const uint8_t* found, size_t n, uint32_t* ins_start, hipStream_t s);
+// In place: merge inserts into leaves that stay within 8 keys. flags |= 1 if any leaf overflows.
+hipError_t overflow_check(const Leaf* leaves, const uint32_t* ins_per_leaf, size_t n_leaves, uint32_t* d_flags,
+                          hipStream_t s);
+hipError_t merge_in_place(Leaf* leaves, const uint32_t* ins_per_leaf, const uint32_t* ins_start,
+                          const uint64_t* bkeys, const uint64_t* bvals, const uint8_t* found, size_t n_leaves,
+                          hipStream_t s);
+// Out of place: new leaf count per old leaf (1 if it fits, else ceil(total / fill)).
+hipError_t new_leaf_counts(const Leaf* leaves, const uint32_t* ins_per_leaf, size_t n_leaves, int fill,
+                           uint32_t* cnt, hipStream_t s);
+hipError_t relayout_leaves(const Leaf* old_leaves, size_t n_old, const uint32_t* ins_per_leaf,
+                           const uint32_t* ins_start, const uint64_t* bkeys, const uint64_t* bvals,
+                           const uint8_t* found, const uint32_t* new_start, int fill, Leaf* new_leaves,
+                           hipStream_t s);
+
+}  // namespace hm

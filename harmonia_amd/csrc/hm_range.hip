@@ -1,0 +1,105 @@
+// Leaf-chain range scan.
+//
+// Replaces range_prefix_sort_1thread_search_1_thread_scan_kernel (H/src/range.cu:101-227): descend
+// to the leaf of `start`, then walk the leaves in key order (the leaf pool is contiguous in key
+// order, leaf i -> i + 1, as the reference's BLeaf pool, H/src/range.cu:188-221), skipping padding.
+// The reference emits at most Range_Size = 8 records per range (H/src/range.cu:32); here a range
+// emits up to its own length, or every key of a closed interval [lo, hi] as
+// find_range(start, end) does (H/regular_bpt/regular_bpt.cpp:320-338).
+// One group of 8 lanes per range: lane s owns slot s of the current leaf, and a ballot over the
+// group packs the qualifying keys into the output in order.
+#include "hm_device.h"
+
+namespace hm {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int G = 8;
+
+template <int F>
+__global__ __launch_bounds__(kThreads) void range_kernel(TreeView tv, const uint64_t* __restrict__ start,
+                                                         const uint64_t* __restrict__ hi,
+                                                         const uint32_t* __restrict__ len, uint32_t cap_const,
+                                                         uint32_t n, const uint64_t* __restrict__ offsets,
+                                                         uint64_t offset_stride, uint64_t* __restrict__ out_keys,
+                                                         uint64_t* __restrict__ out_vals,
+                                                         uint32_t* __restrict__ out_counts, uint64_t capacity) {
+  const int gl = threadIdx.x % G;
+  const int gbase = (threadIdx.x & 63) - gl;
+  const uint32_t gid = threadIdx.x / G;
+  constexpr uint32_t gpb = kThreads / G;
+  const uint64_t lanes_below = (1ull << gl) - 1ull;
+  for (uint32_t rb = blockIdx.x * gpb; rb < n; rb += gridDim.x * gpb) {  // uniform across the block
+    const uint32_t ri = rb + gid;
+    const bool valid = ri < n;
+    const uint64_t s = valid ? start[ri] : 0ull;
+    const uint64_t h = (valid && hi) ? hi[ri] : KEY_MAX;
+    const uint32_t L = valid ? (len ? len[ri] : cap_const) : 0u;
+    const uint64_t off = valid ? (offsets ? offsets[ri] : (uint64_t)ri * offset_stride) : 0ull;
+    const uint64_t sd = s == KEY_MAX ? KEY_MAX - 1 : s;
+    uint32_t leaf = descend<F, G>(tv, 0, nullptr, nullptr, sd, gl, gbase);
+    uint32_t got = 0;
+    bool done = !valid || L == 0 || s > h || s == KEY_MAX;
+    while (!done) {  // uniform inside the group
+      const uint64_t k = tv.leaves[leaf].key[gl];
+      const uint64_t v = tv.leaves[leaf].val[gl];
+      const bool take = k != KEY_MAX && k >= s && k <= h;
+      const uint64_t m = (__ballot(take) >> gbase) & 0xFFull;
+      const uint64_t beyond = (__ballot(k != KEY_MAX && k > h) >> gbase) & 0xFFull;
+      const uint32_t rank = (uint32_t)__popcll(m & lanes_below);
+      if (take && got + rank < L) {
+        const uint64_t o = off + got + rank;
+        if (o < capacity) {
+          out_keys[o] = k;
+          out_vals[o] = v;
+        }
+      }
+      got += (uint32_t)__popcll(m);
+      ++leaf;
+      done = got >= L || beyond != 0 || leaf >= tv.n_leaves;
+    }
+    if (valid && gl == 0) out_counts[ri] = got < L ? got : L;
+  }
+}
+
+__global__ void offsets_tail_kernel(const uint32_t* len, uint64_t* offsets, uint32_t n) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) offsets[n] = n ? offsets[n - 1] + len[n - 1] : 0ull;
+}
+
+}  // namespace
+
+hipError_t launch_range(const TreeView& tv, const uint64_t* start, const uint64_t* hi, const uint32_t* len,
+                        uint32_t cap_const, size_t n, const uint64_t* offsets, uint64_t offset_stride,
+                        uint64_t* out_keys, uint64_t* out_vals, uint32_t* out_counts, uint64_t capacity,
+                        hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  ProfScope prof("range", s);
+  constexpr uint32_t gpb = kThreads / G;
+  size_t nb = (n + gpb - 1) / gpb;
+  const size_t max_nb = (size_t)launch_cfg().num_cus * 8;
+  if (nb > max_nb) nb = max_nb;
+  switch (tv.fanout) {
+    case 32:
+      hipLaunchKernelGGL(range_kernel<32>, dim3((unsigned)nb), dim3(kThreads), 0, s, tv, start, hi, len, cap_const,
+                         (uint32_t)n, offsets, offset_stride, out_keys, out_vals, out_counts, capacity);
+      break;
+    case 64:
+      hipLaunchKernelGGL(range_kernel<64>, dim3((unsigned)nb), dim3(kThreads), 0, s, tv, start, hi, len, cap_const,
+                         (uint32_t)n, offsets, offset_stride, out_keys, out_vals, out_counts, capacity);
+      break;
+    case 128:
+      hipLaunchKernelGGL(range_kernel<128>, dim3((unsigned)nb), dim3(kThreads), 0, s, tv, start, hi, len, cap_const,
+                         (uint32_t)n, offsets, offset_stride, out_keys, out_vals, out_counts, capacity);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t range_offsets_tail(const uint32_t* len, uint64_t* offsets, size_t n, hipStream_t s) {
+  hipLaunchKernelGGL(offsets_tail_kernel, dim3(1), dim3(64), 0, s, len, offsets, (uint32_t)n);
+  return hipGetLastError();
+}
+
+}  // namespace hm

@@ -1,0 +1,196 @@
+// Batched point search over the Harmonia layout.
+//
+// Replaces multi_prefix_sort_kernel (H/regular_bpt/prefix-sort.cu:98-250) and
+// full_multi_prefix_sort_8thread_kernel (H/src/full-multi-prefix-sort-8thread.cu:95-263):
+// level-synchronous descent child = prefix[node] + slot, then an exact match in the leaf.
+// Differences by design: groups of G lanes of a 64-lane wave instead of 32-lane warps with
+// shared-memory flag arrays; the top levels are staged in LDS (the reference's v14 kernels keep
+// the top-level prefix entries in __constant__ memory, H/src/ppi-bpt-v14-1.cu:128,223);
+// the value is read on the device and written with a found flag in the caller's order.
+#include "hm_device.h"
+
+namespace hm {
+namespace {
+
+constexpr int kThreads = 256;
+
+template <int F, int G, int Q, bool SCATTER>
+__global__ __launch_bounds__(kThreads) void search_kernel(TreeView tv, int lds_levels,
+                                                          const uint64_t* __restrict__ q,
+                                                          const uint32_t* __restrict__ pos,
+                                                          uint64_t* __restrict__ out_vals,
+                                                          uint8_t* __restrict__ out_found, uint32_t n,
+                                                          uint32_t chunk) {
+  extern __shared__ __align__(16) uint64_t smem[];
+  uint32_t* spre;
+  stage_top_levels<F>(tv, lds_levels, smem, spre);
+
+  constexpr int W = 8 / G;
+  const uint32_t vb = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t begin = vb * chunk;
+  const uint32_t end = begin + chunk < n ? begin + chunk : n;
+  const int gl = threadIdx.x % G;
+  const int gbase = (threadIdx.x & 63) - gl;
+  const uint32_t gid = threadIdx.x / G;
+  constexpr uint32_t gpb = kThreads / G;
+  constexpr uint32_t tile = gpb * Q;
+
+  for (uint32_t qb = begin; qb < end; qb += tile) {  // uniform across the block
+    uint64_t t[Q], td[Q];
+    uint32_t leaf[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const uint32_t qi = qb + i * gpb + gid;
+      t[i] = q[qi < end ? qi : end - 1];  // branch-free: lanes past the end repeat the last query
+      td[i] = t[i] == KEY_MAX ? KEY_MAX - 1 : t[i];
+    }
+    descend_q<F, G, Q>(tv, lds_levels, smem, spre, td, leaf, gl, gbase);
+    // The leaf record is one 128-byte line: keys and values are read together.
+    uint64_t k[Q][W], v[Q][W];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const Leaf* lf = tv.leaves + leaf[i];
+      load_keys<W>(lf->key + gl * W, k[i]);
+      load_keys<W>(lf->val + gl * W, v[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const uint32_t qi = qb + i * gpb + gid;
+      const int lt = group_count_lt<G, W>(k[i], t[i], gbase);  // lower_bound slot
+      const bool eq = group_any_eq<G, W>(k[i], t[i], gbase);   // exact match (H/regular_bpt/prefix-sort.cu:220-227)
+      const int writer = eq ? lt / W : 0;
+      if (qi < end && gl == writer) {
+        uint64_t val = 0ull;
+#pragma unroll
+        for (int w = 0; w < W; ++w)
+          if (eq && lt % W == w) val = v[i][w];
+        const uint32_t o = SCATTER ? pos[qi] : qi;
+        out_vals[o] = val;
+        out_found[o] = eq ? 1 : 0;
+      }
+    }
+  }
+}
+
+// Locate for the update path: leaf id, lower_bound slot and found flag per key (input order).
+template <int F, int G>
+__global__ __launch_bounds__(kThreads) void locate_kernel(TreeView tv, const uint64_t* __restrict__ keys, uint32_t n,
+                                                          uint32_t* __restrict__ leaf_out,
+                                                          uint32_t* __restrict__ slot_out,
+                                                          uint8_t* __restrict__ found_out) {
+  constexpr int W = 8 / G;
+  const int gl = threadIdx.x % G;
+  const int gbase = (threadIdx.x & 63) - gl;
+  constexpr uint32_t gpb = kThreads / G;
+  const uint32_t stride = gridDim.x * gpb;
+  const uint32_t first = blockIdx.x * gpb;
+  for (uint32_t qb = first; qb < n; qb += stride) {
+    const uint32_t qi = qb + threadIdx.x / G;
+    const bool valid = qi < n;
+    const uint64_t t = valid ? keys[qi] : 0ull;
+    const uint64_t td = t == KEY_MAX ? KEY_MAX - 1 : t;
+    const uint32_t leaf = descend<F, G>(tv, 0, nullptr, nullptr, td, gl, gbase);
+    uint64_t k[W];
+    load_keys<W>(tv.leaves[leaf].key + gl * W, k);
+    const int lt = group_count_lt<G, W>(k, t, gbase);
+    const bool eq = group_any_eq<G, W>(k, t, gbase);
+    if (valid && gl == 0) {
+      leaf_out[qi] = leaf;
+      slot_out[qi] = (uint32_t)lt;
+      found_out[qi] = eq ? 1 : 0;
+    }
+  }
+}
+
+template <int F, int G, int Q>
+hipError_t launch_fgq(const TreeView& tv, const SearchLaunch& sl, const uint64_t* q, const uint32_t* pos,
+                     uint64_t* out_vals, uint8_t* out_found, size_t n, bool scatter, hipStream_t s) {
+  const LaunchCfg& lc = launch_cfg();
+  constexpr uint32_t gpb = kThreads / G * Q;
+  size_t max_blocks = (size_t)lc.num_cus * (size_t)(sl.blocks_per_cu > 0 ? sl.blocks_per_cu : 8);
+  size_t need = (n + gpb - 1) / gpb;
+  size_t nb = need < max_blocks ? need : max_blocks;
+  nb = (nb + 7) / 8 * 8;
+  size_t chunk = (n + nb - 1) / nb;
+  chunk = (chunk + gpb - 1) / gpb * gpb;
+  const size_t lds = lds_bytes_for(tv, sl.lds_levels);
+  if (scatter)
+    hipLaunchKernelGGL((search_kernel<F, G, Q, true>), dim3((unsigned)nb), dim3(kThreads), lds, s, tv, sl.lds_levels, q,
+                       pos, out_vals, out_found, (uint32_t)n, (uint32_t)chunk);
+  else
+    hipLaunchKernelGGL((search_kernel<F, G, Q, false>), dim3((unsigned)nb), dim3(kThreads), lds, s, tv, sl.lds_levels,
+                       q, pos, out_vals, out_found, (uint32_t)n, (uint32_t)chunk);
+  return hipGetLastError();
+}
+
+template <int F, int G>
+hipError_t launch_fg(const TreeView& tv, const SearchLaunch& sl, const uint64_t* q, const uint32_t* pos,
+                     uint64_t* out_vals, uint8_t* out_found, size_t n, bool scatter, hipStream_t s) {
+  switch (sl.queries_per_group) {
+    case 1: return launch_fgq<F, G, 1>(tv, sl, q, pos, out_vals, out_found, n, scatter, s);
+    case 2: return launch_fgq<F, G, 2>(tv, sl, q, pos, out_vals, out_found, n, scatter, s);
+    case 4: return launch_fgq<F, G, 4>(tv, sl, q, pos, out_vals, out_found, n, scatter, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int F>
+hipError_t launch_f(const TreeView& tv, const SearchLaunch& sl, const uint64_t* q, const uint32_t* pos,
+                    uint64_t* out_vals, uint8_t* out_found, size_t n, bool scatter, hipStream_t s) {
+  switch (sl.group) {
+    case 1: return launch_fg<F, 1>(tv, sl, q, pos, out_vals, out_found, n, scatter, s);
+    case 2: return launch_fg<F, 2>(tv, sl, q, pos, out_vals, out_found, n, scatter, s);
+    case 4: return launch_fg<F, 4>(tv, sl, q, pos, out_vals, out_found, n, scatter, s);
+    case 8: return launch_fg<F, 8>(tv, sl, q, pos, out_vals, out_found, n, scatter, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace
+
+size_t lds_bytes_for(const TreeView& tv, int levels) {
+  if (levels <= 0) return 0;
+  const size_t nl = tv.level_start[levels];
+  return nl * (size_t)node_slots(tv.fanout) * sizeof(uint64_t) + nl * sizeof(uint32_t);
+}
+
+hipError_t launch_search(const TreeView& tv, const SearchLaunch& sl, const uint64_t* q, const uint32_t* pos,
+                         uint64_t* out_vals, uint8_t* out_found, size_t n, bool scatter, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  ProfScope prof("search", s);
+  switch (tv.fanout) {
+    case 32: return launch_f<32>(tv, sl, q, pos, out_vals, out_found, n, scatter, s);
+    case 64: return launch_f<64>(tv, sl, q, pos, out_vals, out_found, n, scatter, s);
+    case 128: return launch_f<128>(tv, sl, q, pos, out_vals, out_found, n, scatter, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_locate(const TreeView& tv, const uint64_t* keys, size_t n, uint32_t* leaf_out, uint32_t* slot_out,
+                         uint8_t* found_out, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  ProfScope prof("update", s);
+  constexpr int G = 8;
+  constexpr uint32_t gpb = kThreads / G;
+  size_t nb = (n + gpb - 1) / gpb;
+  if (nb > 8192) nb = 8192;
+  switch (tv.fanout) {
+    case 32:
+      hipLaunchKernelGGL((locate_kernel<32, G>), dim3((unsigned)nb), dim3(kThreads), 0, s, tv, keys, (uint32_t)n,
+                         leaf_out, slot_out, found_out);
+      break;
+    case 64:
+      hipLaunchKernelGGL((locate_kernel<64, G>), dim3((unsigned)nb), dim3(kThreads), 0, s, tv, keys, (uint32_t)n,
+                         leaf_out, slot_out, found_out);
+      break;
+    case 128:
+      hipLaunchKernelGGL((locate_kernel<128, G>), dim3((unsigned)nb), dim3(kThreads), 0, s, tv, keys, (uint32_t)n,
+                         leaf_out, slot_out, found_out);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace hm

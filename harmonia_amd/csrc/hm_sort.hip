@@ -1,0 +1,190 @@
+// Hand-written stable LSD radix sort for u64 keys with a u32 or u64 payload.
+//
+// The PSA step of the reference sorts (key, position) pairs with
+// cub::DeviceRadixSort::SortPairs(keys, pos, n, 48, 64) before every search launch
+// (H/regular_bpt/prefix-sort.cu:288-293, H/src/full-multi-prefix-sort-8thread.cu:305-309).
+// That call is a stable LSD radix sort over key bits [begin_bit, end_bit); for keys in
+// [0, 2^63) CUB's signed-key bit twiddle (H/cub/cub/util_type.cuh:1015-1022) does not change the
+// order, so the result equals a stable sort of (key, pos) by key >> begin_bit.  The same routine
+// (with a u64 value payload over all varying bits) sorts the bulk-build input.
+//
+// One pass = histogram kernel (per-tile digit counts, digit-major) -> exclusive scan over
+// (digit, tile) -> scatter kernel (wave-level digit match via 8 ballots gives each item its
+// stable rank inside its wave; per-wave counters are scanned across the block).
+#include "hm_internal.h"
+
+namespace hm {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kItems = 16;
+constexpr int kTile = kThreads * kItems;
+constexpr int kWaves = kThreads / 64;
+
+// Lanes (among `valid`) whose digit equals this lane's digit.
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t valid) {
+  uint64_t peers = valid;
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const bool bit = (d >> b) & 1u;
+    const uint64_t m = __ballot(bit);
+    peers &= bit ? m : ~m;
+  }
+  return peers;
+}
+
+__global__ __launch_bounds__(kThreads) void radix_hist_kernel(const uint64_t* __restrict__ keys, size_t n, int shift,
+                                                              uint32_t mask, uint32_t* __restrict__ counts,
+                                                              uint32_t nb) {
+  __shared__ uint32_t hist[256];
+  hist[threadIdx.x] = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  const size_t base = (size_t)blockIdx.x * kTile + (size_t)w * 64 * kItems;
+#pragma unroll 4
+  for (int r = 0; r < kItems; ++r) {
+    const size_t i = base + (size_t)r * 64 + lane;
+    const bool valid = i < n;
+    const uint32_t d = valid ? (uint32_t)(keys[i] >> shift) & mask : 0u;
+    const uint64_t peers = match_digit(d, __ballot(valid));
+    if (valid && (peers & lt) == 0) atomicAdd(&hist[d], (uint32_t)__popcll(peers));
+  }
+  __syncthreads();
+  counts[(size_t)threadIdx.x * nb + blockIdx.x] = hist[threadIdx.x];
+}
+
+template <typename V>
+__global__ __launch_bounds__(kThreads) void radix_scatter_kernel(const uint64_t* __restrict__ kin,
+                                                                 const V* __restrict__ vin,
+                                                                 uint64_t* __restrict__ kout, V* __restrict__ vout,
+                                                                 size_t n, int shift, uint32_t mask,
+                                                                 const uint32_t* __restrict__ offsets, uint32_t nb,
+                                                                 int gen_pos) {
+  __shared__ uint32_t wcnt[kWaves][256];
+  __shared__ uint32_t goff[256];
+  for (int i = threadIdx.x; i < kWaves * 256; i += kThreads) (&wcnt[0][0])[i] = 0;
+  goff[threadIdx.x] = offsets[(size_t)threadIdx.x * nb + blockIdx.x];
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  const size_t base = (size_t)blockIdx.x * kTile + (size_t)w * 64 * kItems;
+  uint64_t kk[kItems];
+  V vv[kItems];
+  uint32_t pk[kItems];  // (rank within wave << 8) | digit
+#pragma unroll
+  for (int r = 0; r < kItems; ++r) {
+    const size_t i = base + (size_t)r * 64 + lane;
+    const bool valid = i < n;
+    const uint64_t k = valid ? kin[i] : 0ull;
+    V v = (V)0;
+    if (valid) v = gen_pos ? (V)i : vin[i];
+    const uint32_t d = (uint32_t)(k >> shift) & mask;
+    const uint64_t peers = match_digit(d, __ballot(valid));
+    const uint32_t prior = wcnt[w][d];
+    if (valid && (peers & lt) == 0) wcnt[w][d] = prior + (uint32_t)__popcll(peers);
+    kk[r] = k;
+    vv[r] = v;
+    pk[r] = ((prior + (uint32_t)__popcll(peers & lt)) << 8) | d;
+  }
+  __syncthreads();
+  {
+    const int d = threadIdx.x;
+    uint32_t run = 0;
+#pragma unroll
+    for (int ww = 0; ww < kWaves; ++ww) {
+      const uint32_t t = wcnt[ww][d];
+      wcnt[ww][d] = run + goff[d];
+      run += t;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kItems; ++r) {
+    const size_t i = base + (size_t)r * 64 + lane;
+    if (i < n) {
+      const uint32_t d = pk[r] & 255u;
+      const uint32_t p = wcnt[w][d] + (pk[r] >> 8);
+      kout[p] = kk[r];
+      vout[p] = vv[r];
+    }
+  }
+}
+
+template <typename V>
+__global__ void copy_with_pos_kernel(const uint64_t* __restrict__ kin, const V* __restrict__ vin,
+                                     uint64_t* __restrict__ kout, V* __restrict__ vout, size_t n, int gen_pos) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    kout[i] = kin[i];
+    vout[i] = gen_pos ? (V)i : vin[i];
+  }
+}
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+template <typename V>
+hipError_t radix_sort_impl(const uint64_t* keys_in, const V* vals_in, size_t n, int begin_bit, int end_bit,
+                           uint64_t* k0, V* v0, uint64_t* k1, V* v1, void* tmp, hipStream_t s, bool gen_pos,
+                           int* which) {
+  *which = 0;
+  if (n == 0) return hipSuccess;
+  if (begin_bit < 0 || end_bit > 64 || begin_bit > end_bit) return hipErrorInvalidValue;
+  const size_t nb = (n + kTile - 1) / kTile;
+  uint32_t* counts = reinterpret_cast<uint32_t*>(tmp);
+  void* scan_tmp = reinterpret_cast<char*>(tmp) + align_up(256 * nb * sizeof(uint32_t), 256);
+  if (begin_bit == end_bit) {
+    hipLaunchKernelGGL((copy_with_pos_kernel<V>), dim3(1024), dim3(256), 0, s, keys_in, vals_in, k0, v0, n,
+                       gen_pos ? 1 : 0);
+    return hipGetLastError();
+  }
+  const uint64_t* sk = keys_in;
+  const V* sv = vals_in;
+  int out = 0;
+  bool gen = gen_pos;
+  for (int shift = begin_bit; shift < end_bit; shift += 8) {
+    const int bits = end_bit - shift < 8 ? end_bit - shift : 8;
+    const uint32_t mask = (1u << bits) - 1u;
+    {
+      ProfScope prof("psa_hist", s);
+      hipLaunchKernelGGL(radix_hist_kernel, dim3((unsigned)nb), dim3(kThreads), 0, s, sk, n, shift, mask, counts,
+                         (uint32_t)nb);
+    }
+    hipError_t e = exclusive_scan_u32(counts, counts, 256 * nb, 0u, scan_tmp, s);
+    if (e != hipSuccess) return e;
+    uint64_t* dk = out == 0 ? k0 : k1;
+    V* dv = out == 0 ? v0 : v1;
+    {
+      ProfScope prof("psa_scatter", s);
+      hipLaunchKernelGGL((radix_scatter_kernel<V>), dim3((unsigned)nb), dim3(kThreads), 0, s, sk, sv, dk, dv, n,
+                         shift, mask, (const uint32_t*)counts, (uint32_t)nb, gen ? 1 : 0);
+    }
+    sk = dk;
+    sv = dv;
+    gen = false;
+    *which = out;
+    out ^= 1;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
+size_t radix_tmp_bytes(size_t n) {
+  const size_t nb = (n + kTile - 1) / kTile + 1;
+  return align_up(256 * nb * sizeof(uint32_t), 256) + scan_tmp_bytes(256 * nb) + 256;
+}
+
+hipError_t radix_sort_pos(const uint64_t* keys_in, size_t n, int begin_bit, int end_bit, uint64_t* k0, uint32_t* v0,
+                          uint64_t* k1, uint32_t* v1, void* tmp, hipStream_t s, int* which) {
+  return radix_sort_impl<uint32_t>(keys_in, nullptr, n, begin_bit, end_bit, k0, v0, k1, v1, tmp, s, true, which);
+}
+
+hipError_t radix_sort_u64pay(const uint64_t* keys_in, const uint64_t* vals_in, size_t n, int begin_bit, int end_bit,
+                             uint64_t* k0, uint64_t* v0, uint64_t* k1, uint64_t* v1, void* tmp, hipStream_t s,
+                             int* which) {
+  return radix_sort_impl<uint64_t>(keys_in, vals_in, n, begin_bit, end_bit, k0, v0, k1, v1, tmp, s, false, which);
+}
+
+}  // namespace hm

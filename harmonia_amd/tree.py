@@ -1,0 +1,236 @@
+"""Host-side mirror of the reference's tree interface over the C ABI.
+
+`BPlusTree` keeps the reference's method names (``insert``, ``search``, ``range_search``,
+``insert_parallel``, ``getHeight``, ``getPrefixArray``, ``getLevelIdx``; H/src/bpt.h:456-561 and the
+regular tree's ``find`` / ``find_range``, H/regular_bpt/regular_bpt.h:218-246) and adds the batched
+forms the GPU path is built for.  Every operation runs on the GPU through libharmonia_amd; nothing
+here computes a result on the CPU.
+
+Arrays may be numpy arrays (host memory, staged by the library) or torch tensors on the GPU
+(device pointers, used in place).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import HmParams, HmSearchOpts, HmStats, check
+
+KEY_MAX = _lib.HM_KEY_MAX
+
+
+def _ptr(x):
+    """Raw address of a numpy array or torch tensor (None -> NULL)."""
+    if x is None:
+        return None
+    if isinstance(x, np.ndarray):
+        if not x.flags["C_CONTIGUOUS"]:
+            raise ValueError("arrays must be C-contiguous")
+        return x.ctypes.data
+    if hasattr(x, "data_ptr"):
+        if hasattr(x, "is_contiguous") and not x.is_contiguous():
+            raise ValueError("tensors must be contiguous")
+        return x.data_ptr()
+    raise TypeError(f"unsupported buffer type {type(x)!r}")
+
+
+def _u64(a):
+    if hasattr(a, "data_ptr"):
+        return a
+    return np.ascontiguousarray(a, dtype=np.uint64)
+
+
+def _len(a):
+    return int(a.numel()) if hasattr(a, "numel") else int(len(a))
+
+
+def _stream(stream):
+    if stream is None:
+        return None
+    if hasattr(stream, "cuda_stream"):
+        return stream.cuda_stream
+    return int(stream)
+
+
+class BPlusTree:
+    """A device-resident Harmonia B+tree (fanout F in {32, 64, 128}, 8-key leaves)."""
+
+    def __init__(self, fanout=64, leaf_fill=8, lds_levels=-1, group_size=0, psa_bits=16,
+                 sort_threshold=1 << 16, mem_budget=0):
+        self._lib = _lib.load()
+        self.params = HmParams()
+        check(self._lib.hm_params_init(ctypes.byref(self.params)))
+        self.params.fanout = fanout
+        self.params.leaf_fill = leaf_fill
+        self.params.lds_levels = lds_levels
+        self.params.group_size = group_size
+        self.params.psa_bits = psa_bits
+        self.params.sort_threshold = sort_threshold
+        self.params.mem_budget = mem_budget
+        self._h = None
+        self._pending_keys = []
+        self._pending_vals = []
+
+    # ---- construction -------------------------------------------------------------------
+    @classmethod
+    def from_keys(cls, keys, vals, keys_sorted=False, **params):
+        t = cls(**params)
+        t.build(keys, vals, keys_sorted=keys_sorted)
+        return t
+
+    def build(self, keys=None, vals=None, keys_sorted=False):
+        """Bulk build (replaces the reference's insert loop + BPlusTree::sort)."""
+        if keys is None:
+            keys = np.asarray(self._pending_keys, dtype=np.uint64)
+            vals = np.asarray(self._pending_vals, dtype=np.uint64)
+            self._pending_keys, self._pending_vals = [], []
+        keys, vals = _u64(keys), _u64(vals)
+        n = _len(keys)
+        if _len(vals) != n:
+            raise ValueError("keys and vals differ in length")
+        self.close()
+        p = HmParams.from_buffer_copy(self.params)
+        if keys_sorted:
+            p.flags |= _lib.HM_FLAG_KEYS_SORTED
+        h = ctypes.c_void_p()
+        check(self._lib.hm_tree_build(_ptr(keys), _ptr(vals), n, ctypes.byref(p), ctypes.byref(h)))
+        self._h = h
+        return self
+
+    def insert(self, key, val):
+        """Reference-style single insert: staged before build, an upsert after it."""
+        if self._h is None:
+            self._pending_keys.append(key)
+            self._pending_vals.append(val)
+            return True
+        self.insert_parallel(np.array([key], dtype=np.uint64), np.array([val], dtype=np.uint64))
+        return True
+
+    def close(self):
+        if self._h is not None:
+            self._lib.hm_tree_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _handle(self):
+        if self._h is None:
+            raise RuntimeError("tree is not built")
+        return self._h
+
+    # ---- point search --------------------------------------------------------------------
+    def search_batch(self, q, out_vals=None, out_found=None, stream=None, group_size=0, lds_levels=-1,
+                     psa_bits=-1, blocks_per_cu=0, queries_per_group=0, sorted_output=False, out_pos=None):
+        """Batched search; returns (values, found) in input order (or PSA order with out_pos)."""
+        q = _u64(q)
+        n = _len(q)
+        if out_vals is None:
+            out_vals = np.empty(n, dtype=np.uint64)
+        if out_found is None:
+            out_found = np.empty(n, dtype=np.uint8)
+        o = HmSearchOpts(group_size, lds_levels, psa_bits, blocks_per_cu, 1 if sorted_output else 0,
+                         queries_per_group, _ptr(out_pos))
+        check(self._lib.hm_search_ex(self._handle(), _ptr(q), n, _ptr(out_vals), _ptr(out_found),
+                                     ctypes.byref(o), _stream(stream)))
+        return out_vals, out_found
+
+    def search(self, key):
+        """Single search: the value, or None when absent (the reference returns -1)."""
+        v, f = self.search_batch(np.array([key], dtype=np.uint64), psa_bits=0)
+        return int(v[0]) if f[0] else None
+
+    find = search
+
+    # ---- range ----------------------------------------------------------------------------
+    def range_batch(self, starts, lens, stream=None):
+        """First len[i] keys >= start[i]; returns (offsets, keys, vals, counts)."""
+        starts = _u64(starts)
+        lens = np.ascontiguousarray(lens, dtype=np.uint32) if not hasattr(lens, "data_ptr") else lens
+        n = _len(starts)
+        cap = int(np.asarray(lens, dtype=np.uint64).sum()) if isinstance(lens, np.ndarray) else int(lens.sum())
+        offsets = np.empty(n + 1, dtype=np.uint64)
+        keys = np.empty(max(cap, 1), dtype=np.uint64)
+        vals = np.empty(max(cap, 1), dtype=np.uint64)
+        counts = np.empty(n, dtype=np.uint32)
+        check(self._lib.hm_range(self._handle(), _ptr(starts), _ptr(lens), n, _ptr(offsets), _ptr(keys),
+                                 _ptr(vals), _ptr(counts), cap, _stream(stream)))
+        return offsets, keys, vals, counts
+
+    def range_between(self, lo, hi, cap, stream=None):
+        """Keys in [lo[i], hi[i]], at most cap per range; returns (keys, vals, counts)."""
+        lo, hi = _u64(lo), _u64(hi)
+        n = _len(lo)
+        keys = np.empty(max(n * cap, 1), dtype=np.uint64)
+        vals = np.empty(max(n * cap, 1), dtype=np.uint64)
+        counts = np.empty(n, dtype=np.uint32)
+        check(self._lib.hm_range_between(self._handle(), _ptr(lo), _ptr(hi), n, cap, _ptr(keys), _ptr(vals),
+                                         _ptr(counts), _stream(stream)))
+        return keys, vals, counts
+
+    def range_search(self, start_key, end_key, cap=4096):
+        """All (key, value) with start <= key <= end, ascending (BPlusTree::range_search)."""
+        while True:
+            keys, vals, counts = self.range_between([start_key], [end_key], cap)
+            c = int(counts[0])
+            if c < cap:
+                return list(zip(keys[:c].tolist(), vals[:c].tolist()))
+            cap *= 4
+
+    # ---- update ---------------------------------------------------------------------------
+    def update(self, keys, vals, stream=None):
+        """Stage an upsert batch; visible after publish()."""
+        keys, vals = _u64(keys), _u64(vals)
+        n = _len(keys)
+        if _len(vals) != n:
+            raise ValueError("keys and vals differ in length")
+        check(self._lib.hm_update(self._handle(), _ptr(keys), _ptr(vals), n, _stream(stream)))
+
+    def publish(self, stream=None):
+        """Apply staged updates after every earlier search; returns True if leaves were re-laid out."""
+        r = ctypes.c_int(0)
+        check(self._lib.hm_epoch_publish(self._handle(), _stream(stream), ctypes.byref(r)))
+        return bool(r.value)
+
+    def insert_parallel(self, keys, vals):
+        """Batched upsert applied immediately (BPlusTree::insert_parallel)."""
+        self.update(keys, vals)
+        return self.publish()
+
+    # ---- introspection --------------------------------------------------------------------
+    def stats(self):
+        s = HmStats()
+        check(self._lib.hm_tree_stats(self._handle(), ctypes.byref(s)))
+        return s
+
+    def getHeight(self):
+        return int(self.stats().height)
+
+    def getLevelIdx(self, level):
+        return int(self.stats().level_start[level])
+
+    def export(self):
+        """Host copies of the layout: inner (n_inner, node_slots), prefix, leaves (n_leaves, 16)."""
+        s = self.stats()
+        inner = np.empty((s.n_inner, s.node_slots), dtype=np.uint64)
+        prefix = np.empty(s.n_inner, dtype=np.uint32)
+        leaves = np.empty((s.n_leaves, 16), dtype=np.uint64)
+        check(self._lib.hm_tree_export(self._handle(), _ptr(inner), _ptr(prefix), _ptr(leaves)))
+        return {"inner": inner, "prefix": prefix, "leaves": leaves, "stats": s}
+
+    def getPrefixArray(self):
+        return self.export()["prefix"]
+
+
+def psa_sort(keys, begin_bit=48, end_bit=64, stream=None):
+    """Stable radix sort of (key, position) over key bits [begin_bit, end_bit) on the GPU."""
+    lib = _lib.load()
+    keys = _u64(keys)
+    n = _len(keys)
+    ko = np.empty(n, dtype=np.uint64)
+    po = np.empty(n, dtype=np.uint32)
+    check(lib.hm_psa_sort(_ptr(keys), n, begin_bit, end_bit, _ptr(ko), _ptr(po), _stream(stream)))
+    return ko, po

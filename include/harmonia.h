@@ -1,0 +1,151 @@
+/*
+ * harmonia.h -- C ABI of the MI355X-native Harmonia B+tree.
+ *
+ * The tree keeps Harmonia's layout: inner nodes in one breadth-first key
+ * region (every node = an 8-key index granule followed by F key slots) plus a
+ * prefix-sum array holding each inner node's first-child index, so child i of
+ * node p is node prefix[p] + i.  Leaves are 128-byte records of 8 keys and
+ * their 8 values, contiguous in key order (the leaf chain is leaf i -> i + 1).
+ * Unused key slots hold HM_KEY_MAX, which is therefore not a storable key.
+ *
+ * Every entry point returns an int status (HM_OK or a negative HM_E* code);
+ * nothing here calls exit().  hm_last_error() returns the calling thread's
+ * last message.  Pointers to keys, values and results may be device memory or
+ * host memory (host memory is staged through the library's buffers).
+ * `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ *
+ * Reference interfaces these entry points replace are cited per function
+ * (paths relative to the reference repository root, H = open-harmonia/).
+ */
+#ifndef HARMONIA_H
+#define HARMONIA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HM_OK 0
+#define HM_EINVAL (-1)  /* invalid argument (bad size, null pointer, duplicate or reserved key) */
+#define HM_ENOMEM (-2)  /* device allocation failed */
+#define HM_EBUDGET (-3) /* request exceeds the tree's device-memory budget */
+#define HM_EDEVICE (-4) /* HIP runtime / kernel error */
+
+#define HM_KEY_MAX 0xFFFFFFFFFFFFFFFFull /* padding value; never a valid key */
+
+/* hm_params.flags */
+#define HM_FLAG_KEYS_SORTED 1u /* build input is already strictly ascending */
+
+typedef struct hm_tree hm_tree;
+
+typedef struct hm_params {
+  uint32_t fanout;         /* inner-node fanout F: 32, 64 or 128 (H/CMakeLists.txt:55 DEFAULT_ORDER) */
+  uint32_t leaf_fill;      /* keys per leaf at bulk build, 1..8 (slack = 8 - fill for inserts) */
+  int32_t lds_levels;      /* top inner levels staged in LDS, 0..3; -1 = pick automatically */
+  uint32_t group_size;     /* lanes cooperating on one query: 1, 2, 4 or 8; 0 = automatic */
+  uint32_t psa_bits;       /* partial-sort bits for batched search (reference: 16); 0 = PSA off */
+  uint32_t reserved0;
+  uint64_t sort_threshold; /* PSA applies only to batches of at least this many queries */
+  uint64_t mem_budget;     /* device bytes the tree may hold (tree + growth); 0 = unlimited */
+  uint32_t flags;          /* HM_FLAG_* */
+  uint32_t reserved1;
+} hm_params;
+
+typedef struct hm_stats {
+  uint64_t n_keys;
+  uint64_t n_leaves;
+  uint64_t n_inner;        /* inner nodes over all levels */
+  uint32_t height;         /* number of inner levels (leaf level not counted) */
+  uint32_t fanout;
+  uint32_t node_slots;     /* u64 slots per inner node (index granule + F keys) */
+  uint32_t lds_levels;     /* LDS-resident levels the search uses by default */
+  uint64_t level_start[17];/* BFS index of the first node of each inner level; [height] = n_inner */
+  uint64_t bytes_inner;
+  uint64_t bytes_prefix;
+  uint64_t bytes_leaves;
+  uint64_t bytes_device;   /* everything the tree holds on the device */
+  uint64_t epoch;          /* number of published update epochs */
+} hm_stats;
+
+/* Per-call overrides for batched search (used by the autotuner); 0 / -1 = tree default. */
+typedef struct hm_search_opts {
+  uint32_t group_size;     /* 1, 2, 4, 8 */
+  int32_t lds_levels;      /* -1 = default */
+  int32_t psa_bits;        /* -1 = default, 0 = off, 1..32 */
+  uint32_t blocks_per_cu;  /* 0 = default */
+  uint32_t sorted_output;  /* 1 = write results in PSA-sorted order and the permutation to out_pos */
+  uint32_t queries_per_group; /* independent queries each group keeps in flight: 1, 2, 4; 0 = default */
+  uint32_t* out_pos;       /* with sorted_output: pos[i] = input index of result i (H/regular_bpt/prefix-sort.cu:296-297) */
+} hm_search_opts;
+
+/* Fill *p with defaults: F=64, fill 8, LDS auto, group auto, PSA 16 bits, threshold 2^16. */
+int hm_params_init(hm_params* p);
+
+/* Bulk build.  Replaces the insert loop + relayout of the reference:
+ *   BPlusTree::insert + BPlusTree::sort + prepareArray (H/src/bpt.cpp:357-409,
+ *   H/src/traversal.cpp:35-195), insert + generateCFB (H/regular_bpt/regular_bpt.cpp:759,
+ *   H/regular_bpt/traversal.cpp:36-99) and the upload prepareGPU (H/regular_bpt/helper.cu:12-28).
+ * keys need not be sorted; duplicates and HM_KEY_MAX are rejected with HM_EINVAL.
+ * n == 0 builds an empty tree. */
+int hm_tree_build(const uint64_t* keys, const uint64_t* vals, uint64_t n, const hm_params* p, hm_tree** out);
+
+int hm_tree_destroy(hm_tree* t);
+
+/* Batched point search, results in input order: out_found[i] = 1 and out_vals[i] = value when
+ * q[i] is present, else out_found[i] = 0 and out_vals[i] = 0.
+ * Replaces multi_prefix_sort / full_multi_prefix_sort_8thread (H/regular_bpt/prefix-sort.cu:311,
+ * H/src/full-multi-prefix-sort-8thread.cu:329) and BPlusTree::search (H/src/bpt.cpp:673-680). */
+int hm_search(hm_tree* t, const uint64_t* q, uint64_t n, uint64_t* out_vals, uint8_t* out_found, void* stream);
+int hm_search_ex(hm_tree* t, const uint64_t* q, uint64_t n, uint64_t* out_vals, uint8_t* out_found,
+                 const hm_search_opts* opts, void* stream);
+
+/* Range scan: for each i, the first min(len[i], #keys >= start[i]) keys >= start[i] in ascending
+ * order.  out_offsets[0..n] = exclusive scan of len (capacity per range); range i writes its pairs at
+ * out_keys/out_vals[out_offsets[i] ..] and its count to out_counts[i].  Writes never pass
+ * `capacity` entries.  Replaces range_prefix_sort_1thread_search_and_1thread_scan (H/src/range.cu:312)
+ * and BPlusTree::range_search / find_range (H/src/bpt.cpp:695-769, H/regular_bpt/regular_bpt.cpp:320-338). */
+int hm_range(hm_tree* t, const uint64_t* start, const uint32_t* len, uint64_t n, uint64_t* out_offsets,
+             uint64_t* out_keys, uint64_t* out_vals, uint32_t* out_counts, uint64_t capacity, void* stream);
+
+/* Closed-interval form of the same scan: keys in [lo[i], hi[i]], at most `cap` per range
+ * (find_range(start, end), H/regular_bpt/regular_bpt.cpp:320-338). out_offsets[i] = i * cap. */
+int hm_range_between(hm_tree* t, const uint64_t* lo, const uint64_t* hi, uint64_t n, uint32_t cap,
+                     uint64_t* out_keys, uint64_t* out_vals, uint32_t* out_counts, void* stream);
+
+/* Batched upsert, staged: the batch becomes visible at the next hm_epoch_publish.  Within one
+ * epoch a later write of the same key wins.  Replaces BPlusTree::insert_parallel* and
+ * rebuild_without_leaf (H/src/bpt.cpp:518-670, H/src/bpt-ppi-update-batch.cpp:86-200,
+ * H/src/bpt-ppi-update.cpp:320-463). */
+int hm_update(hm_tree* t, const uint64_t* keys, const uint64_t* vals, uint64_t n, void* stream);
+
+/* Apply all staged updates on `stream` after every search/range issued so far on any stream,
+ * then make later searches wait for the new snapshot.  out_rebuilt (may be NULL) reports
+ * whether leaves had to be re-laid-out (leaf overflow) and the prefix array re-scanned. */
+int hm_epoch_publish(hm_tree* t, void* stream, int* out_rebuilt);
+
+int hm_tree_stats(const hm_tree* t, hm_stats* s);
+
+/* Copy the device layout to host buffers (any may be NULL): inner (n_inner * node_slots u64),
+ * prefix (n_inner u32), leaves (n_leaves * 16 u64: 8 keys then 8 values per leaf). */
+int hm_tree_export(const hm_tree* t, uint64_t* inner, uint32_t* prefix, uint64_t* leaves);
+
+/* PSA partial sort: stable LSD radix sort of (key, position) over key bits [begin_bit, end_bit),
+ * the semantics of cub::DeviceRadixSort::SortPairs(keys, pos, n, begin_bit, end_bit) with
+ * pos = 0..n-1 (H/regular_bpt/prefix-sort.cu:288-293, H/cub/cub/device/device_radix_sort.cuh:148-175). */
+int hm_psa_sort(const uint64_t* keys, uint64_t n, int begin_bit, int end_bit, uint64_t* keys_out,
+                uint32_t* pos_out, void* stream);
+
+/* Per-stage kernel timing with HIP events on the launch stream (off by default). */
+int hm_profile_enable(int on);
+int hm_profile_reset(void);
+/* stage: "search", "psa_hist", "psa_scatter", "scan", "range", "build", "update" ... */
+int hm_profile_get(const char* stage, double* total_ms, uint64_t* launches);
+
+int hm_device_synchronize(void);
+int hm_last_error(char* buf, uint64_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HARMONIA_H */

@@ -1,0 +1,282 @@
+"""GPU parity: every result of the HIP path is compared bit for bit with the oracle, the reference's
+golden vectors and closed forms.  All calls go through the C ABI (libharmonia_amd)."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from harmonia_amd import BPlusTree, HarmoniaError, psa_sort
+from harmonia_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+U64 = np.uint64
+KEY_MAX = U64(0xFFFFFFFFFFFFFFFF)
+
+
+def gold(name):
+    return np.load(os.path.join(GOLD, name), allow_pickle=False)
+
+
+def make(n, F=64, fill=8, seed=1, **kw):
+    keys, vals = W.gen_keys(n, seed)
+    t = BPlusTree.from_keys(keys, vals, fanout=F, leaf_fill=fill, **kw)
+    sk = np.sort(keys)
+    return t, keys, vals, sk, W.splitmix64_mix(sk)
+
+
+def edge(sk):
+    e = [0, 1, int(KEY_MAX), int(KEY_MAX) - 1, (1 << 63) - 1, (1 << 63) - 2]
+    if len(sk):
+        e += [int(sk[0]), int(sk[-1]), int(sk[0]) + 1, int(sk[-1]) + 1, max(int(sk[0]) - 1, 0)]
+    return np.array(e, dtype=U64)
+
+
+def queries(sk, m, seed):
+    base = W.point_queries(sk, m, seed) if len(sk) else W.gen_misses(0, m, seed)
+    return np.concatenate([base, edge(sk)])
+
+
+# ---------------------------------------------------------------- layout
+@pytest.mark.parametrize("n,F,fill", [(0, 64, 8), (1, 64, 8), (8, 32, 8), (9, 32, 8), (513, 64, 8), (4096, 128, 8),
+                                      (100003, 64, 8), (100003, 32, 5), (250000, 128, 6), (70000, 64, 1)])
+def test_device_layout_equals_oracle(gpu, n, F, fill):
+    print(f"layout n={n} F={F} fill={fill}", flush=True)
+    t, keys, vals, sk, sv = make(n, F, fill, seed=n + F)
+    e = t.export()
+    L = oracle.build_bfs(sk, sv, F, fill)
+    s = e["stats"]
+    assert s.height == L.height and s.n_inner == L.n_inner and s.n_leaves == L.n_leaves and s.n_keys == n
+    assert np.array_equal(e["prefix"], L.prefix)
+    assert np.array_equal(e["inner"], L.inner)
+    assert np.array_equal(e["leaves"], L.leaves)
+    assert oracle.check_layout(oracle.Layout(e["inner"], e["prefix"], e["leaves"], s.height, F,
+                                             np.array(s.level_start[:], dtype=np.uint32))) == 0
+
+
+def test_sorted_input_flag_and_rejections(gpu):
+    keys, vals = W.gen_keys(5000, 3)
+    sk = np.sort(keys)
+    t = BPlusTree.from_keys(sk, W.splitmix64_mix(sk), keys_sorted=True)
+    assert np.array_equal(t.export()["leaves"], oracle.build_bfs(sk, W.splitmix64_mix(sk), 64, 8).leaves)
+    with pytest.raises(HarmoniaError):  # duplicate keys
+        BPlusTree.from_keys(np.concatenate([keys, keys[:1]]), np.concatenate([vals, vals[:1]]))
+    with pytest.raises(HarmoniaError):  # reserved padding key
+        BPlusTree.from_keys(np.concatenate([keys, [KEY_MAX]]), np.concatenate([vals, [U64(0)]]))
+    with pytest.raises(HarmoniaError):  # flagged sorted but not sorted
+        BPlusTree.from_keys(keys, vals, keys_sorted=True)
+
+
+# ---------------------------------------------------------------- point search
+@pytest.mark.parametrize("F", [32, 64, 128])
+def test_search_all_launch_shapes(gpu, F):
+    t, keys, vals, sk, sv = make(100003, F, 8, seed=F)
+    q = queries(sk, 50000, F)
+    ev, ef = oracle.search_sorted(sk, sv, q)
+    L = oracle.build_bfs(sk, sv, F, 8)
+    cv, cf = oracle.search_cfb(L, q)
+    assert np.array_equal(cv, ev) and np.array_equal(cf, ef)
+    for g in (1, 2, 4, 8):
+        for qpg in (1, 2, 4):
+            for lds in (0, 1, 2):
+                for psa in (0, 16):
+                    v, f = t.search_batch(q, group_size=g, queries_per_group=qpg, lds_levels=lds, psa_bits=psa)
+                    assert np.array_equal(f, ef), (g, qpg, lds, psa)
+                    assert np.array_equal(v, ev), (g, qpg, lds, psa)
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 7, 8, 9, 64, 65, 511, 4097])
+def test_search_small_and_ragged(gpu, n):
+    t, keys, vals, sk, sv = make(n, 64, 8, seed=n)
+    for m in (1, 3, 63, 64, 65, 1000):
+        q = queries(sk, m, m)
+        ev, ef = oracle.search_sorted(sk, sv, q)
+        v, f = t.search_batch(q, psa_bits=16)
+        assert np.array_equal(f, ef) and np.array_equal(v, ev)
+        v, f = t.search_batch(q, psa_bits=0)
+        assert np.array_equal(f, ef) and np.array_equal(v, ev)
+    v, f = t.search_batch(np.empty(0, dtype=U64))
+    assert len(v) == 0
+
+
+@pytest.mark.parametrize("name,F", [("search_rbpt64.npz", 64), ("search_rbpt128.npz", 128)])
+@pytest.mark.parametrize("fill", [8, 4])
+def test_search_reference_golden(gpu, name, F, fill):
+    g = gold(name)
+    t = BPlusTree.from_keys(g["keys"], g["vals"], fanout=F, leaf_fill=fill)
+    for psa in (0, 16):
+        v, f = t.search_batch(g["queries"], psa_bits=psa)
+        assert np.array_equal(f, g["rbpt_found"])
+        assert np.array_equal(v[f == 1], g["rbpt_vals"][f == 1]) and np.all(v[f == 0] == 0)
+        if "harmonia_vals" in g:
+            assert np.array_equal(np.where(f == 1, v.view(np.int64), -1), g["harmonia_vals"])
+
+
+@pytest.mark.parametrize("order", ["unsorted", "runs4096", "sorted"])
+def test_search_query_orders(gpu, order):
+    t, keys, vals, sk, sv = make(300000, 64, 8, seed=11)
+    q = W.point_queries(sk, 200000, 12, order=order)
+    ev, ef = oracle.search_sorted(sk, sv, q)
+    for psa in (0, 16, 24):
+        v, f = t.search_batch(q, psa_bits=psa)
+        assert np.array_equal(f, ef) and np.array_equal(v, ev)
+
+
+def test_search_device_pointers(gpu):
+    import torch
+
+    t, keys, vals, sk, sv = make(200000, 64, 8, seed=21)
+    q = queries(sk, 100000, 22)
+    ev, ef = oracle.search_sorted(sk, sv, q)
+    dq = torch.from_numpy(q.view(np.int64)).to(gpu)
+    dv = torch.empty(len(q), dtype=torch.int64, device=gpu)
+    df = torch.empty(len(q), dtype=torch.uint8, device=gpu)
+    s = torch.cuda.Stream()
+    for psa in (0, 16):
+        t.search_batch(dq, out_vals=dv, out_found=df, stream=s, psa_bits=psa)
+        s.synchronize()
+        assert np.array_equal(df.cpu().numpy(), ef)
+        assert np.array_equal(dv.cpu().numpy().view(U64), ev)
+
+
+def test_sorted_output_matches_psa_order(gpu):
+    t, keys, vals, sk, sv = make(50000, 64, 8, seed=31)
+    q = queries(sk, 70000, 32)
+    pos = np.empty(len(q), dtype=np.uint32)
+    v, f = t.search_batch(q, psa_bits=16, sorted_output=True, out_pos=pos)
+    ko, po = oracle.psa_sort(q, 48, 64)
+    assert np.array_equal(pos, po)
+    ev, ef = oracle.search_sorted(sk, sv, ko)
+    assert np.array_equal(f, ef) and np.array_equal(v, ev)
+
+
+# ---------------------------------------------------------------- PSA sort
+@pytest.mark.parametrize("n", [1, 100, 4095, 4096, 4097, 300001])
+@pytest.mark.parametrize("bits", [(48, 64), (0, 64), (56, 64), (40, 56), (60, 64), (3, 9)])
+def test_psa_sort_bit_exact(gpu, n, bits):
+    g = W.rng(n, bits[0])
+    keys = g.integers(0, 1 << 63, n, dtype=np.uint64)
+    if n > 10:
+        keys[: n // 4] = keys[n // 2: n // 2 + n // 4]  # equal keys exercise stability
+    ko, po = psa_sort(keys, *bits)
+    eo, ep = oracle.psa_sort(keys, *bits)
+    assert np.array_equal(po, ep) and np.array_equal(ko, eo)
+
+
+# ---------------------------------------------------------------- range
+@pytest.mark.parametrize("F,fill", [(64, 8), (128, 8), (32, 3)])
+def test_range_parity(gpu, F, fill):
+    t, keys, vals, sk, sv = make(60000, F, fill, seed=41 + F)
+    starts, lens = W.range_queries(20000, 41)
+    starts[:100] = sk[W.rng(1, 1).integers(0, len(sk), 100)]
+    starts[100] = sk[-1] + U64(1)
+    starts[101] = 0
+    starts[102] = KEY_MAX
+    lens[103] = 0
+    L = oracle.build_bfs(sk, sv, F, fill)
+    eo, ek, ev, ec = oracle.range_walk(L, starts, lens=lens)
+    so, sk2, sv2, sc = oracle.range_sorted(sk, sv, starts, lens=lens)
+    assert np.array_equal(ec, sc)
+    off, k, v, c = t.range_batch(starts, lens)
+    assert np.array_equal(off, eo) and np.array_equal(c, ec)
+    for i in range(len(c)):
+        a, b = int(off[i]), int(off[i]) + int(c[i])
+        assert np.array_equal(k[a:b], ek[a:b]) and np.array_equal(v[a:b], ev[a:b]), i
+
+
+def test_range_reference_golden(gpu):
+    g = gold("range_rbpt64.npz")
+    t = BPlusTree.from_keys(g["keys"], g["vals"])
+    cap = int(g["harmonia_counts"].max()) + 1
+    k, v, c = t.range_between(g["lo"], g["hi"], cap)
+    assert np.array_equal(c, g["harmonia_counts"].astype(np.uint32))
+    got = np.concatenate([k[i * cap:i * cap + c[i]] for i in range(len(c))])
+    gotv = np.concatenate([v[i * cap:i * cap + c[i]] for i in range(len(c))])
+    assert np.array_equal(got, g["harmonia_keys"]) and np.array_equal(gotv, g["harmonia_vals"])
+    off, k, v, c = t.range_batch(g["starts"], g["lens"])
+    assert np.array_equal(c, g["len_counts"])
+    got = np.concatenate([k[int(off[i]):int(off[i]) + c[i]] for i in range(len(c))])
+    assert np.array_equal(got, g["len_keys"])
+    assert t.range_search(int(g["lo"][0]), int(g["hi"][0])) == list(
+        zip(g["harmonia_keys"][: g["harmonia_counts"][0]].tolist(), g["harmonia_vals"][: g["harmonia_counts"][0]].tolist()))
+
+
+# ---------------------------------------------------------------- update
+def full_check(t, sk, sv, seed):
+    q = np.concatenate([sk, W.gen_misses(len(sk), 5000, seed) ^ U64(1 << 40), edge(sk)])
+    ev, ef = oracle.search_sorted(sk, sv, q)
+    v, f = t.search_batch(q)
+    assert np.array_equal(f, ef) and np.array_equal(v, ev)
+    e = t.export()
+    s = e["stats"]
+    lay = oracle.Layout(e["inner"], e["prefix"], e["leaves"], s.height, s.fanout,
+                        np.array(s.level_start[:], dtype=np.uint32))
+    assert oracle.check_layout(lay) == 0
+    assert s.n_keys == len(sk)
+    lk = e["leaves"][:, :8].ravel()
+    assert np.array_equal(lk[lk != KEY_MAX], sk)  # the leaf chain holds exactly the key set, in order
+    starts, lens = W.range_queries(3000, seed)
+    off, k, vv, c = t.range_batch(starts, lens)
+    eo, ek, evv, ec = oracle.range_sorted(sk, sv, starts, lens=lens)
+    assert np.array_equal(c, ec)
+    m = np.concatenate([np.arange(int(off[i]), int(off[i]) + int(c[i])) for i in range(len(c))]).astype(np.int64)
+    assert np.array_equal(k[m], ek[m]) and np.array_equal(vv[m], evv[m])
+
+
+@pytest.mark.parametrize("fill", [6, 8])
+def test_update_in_place_and_relayout(gpu, fill):
+    t, keys, vals, sk, sv = make(80000, 64, fill, seed=51)
+    g = W.rng(51, 7)
+    cur_k, cur_v = sk, sv
+    rebuilt_any = False
+    for step in range(4):
+        new = W.gen_misses(len(keys), 400 * (step + 1), 51 + step) ^ U64(step << 50)
+        new = new[~np.isin(new, cur_k)]
+        old = cur_k[g.integers(0, len(cur_k), 300)]
+        bk = np.concatenate([new, old, old[:50]])  # repeated keys: the later write wins
+        bv = g.integers(0, 1 << 64, len(bk), dtype=np.uint64)
+        g.shuffle(idx := np.arange(len(bk)))
+        bk, bv = bk[idx], bv[idx]
+        rebuilt_any |= t.insert_parallel(bk, bv)
+        cur_k, cur_v = oracle.upsert_sorted(cur_k, cur_v, bk, bv)
+        full_check(t, cur_k, cur_v, 60 + step)
+    if fill == 8:
+        assert rebuilt_any  # full leaves force the re-layout path
+
+
+def test_update_epoch_snapshot(gpu):
+    t, keys, vals, sk, sv = make(20000, 64, 6, seed=61)
+    bk = sk[:100]
+    bv = np.full(100, 7, dtype=U64)
+    t.update(bk, bv)
+    v, f = t.search_batch(bk)  # staged, not yet published: old snapshot
+    assert np.array_equal(v, sv[:100])
+    t.publish()
+    v, f = t.search_batch(bk)
+    assert np.all(v == 7) and np.all(f == 1)
+    with pytest.raises(HarmoniaError):
+        t.update(np.array([KEY_MAX], dtype=U64), np.array([1], dtype=U64))
+    assert t.stats().epoch == 1
+
+
+def test_update_reference_golden(gpu):
+    g = gold("update_rbpt64.npz")
+    for fill in (8, 6):
+        t = BPlusTree.from_keys(g["base_keys"], g["base_vals"], leaf_fill=fill)
+        t.insert_parallel(g["new_keys"], g["new_vals"])
+        v, f = t.search_batch(g["queries"])
+        assert np.array_equal(f, g["rbpt_found"])
+        assert np.array_equal(v[f == 1], g["rbpt_vals"][f == 1])
+
+
+def test_update_empty_tree_and_single_inserts(gpu):
+    t = BPlusTree.from_keys(np.empty(0, dtype=U64), np.empty(0, dtype=U64))
+    ks = []
+    for i, k in enumerate([50, 10, 30, 20, 40, 60, 70, 80, 90, 5]):
+        t.insert(k, k * 3)
+        ks.append(k)
+        sk = np.array(sorted(ks), dtype=U64)
+        full_check(t, sk, sk * U64(3), i)
+    assert t.search(30) == 90 and t.search(31) is None
