@@ -57,7 +57,8 @@ __global__ __launch_bounds__(kThreads) void search_kernel(TreeView tv, int lds_l
     for (int i = 0; i < Q; ++i) {
       const uint32_t qi = qb + i * gpb + gid;
       const int lt = group_count_lt<G, W>(k[i], t[i], gbase);  // lower_bound slot
-      const bool eq = group_any_eq<G, W>(k[i], t[i], gbase);   // exact match (H/regular_bpt/prefix-sort.cu:220-227)
+      // exact match (H/regular_bpt/prefix-sort.cu:220-227); KEY_MAX is padding, never a stored key
+      const bool eq = group_any_eq<G, W>(k[i], t[i], gbase) && t[i] != KEY_MAX;
       const int writer = eq ? lt / W : 0;
       if (qi < end && gl == writer) {
         uint64_t val = 0ull;
@@ -93,7 +94,7 @@ __global__ __launch_bounds__(kThreads) void locate_kernel(TreeView tv, const uin
     uint64_t k[W];
     load_keys<W>(tv.leaves[leaf].key + gl * W, k);
     const int lt = group_count_lt<G, W>(k, t, gbase);
-    const bool eq = group_any_eq<G, W>(k, t, gbase);
+    const bool eq = group_any_eq<G, W>(k, t, gbase) && t != KEY_MAX;
     if (valid && gl == 0) {
       leaf_out[qi] = leaf;
       slot_out[qi] = (uint32_t)lt;
