@@ -16,6 +16,10 @@ HM_EBUDGET = -3
 HM_EDEVICE = -4
 HM_KEY_MAX = 0xFFFFFFFFFFFFFFFF
 HM_FLAG_KEYS_SORTED = 1
+HM_OUT_DEFAULT = 0
+HM_OUT_UNPERMUTE = 1
+HM_OUT_SCATTER = 2
+HM_OUT_PSA_ORDER = 3
 
 u8p = ctypes.POINTER(ctypes.c_uint8)
 u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -62,7 +66,7 @@ class HmSearchOpts(ctypes.Structure):
         ("lds_levels", ctypes.c_int32),
         ("psa_bits", ctypes.c_int32),
         ("blocks_per_cu", ctypes.c_uint32),
-        ("sorted_output", ctypes.c_uint32),
+        ("output_mode", ctypes.c_uint32),
         ("queries_per_group", ctypes.c_uint32),
         ("out_pos", vp),
     ]

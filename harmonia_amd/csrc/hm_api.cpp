@@ -120,6 +120,13 @@ ProfScope::~ProfScope() {
 // tree object
 namespace {
 
+struct SearchCfg {
+  int group;
+  int queries_per_group;
+  int blocks_per_cu;
+  int output_mode;
+};
+
 struct Workspace {
   void* buf = nullptr;
   size_t bytes = 0;
@@ -141,7 +148,8 @@ struct hm_tree {
   uint32_t* d_prefix = nullptr;
   Leaf* d_leaves = nullptr;
   int lds_levels = 0;
-  int group = 4;
+  SearchCfg cfg_direct{4, 4, 4, HM_OUT_UNPERMUTE};  // unsorted batches: memory-transaction bound
+  SearchCfg cfg_psa{1, 2, 4, HM_OUT_UNPERMUTE};     // PSA-ordered batches: neighbours share nodes
   std::mutex mu;
   std::unordered_map<hipStream_t, Workspace> ws;
   hipStream_t own = nullptr;
@@ -346,7 +354,7 @@ int hm_tree_build(const uint64_t* keys, const uint64_t* vals, uint64_t n, const 
   t->F = (int)p->fanout;
   t->fill = (int)p->leaf_fill;
   t->NS = node_slots(t->F);
-  t->group = p->group_size ? (int)p->group_size : 4;
+  if (p->group_size) t->cfg_direct.group = t->cfg_psa.group = (int)p->group_size;
   auto fail = [&](int code) {
     free_tree(t);
     return code;
@@ -473,73 +481,82 @@ int hm_search_ex(hm_tree* t, const uint64_t* q, uint64_t n, uint64_t* out_vals, 
   hipStream_t s = (hipStream_t)stream;
   std::lock_guard<std::mutex> g(t->mu);
 
-  SearchLaunch sl;
-  sl.group = (o && o->group_size) ? (int)o->group_size : t->group;
-  if (sl.group != 1 && sl.group != 2 && sl.group != 4 && sl.group != 8)
-    return set_err(HM_EINVAL, "group_size %d not in {1,2,4,8}", sl.group);
-  sl.lds_levels = (o && o->lds_levels >= 0) ? choose_lds_levels(t, o->lds_levels) : t->lds_levels;
-  sl.blocks_per_cu = (o && o->blocks_per_cu) ? (int)o->blocks_per_cu : 8;
-  sl.queries_per_group = (o && o->queries_per_group) ? (int)o->queries_per_group : 2;
-  if (sl.queries_per_group != 1 && sl.queries_per_group != 2 && sl.queries_per_group != 4)
-    return set_err(HM_EINVAL, "queries_per_group %d not in {1,2,4}", sl.queries_per_group);
   int bits = (o && o->psa_bits >= 0) ? o->psa_bits : (int)t->p.psa_bits;
   if (bits < 0 || bits > 32) return set_err(HM_EINVAL, "psa_bits %d not in 0..32", bits);
   const bool psa = bits > 0 && (n >= t->p.sort_threshold || (o && o->psa_bits > 0));
-  const bool sorted_out = o && o->sorted_output;
-  if (sorted_out && (!psa || !o->out_pos)) return set_err(HM_EINVAL, "sorted_output needs PSA on and out_pos");
+  const SearchCfg& base = psa ? t->cfg_psa : t->cfg_direct;
+  SearchLaunch sl;
+  sl.group = (o && o->group_size) ? (int)o->group_size : base.group;
+  if (sl.group != 1 && sl.group != 2 && sl.group != 4 && sl.group != 8)
+    return set_err(HM_EINVAL, "group_size %d not in {1,2,4,8}", sl.group);
+  sl.lds_levels = (o && o->lds_levels >= 0) ? choose_lds_levels(t, o->lds_levels) : t->lds_levels;
+  sl.blocks_per_cu = (o && o->blocks_per_cu) ? (int)o->blocks_per_cu : base.blocks_per_cu;
+  sl.queries_per_group = (o && o->queries_per_group) ? (int)o->queries_per_group : base.queries_per_group;
+  if (sl.queries_per_group != 1 && sl.queries_per_group != 2 && sl.queries_per_group != 4)
+    return set_err(HM_EINVAL, "queries_per_group %d not in {1,2,4}", sl.queries_per_group);
+  const int out_mode = (o && o->output_mode) ? (int)o->output_mode : base.output_mode;
+  if (out_mode < HM_OUT_UNPERMUTE || out_mode > HM_OUT_PSA_ORDER)
+    return set_err(HM_EINVAL, "output_mode %d not in 0..3", out_mode);
+  const bool psa_order = out_mode == HM_OUT_PSA_ORDER;
+  if (psa_order && (!psa || !o || !o->out_pos))
+    return set_err(HM_EINVAL, "PSA-order output needs PSA on and out_pos");
+  const bool unpermute = psa && out_mode == HM_OUT_UNPERMUTE;
 
   const bool hq = !is_device_ptr(q), hv = !is_device_ptr(out_vals), hf = !is_device_ptr(out_found);
-  const bool hp = sorted_out && !is_device_ptr(o->out_pos);
+  const bool hp = psa_order && !is_device_ptr(o->out_pos);
   size_t need = 0;
   const size_t kq = align256(n * 8), kp = align256(n * 4);
   if (hq) need += kq;
   if (hv) need += kq;
   if (hf) need += align256(n);
   if (psa) need += 2 * kq + 2 * kp + align256(radix_tmp_bytes(n));
+  if (unpermute) need += kq + kp;
   Workspace* w = nullptr;
   int rc = ensure_ws(t, s, need, &w);
   if (rc) return rc;
   if (t->publish_valid) HM_TRY(hipStreamWaitEvent(s, t->publish_ev, 0));
   char* cur = (char*)w->buf;
+  auto take = [&](size_t b) {
+    char* r = cur;
+    cur += b;
+    return r;
+  };
   const uint64_t* dq = q;
   uint64_t* dvals = out_vals;
   uint8_t* dfound = out_found;
   if (hq) {
-    HM_TRY(hipMemcpyAsync(cur, q, n * 8, hipMemcpyHostToDevice, s));
-    dq = (const uint64_t*)cur;
-    cur += kq;
+    uint64_t* b = (uint64_t*)take(kq);
+    HM_TRY(hipMemcpyAsync(b, q, n * 8, hipMemcpyHostToDevice, s));
+    dq = b;
   }
-  if (hv) {
-    dvals = (uint64_t*)cur;
-    cur += kq;
-  }
-  if (hf) {
-    dfound = (uint8_t*)cur;
-    cur += align256(n);
-  }
+  if (hv) dvals = (uint64_t*)take(kq);
+  if (hf) dfound = (uint8_t*)take(align256(n));
   const TreeView tv = t->view();
-  uint32_t* dpos_res = nullptr;
   if (psa) {
-    uint64_t* k0 = (uint64_t*)cur;
-    cur += kq;
-    uint64_t* k1 = (uint64_t*)cur;
-    cur += kq;
-    uint32_t* p0 = (uint32_t*)cur;
-    cur += kp;
-    uint32_t* p1 = (uint32_t*)cur;
-    cur += kp;
-    void* tmp = cur;
+    uint64_t* k0 = (uint64_t*)take(kq);
+    uint64_t* k1 = (uint64_t*)take(kq);
+    uint32_t* p0 = (uint32_t*)take(kp);
+    uint32_t* p1 = (uint32_t*)take(kp);
+    void* tmp = take(align256(radix_tmp_bytes(n)));
     int which = 0;
     HM_TRY(radix_sort_pos(dq, n, 64 - bits, 64, k0, p0, k1, p1, tmp, s, &which));
     const uint64_t* sk = which ? k1 : k0;
     const uint32_t* sp = which ? p1 : p0;
-    HM_TRY(launch_search(tv, sl, sk, sp, dvals, dfound, n, !sorted_out, s));
-    dpos_res = (uint32_t*)sp;
+    if (unpermute) {
+      // search in PSA order writing coalesced (value, pos|found) records, then sort them back by
+      // position; the sort buffers are free once the search has consumed them (stream order)
+      uint64_t* rv = (uint64_t*)take(kq);
+      uint32_t* rp = (uint32_t*)take(kp);
+      HM_TRY(launch_search(tv, sl, sk, sp, rv, (uint8_t*)rp, n, OUT_RECORDS, s));
+      HM_TRY(unpermute_results(rp, rv, n, p0, k0, p1, k1, dvals, dfound, tmp, s));
+    } else if (psa_order) {
+      HM_TRY(launch_search(tv, sl, sk, sp, dvals, dfound, n, OUT_DIRECT, s));
+      HM_TRY(hipMemcpyAsync(o->out_pos, sp, n * 4, hp ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, s));
+    } else {
+      HM_TRY(launch_search(tv, sl, sk, sp, dvals, dfound, n, OUT_SCATTER, s));
+    }
   } else {
-    HM_TRY(launch_search(tv, sl, dq, nullptr, dvals, dfound, n, false, s));
-  }
-  if (sorted_out) {
-    HM_TRY(hipMemcpyAsync(o->out_pos, dpos_res, n * 4, hp ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, s));
+    HM_TRY(launch_search(tv, sl, dq, nullptr, dvals, dfound, n, OUT_DIRECT, s));
   }
   if (hv) HM_TRY(hipMemcpyAsync(out_vals, dvals, n * 8, hipMemcpyDeviceToHost, s));
   if (hf) HM_TRY(hipMemcpyAsync(out_found, dfound, n, hipMemcpyDeviceToHost, s));

@@ -77,6 +77,12 @@ hipError_t radix_sort_pos(const uint64_t* keys_in, size_t n, int begin_bit, int 
 hipError_t radix_sort_u64pay(const uint64_t* keys_in, const uint64_t* vals_in, size_t n, int begin_bit, int end_bit,
                              uint64_t* k0, uint64_t* v0, uint64_t* k1, uint64_t* v1, void* tmp, hipStream_t s,
                              int* which);
+// Return PSA-ordered results to input order: records (posf[i] = pos | found << 31, vals[i]) are
+// radix-sorted by position (LSD, 8-bit digits, p/v ping-pong buffers), the last pass writing
+// out_vals[pos] / out_found[pos] directly.  n < 2^31.
+hipError_t unpermute_results(const uint32_t* posf, const uint64_t* vals, size_t n, uint32_t* p0, uint64_t* v0,
+                             uint32_t* p1, uint64_t* v1, uint64_t* out_vals, uint8_t* out_found, void* tmp,
+                             hipStream_t s);
 
 // ---- build (hm_build.hip) --------------------------------------------------------------
 // flags |= 1 if a key is KEY_MAX, |= 2 if keys are not strictly ascending.
@@ -98,8 +104,14 @@ struct SearchLaunch {
   int blocks_per_cu = 8;
   int queries_per_group = 2;  // independent descents interleaved per group
 };
+// Where a search writes result i of its (possibly PSA-sorted) query array:
+enum SearchOut {
+  OUT_DIRECT = 0,   // out_vals[i], out_found[i]
+  OUT_SCATTER = 1,  // out_vals[pos[i]], out_found[pos[i]]
+  OUT_RECORDS = 2,  // out_vals[i] and ((uint32_t*)out_found)[i] = pos[i] | found << 31 (for unpermute_results)
+};
 hipError_t launch_search(const TreeView& tv, const SearchLaunch& sl, const uint64_t* q, const uint32_t* pos,
-                         uint64_t* out_vals, uint8_t* out_found, size_t n, bool scatter, hipStream_t s);
+                         uint64_t* out_vals, uint8_t* out_found, size_t n, SearchOut mode, hipStream_t s);
 // Bytes of LDS the top `levels` need.
 size_t lds_bytes_for(const TreeView& tv, int levels);
 // Locate (leaf, slot, found) per key (update path).

@@ -2,6 +2,7 @@
 // radix-sort digit offsets, range output offsets and the leaf re-layout after updates.
 // The prefix array semantics follow H/regular_bpt/traversal.cpp:40,58 and H/src/traversal.cpp:180-195
 // (prefix[0] = 1, prefix[k+1] = prefix[k] + children_k), computed here as 1 + exclusive scan.
+#include "hm_block.h"
 #include "hm_internal.h"
 
 namespace hm {
@@ -11,40 +12,6 @@ constexpr int kThreads = 256;
 constexpr int kItems = 8;
 constexpr int kTile = kThreads * kItems;
 constexpr int kWaves = kThreads / 64;
-
-template <typename T>
-__device__ __forceinline__ T wave_inclusive_scan(T x) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    T y = __shfl_up(x, o, 64);
-    if (lane >= o) x += y;
-  }
-  return x;
-}
-
-// Exclusive scan of one value per thread across the block; also returns the block total.
-template <typename T>
-__device__ __forceinline__ T block_exclusive_scan(T v, T* sh, T& total) {
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  T inc = wave_inclusive_scan(v);
-  if (lane == 63) sh[w] = inc;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    T run = 0;
-    for (int i = 0; i < kWaves; ++i) {
-      T t = sh[i];
-      sh[i] = run;
-      run += t;
-    }
-    sh[kWaves] = run;
-  }
-  __syncthreads();
-  total = sh[kWaves];
-  T r = inc - v + sh[w];
-  __syncthreads();
-  return r;
-}
 
 template <typename Tin, typename T>
 __global__ __launch_bounds__(kThreads) void scan_reduce_kernel(const Tin* __restrict__ in, size_t n,
@@ -58,7 +25,7 @@ __global__ __launch_bounds__(kThreads) void scan_reduce_kernel(const Tin* __rest
     if (idx < n) s += (T)in[idx];
   }
   T total;
-  (void)block_exclusive_scan<T>(s, sh, total);
+  (void)block_exclusive_scan<T, kThreads>(s, sh, total);
   if (threadIdx.x == 0) partials[blockIdx.x] = total;
 }
 
@@ -71,7 +38,7 @@ __global__ __launch_bounds__(kThreads) void scan_partials_kernel(T* __restrict__
     size_t idx = base + threadIdx.x;
     T v = idx < m ? partials[idx] : (T)0;
     T total;
-    T ex = block_exclusive_scan<T>(v, sh, total);
+    T ex = block_exclusive_scan<T, kThreads>(v, sh, total);
     if (idx < m) partials[idx] = ex + carry;
     carry += total;
   }
@@ -91,7 +58,7 @@ __global__ __launch_bounds__(kThreads) void scan_down_kernel(const Tin* __restri
     s += v[i];
   }
   T total;
-  T ex = block_exclusive_scan<T>(s, sh, total) + partials[blockIdx.x];
+  T ex = block_exclusive_scan<T, kThreads>(s, sh, total) + partials[blockIdx.x];
 #pragma unroll
   for (int i = 0; i < kItems; ++i) {
     size_t idx = base + i;

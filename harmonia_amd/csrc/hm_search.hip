@@ -14,7 +14,7 @@ namespace {
 
 constexpr int kThreads = 256;
 
-template <int F, int G, int Q, bool SCATTER>
+template <int F, int G, int Q, int OUT>
 __global__ __launch_bounds__(kThreads) void search_kernel(TreeView tv, int lds_levels,
                                                           const uint64_t* __restrict__ q,
                                                           const uint32_t* __restrict__ pos,
@@ -26,6 +26,9 @@ __global__ __launch_bounds__(kThreads) void search_kernel(TreeView tv, int lds_l
   stage_top_levels<F>(tv, lds_levels, smem, spre);
 
   constexpr int W = 8 / G;
+  // G == 1: one lane per query reads its 8 leaf keys, then only the matching value (same 128-B
+  // line, an L2 hit); G > 1: the group reads keys and values of the leaf line together.
+  constexpr bool kValsWithKeys = G > 1;
   const uint32_t vb = xcd_block(blockIdx.x, gridDim.x);
   const uint32_t begin = vb * chunk;
   const uint32_t end = begin + chunk < n ? begin + chunk : n;
@@ -45,13 +48,12 @@ __global__ __launch_bounds__(kThreads) void search_kernel(TreeView tv, int lds_l
       td[i] = t[i] == KEY_MAX ? KEY_MAX - 1 : t[i];
     }
     descend_q<F, G, Q>(tv, lds_levels, smem, spre, td, leaf, gl, gbase);
-    // The leaf record is one 128-byte line: keys and values are read together.
     uint64_t k[Q][W], v[Q][W];
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
       const Leaf* lf = tv.leaves + leaf[i];
       load_keys<W>(lf->key + gl * W, k[i]);
-      load_keys<W>(lf->val + gl * W, v[i]);
+      if constexpr (kValsWithKeys) load_keys<W>(lf->val + gl * W, v[i]);
     }
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
@@ -62,12 +64,24 @@ __global__ __launch_bounds__(kThreads) void search_kernel(TreeView tv, int lds_l
       const int writer = eq ? lt / W : 0;
       if (qi < end && gl == writer) {
         uint64_t val = 0ull;
+        if constexpr (kValsWithKeys) {
 #pragma unroll
-        for (int w = 0; w < W; ++w)
-          if (eq && lt % W == w) val = v[i][w];
-        const uint32_t o = SCATTER ? pos[qi] : qi;
-        out_vals[o] = val;
-        out_found[o] = eq ? 1 : 0;
+          for (int w = 0; w < W; ++w)
+            if (eq && lt % W == w) val = v[i][w];
+        } else {
+          if (eq) val = tv.leaves[leaf[i]].val[lt];
+        }
+        if constexpr (OUT == OUT_DIRECT) {
+          out_vals[qi] = val;
+          out_found[qi] = eq ? 1 : 0;
+        } else if constexpr (OUT == OUT_SCATTER) {
+          const uint32_t o = pos[qi];
+          out_vals[o] = val;
+          out_found[o] = eq ? 1 : 0;
+        } else {
+          out_vals[qi] = val;
+          reinterpret_cast<uint32_t*>(out_found)[qi] = pos[qi] | (eq ? 0x80000000u : 0u);
+        }
       }
     }
   }
@@ -105,7 +119,7 @@ __global__ __launch_bounds__(kThreads) void locate_kernel(TreeView tv, const uin
 
 template <int F, int G, int Q>
 hipError_t launch_fgq(const TreeView& tv, const SearchLaunch& sl, const uint64_t* q, const uint32_t* pos,
-                     uint64_t* out_vals, uint8_t* out_found, size_t n, bool scatter, hipStream_t s) {
+                     uint64_t* out_vals, uint8_t* out_found, size_t n, SearchOut mode, hipStream_t s) {
   const LaunchCfg& lc = launch_cfg();
   constexpr uint32_t gpb = kThreads / G * Q;
   size_t max_blocks = (size_t)lc.num_cus * (size_t)(sl.blocks_per_cu > 0 ? sl.blocks_per_cu : 8);
@@ -115,34 +129,42 @@ hipError_t launch_fgq(const TreeView& tv, const SearchLaunch& sl, const uint64_t
   size_t chunk = (n + nb - 1) / nb;
   chunk = (chunk + gpb - 1) / gpb * gpb;
   const size_t lds = lds_bytes_for(tv, sl.lds_levels);
-  if (scatter)
-    hipLaunchKernelGGL((search_kernel<F, G, Q, true>), dim3((unsigned)nb), dim3(kThreads), lds, s, tv, sl.lds_levels, q,
-                       pos, out_vals, out_found, (uint32_t)n, (uint32_t)chunk);
-  else
-    hipLaunchKernelGGL((search_kernel<F, G, Q, false>), dim3((unsigned)nb), dim3(kThreads), lds, s, tv, sl.lds_levels,
-                       q, pos, out_vals, out_found, (uint32_t)n, (uint32_t)chunk);
+  switch (mode) {
+    case OUT_DIRECT:
+      hipLaunchKernelGGL((search_kernel<F, G, Q, OUT_DIRECT>), dim3((unsigned)nb), dim3(kThreads), lds, s, tv,
+                         sl.lds_levels, q, pos, out_vals, out_found, (uint32_t)n, (uint32_t)chunk);
+      break;
+    case OUT_SCATTER:
+      hipLaunchKernelGGL((search_kernel<F, G, Q, OUT_SCATTER>), dim3((unsigned)nb), dim3(kThreads), lds, s, tv,
+                         sl.lds_levels, q, pos, out_vals, out_found, (uint32_t)n, (uint32_t)chunk);
+      break;
+    default:
+      hipLaunchKernelGGL((search_kernel<F, G, Q, OUT_RECORDS>), dim3((unsigned)nb), dim3(kThreads), lds, s, tv,
+                         sl.lds_levels, q, pos, out_vals, out_found, (uint32_t)n, (uint32_t)chunk);
+      break;
+  }
   return hipGetLastError();
 }
 
 template <int F, int G>
 hipError_t launch_fg(const TreeView& tv, const SearchLaunch& sl, const uint64_t* q, const uint32_t* pos,
-                     uint64_t* out_vals, uint8_t* out_found, size_t n, bool scatter, hipStream_t s) {
+                     uint64_t* out_vals, uint8_t* out_found, size_t n, SearchOut mode, hipStream_t s) {
   switch (sl.queries_per_group) {
-    case 1: return launch_fgq<F, G, 1>(tv, sl, q, pos, out_vals, out_found, n, scatter, s);
-    case 2: return launch_fgq<F, G, 2>(tv, sl, q, pos, out_vals, out_found, n, scatter, s);
-    case 4: return launch_fgq<F, G, 4>(tv, sl, q, pos, out_vals, out_found, n, scatter, s);
+    case 1: return launch_fgq<F, G, 1>(tv, sl, q, pos, out_vals, out_found, n, mode, s);
+    case 2: return launch_fgq<F, G, 2>(tv, sl, q, pos, out_vals, out_found, n, mode, s);
+    case 4: return launch_fgq<F, G, 4>(tv, sl, q, pos, out_vals, out_found, n, mode, s);
     default: return hipErrorInvalidValue;
   }
 }
 
 template <int F>
 hipError_t launch_f(const TreeView& tv, const SearchLaunch& sl, const uint64_t* q, const uint32_t* pos,
-                    uint64_t* out_vals, uint8_t* out_found, size_t n, bool scatter, hipStream_t s) {
+                    uint64_t* out_vals, uint8_t* out_found, size_t n, SearchOut mode, hipStream_t s) {
   switch (sl.group) {
-    case 1: return launch_fg<F, 1>(tv, sl, q, pos, out_vals, out_found, n, scatter, s);
-    case 2: return launch_fg<F, 2>(tv, sl, q, pos, out_vals, out_found, n, scatter, s);
-    case 4: return launch_fg<F, 4>(tv, sl, q, pos, out_vals, out_found, n, scatter, s);
-    case 8: return launch_fg<F, 8>(tv, sl, q, pos, out_vals, out_found, n, scatter, s);
+    case 1: return launch_fg<F, 1>(tv, sl, q, pos, out_vals, out_found, n, mode, s);
+    case 2: return launch_fg<F, 2>(tv, sl, q, pos, out_vals, out_found, n, mode, s);
+    case 4: return launch_fg<F, 4>(tv, sl, q, pos, out_vals, out_found, n, mode, s);
+    case 8: return launch_fg<F, 8>(tv, sl, q, pos, out_vals, out_found, n, mode, s);
     default: return hipErrorInvalidValue;
   }
 }
@@ -156,13 +178,13 @@ size_t lds_bytes_for(const TreeView& tv, int levels) {
 }
 
 hipError_t launch_search(const TreeView& tv, const SearchLaunch& sl, const uint64_t* q, const uint32_t* pos,
-                         uint64_t* out_vals, uint8_t* out_found, size_t n, bool scatter, hipStream_t s) {
+                         uint64_t* out_vals, uint8_t* out_found, size_t n, SearchOut mode, hipStream_t s) {
   if (n == 0) return hipSuccess;
   ProfScope prof("search", s);
   switch (tv.fanout) {
-    case 32: return launch_f<32>(tv, sl, q, pos, out_vals, out_found, n, scatter, s);
-    case 64: return launch_f<64>(tv, sl, q, pos, out_vals, out_found, n, scatter, s);
-    case 128: return launch_f<128>(tv, sl, q, pos, out_vals, out_found, n, scatter, s);
+    case 32: return launch_f<32>(tv, sl, q, pos, out_vals, out_found, n, mode, s);
+    case 64: return launch_f<64>(tv, sl, q, pos, out_vals, out_found, n, mode, s);
+    case 128: return launch_f<128>(tv, sl, q, pos, out_vals, out_found, n, mode, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -124,16 +124,22 @@ class BPlusTree:
 
     # ---- point search --------------------------------------------------------------------
     def search_batch(self, q, out_vals=None, out_found=None, stream=None, group_size=0, lds_levels=-1,
-                     psa_bits=-1, blocks_per_cu=0, queries_per_group=0, sorted_output=False, out_pos=None):
-        """Batched search; returns (values, found) in input order (or PSA order with out_pos)."""
+                     psa_bits=-1, blocks_per_cu=0, queries_per_group=0, output_mode=0, sorted_output=False,
+                     out_pos=None):
+        """Batched search; returns (values, found) in input order (or PSA order with out_pos).
+
+        output_mode: 0 tree default, 1 input order via radix un-permute, 2 input order via direct
+        scatter, 3 (or sorted_output=True) PSA order with the permutation in out_pos."""
+        if sorted_output:
+            output_mode = _lib.HM_OUT_PSA_ORDER
         q = _u64(q)
         n = _len(q)
         if out_vals is None:
             out_vals = np.empty(n, dtype=np.uint64)
         if out_found is None:
             out_found = np.empty(n, dtype=np.uint8)
-        o = HmSearchOpts(group_size, lds_levels, psa_bits, blocks_per_cu, 1 if sorted_output else 0,
-                         queries_per_group, _ptr(out_pos))
+        o = HmSearchOpts(group_size, lds_levels, psa_bits, blocks_per_cu, output_mode, queries_per_group,
+                         _ptr(out_pos))
         check(self._lib.hm_search_ex(self._handle(), _ptr(q), n, _ptr(out_vals), _ptr(out_found),
                                      ctypes.byref(o), _stream(stream)))
         return out_vals, out_found

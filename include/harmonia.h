@@ -68,15 +68,21 @@ typedef struct hm_stats {
   uint64_t epoch;          /* number of published update epochs */
 } hm_stats;
 
+/* hm_search_opts.output_mode (with PSA on; without PSA results are always in input order) */
+#define HM_OUT_DEFAULT 0 /* the tree's choice */
+#define HM_OUT_UNPERMUTE 1 /* input order: results radix-sorted back by position (coalesced writes) */
+#define HM_OUT_SCATTER 2 /* input order: each result written straight to its position */
+#define HM_OUT_PSA_ORDER 3 /* PSA order plus out_pos, as the reference returns them (H/regular_bpt/prefix-sort.cu:296-297) */
+
 /* Per-call overrides for batched search (used by the autotuner); 0 / -1 = tree default. */
 typedef struct hm_search_opts {
-  uint32_t group_size;     /* 1, 2, 4, 8 */
+  uint32_t group_size;     /* lanes per query: 1, 2, 4, 8 */
   int32_t lds_levels;      /* -1 = default */
-  int32_t psa_bits;        /* -1 = default, 0 = off, 1..32 */
+  int32_t psa_bits;        /* -1 = default (PSA for batches >= sort_threshold), 0 = off, 1..32 = on */
   uint32_t blocks_per_cu;  /* 0 = default */
-  uint32_t sorted_output;  /* 1 = write results in PSA-sorted order and the permutation to out_pos */
+  uint32_t output_mode;    /* HM_OUT_* */
   uint32_t queries_per_group; /* independent queries each group keeps in flight: 1, 2, 4; 0 = default */
-  uint32_t* out_pos;       /* with sorted_output: pos[i] = input index of result i (H/regular_bpt/prefix-sort.cu:296-297) */
+  uint32_t* out_pos;       /* with HM_OUT_PSA_ORDER: out_pos[i] = input index of result i */
 } hm_search_opts;
 
 /* Fill *p with defaults: F=64, fill 8, LDS auto, group auto, PSA 16 bits, threshold 2^16. */

@@ -141,6 +141,24 @@ def test_search_device_pointers(gpu):
         assert np.array_equal(dv.cpu().numpy().view(U64), ev)
 
 
+@pytest.mark.parametrize("n_q", [1, 2, 255, 256, 257, 65537, 300001])
+def test_output_modes(gpu, n_q):
+    """PSA results come back in input order by radix un-permute (1) or direct scatter (2)."""
+    t, keys, vals, sk, sv = make(120000, 64, 8, seed=n_q)
+    q = queries(sk, n_q, n_q + 1)
+    ev, ef = oracle.search_sorted(sk, sv, q)
+    for mode in (1, 2):
+        for g, qpg in ((1, 1), (1, 2), (4, 2), (8, 4)):
+            for bits in (16, 24):
+                v, f = t.search_batch(q, psa_bits=bits, output_mode=mode, group_size=g, queries_per_group=qpg)
+                assert np.array_equal(f, ef), (mode, g, qpg, bits)
+                assert np.array_equal(v, ev), (mode, g, qpg, bits)
+    pos = np.empty(len(q), dtype=np.uint32)
+    v, f = t.search_batch(q, psa_bits=16, output_mode=3, out_pos=pos)
+    assert np.array_equal(np.sort(pos), np.arange(len(q), dtype=np.uint32))
+    assert np.array_equal(f, ef[pos]) and np.array_equal(v, ev[pos])
+
+
 def test_sorted_output_matches_psa_order(gpu):
     t, keys, vals, sk, sv = make(50000, 64, 8, seed=31)
     q = queries(sk, 70000, 32)
