@@ -15,6 +15,7 @@
 
 #include "../../include/harmonia.h"
 #include "hm_internal.h"
+#include "hm_tree.h"
 
 namespace hm {
 hipError_t range_offsets_tail(const uint32_t* len, uint64_t* offsets, size_t n, hipStream_t s);
@@ -23,8 +24,8 @@ hipError_t range_offsets_tail(const uint32_t* len, uint64_t* offsets, size_t n, 
 using namespace hm;
 
 // ---------------------------------------------------------------------------------------------
-// errors
-namespace {
+// errors and small helpers
+namespace hm {
 thread_local std::string g_err;
 
 int set_err(int code, const char* fmt, ...) {
@@ -37,18 +38,18 @@ int set_err(int code, const char* fmt, ...) {
   return code;
 }
 
-#define HM_TRY(expr)                                                                                       \
-  do {                                                                                                     \
-    hipError_t e_ = (expr);                                                                                \
-    if (e_ != hipSuccess) {                                                                                \
-      int code_ = (e_ == hipErrorOutOfMemory) ? HM_ENOMEM : HM_EDEVICE;                                    \
-      return set_err(code_, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__);    \
-    }                                                                                                      \
-  } while (0)
-
 size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
-}  // namespace
+bool is_device_ptr(const void* p) {
+  if (!p) return false;
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+}  // namespace hm
 
 // ---------------------------------------------------------------------------------------------
 // launch configuration and profiling
@@ -93,6 +94,8 @@ hipEvent_t prof_event() {
 
 ProfScope::ProfScope(const char* stage, hipStream_t s) : slot(-1), stream(s) {
   if (!g_prof_on) return;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;  // no events in graphs
   std::lock_guard<std::mutex> g(g_prof_mu);
   int id = -1;
   for (size_t i = 0; i < g_stage_names.size(); ++i)
@@ -117,77 +120,8 @@ ProfScope::~ProfScope() {
 }  // namespace hm
 
 // ---------------------------------------------------------------------------------------------
-// tree object
-namespace {
-
-struct SearchCfg {
-  int group;
-  int queries_per_group;
-  int blocks_per_cu;
-  int output_mode;
-};
-
-struct Workspace {
-  void* buf = nullptr;
-  size_t bytes = 0;
-  hipEvent_t last_use = nullptr;  // the latest search/range issued on this stream
-};
-
-}  // namespace
-
-struct hm_tree {
-  hm_params p{};
-  int F = 64, fill = 8, NS = 72;
-  int device = 0;
-  uint64_t n_keys = 0;
-  uint64_t n_leaves = 0, leaf_cap = 0;
-  uint32_t n_inner = 0, inner_cap = 0;
-  int height = 1;
-  uint32_t level_start[MAX_LEVELS + 1] = {0};
-  uint64_t* d_inner = nullptr;
-  uint32_t* d_prefix = nullptr;
-  Leaf* d_leaves = nullptr;
-  int lds_levels = 0;
-  SearchCfg cfg_direct{4, 4, 4, HM_OUT_UNPERMUTE};  // unsorted batches: memory-transaction bound
-  SearchCfg cfg_psa{1, 2, 4, HM_OUT_UNPERMUTE};     // PSA-ordered batches: neighbours share nodes
-  std::mutex mu;
-  std::unordered_map<hipStream_t, Workspace> ws;
-  hipStream_t own = nullptr;
-  hipEvent_t publish_ev = nullptr;
-  bool publish_valid = false;
-  uint64_t epoch = 0;
-  uint64_t* pend_k = nullptr;
-  uint64_t* pend_v = nullptr;
-  size_t pend_n = 0, pend_cap = 0;
-
-  TreeView view() const {
-    TreeView tv{};
-    tv.inner = d_inner;
-    tv.prefix = d_prefix;
-    tv.leaves = d_leaves;
-    tv.n_inner = n_inner;
-    tv.n_leaves = (uint32_t)n_leaves;
-    tv.height = height;
-    tv.fanout = F;
-    for (int i = 0; i <= MAX_LEVELS; ++i) tv.level_start[i] = level_start[i];
-    return tv;
-  }
-  uint64_t device_bytes() const {
-    return (uint64_t)inner_cap * NS * 8 + (uint64_t)inner_cap * 4 + leaf_cap * sizeof(Leaf) + pend_cap * 16;
-  }
-};
-
-namespace {
-
-bool is_device_ptr(const void* p) {
-  if (!p) return false;
-  hipPointerAttribute_t a;
-  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
-    (void)hipGetLastError();
-    return false;
-  }
-  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
-}
+// tree object helpers
+namespace hm {
 
 int ensure_ws(hm_tree* t, hipStream_t s, size_t bytes, Workspace** out) {
   Workspace& w = t->ws[s];
@@ -219,6 +153,10 @@ int choose_lds_levels(const hm_tree* t, int req) {
     if (lds_bytes_for(tv, l) <= cap) best = l;
   return best;
 }
+
+}  // namespace hm
+
+namespace {
 
 // Level sizes for n_leaves leaves; returns height.
 int plan_levels(uint64_t n_leaves, int F, uint32_t* level_start, uint32_t* n_inner) {

@@ -73,7 +73,8 @@ hipError_t exclusive_scan_u32_to_u64(const uint32_t* in, uint64_t* out, size_t n
 size_t radix_tmp_bytes(size_t n);
 // payload = position 0..n-1 generated on the first pass (vals_in unused)
 hipError_t radix_sort_pos(const uint64_t* keys_in, size_t n, int begin_bit, int end_bit, uint64_t* k0, uint32_t* v0,
-                          uint64_t* k1, uint32_t* v1, void* tmp, hipStream_t s, int* which);
+                          uint64_t* k1, uint32_t* v1, void* tmp, hipStream_t s, int* which,
+                          const uint32_t* dn = nullptr);
 hipError_t radix_sort_u64pay(const uint64_t* keys_in, const uint64_t* vals_in, size_t n, int begin_bit, int end_bit,
                              uint64_t* k0, uint64_t* v0, uint64_t* k1, uint64_t* v1, void* tmp, hipStream_t s,
                              int* which);
@@ -82,7 +83,7 @@ hipError_t radix_sort_u64pay(const uint64_t* keys_in, const uint64_t* vals_in, s
 // out_vals[pos] / out_found[pos] directly.  n < 2^31.
 hipError_t unpermute_results(const uint32_t* posf, const uint64_t* vals, size_t n, uint32_t* p0, uint64_t* v0,
                              uint32_t* p1, uint64_t* v1, uint64_t* out_vals, uint8_t* out_found, void* tmp,
-                             hipStream_t s);
+                             hipStream_t s, const uint32_t* dn = nullptr);
 
 // ---- build (hm_build.hip) --------------------------------------------------------------
 // flags |= 1 if a key is KEY_MAX, |= 2 if keys are not strictly ascending.
@@ -110,8 +111,10 @@ enum SearchOut {
   OUT_SCATTER = 1,  // out_vals[pos[i]], out_found[pos[i]]
   OUT_RECORDS = 2,  // out_vals[i] and ((uint32_t*)out_found)[i] = pos[i] | found << 31 (for unpermute_results)
 };
+// n sizes the grid; with dn != nullptr the kernel reads the actual count (<= n) from device memory.
 hipError_t launch_search(const TreeView& tv, const SearchLaunch& sl, const uint64_t* q, const uint32_t* pos,
-                         uint64_t* out_vals, uint8_t* out_found, size_t n, SearchOut mode, hipStream_t s);
+                         uint64_t* out_vals, uint8_t* out_found, size_t n, SearchOut mode, hipStream_t s,
+                         const uint32_t* dn = nullptr);
 // Bytes of LDS the top `levels` need.
 size_t lds_bytes_for(const TreeView& tv, int levels);
 // Locate (leaf, slot, found) per key (update path).

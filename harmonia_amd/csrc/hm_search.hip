@@ -20,7 +20,7 @@ __global__ __launch_bounds__(kThreads) void search_kernel(TreeView tv, int lds_l
                                                           const uint32_t* __restrict__ pos,
                                                           uint64_t* __restrict__ out_vals,
                                                           uint8_t* __restrict__ out_found, uint32_t n,
-                                                          uint32_t chunk) {
+                                                          const uint32_t* __restrict__ dn) {
   extern __shared__ __align__(16) uint64_t smem[];
   uint32_t* spre;
   stage_top_levels<F>(tv, lds_levels, smem, spre);
@@ -29,14 +29,19 @@ __global__ __launch_bounds__(kThreads) void search_kernel(TreeView tv, int lds_l
   // G == 1: one lane per query reads its 8 leaf keys, then only the matching value (same 128-B
   // line, an L2 hit); G > 1: the group reads keys and values of the leaf line together.
   constexpr bool kValsWithKeys = G > 1;
-  const uint32_t vb = xcd_block(blockIdx.x, gridDim.x);
-  const uint32_t begin = vb * chunk;
-  const uint32_t end = begin + chunk < n ? begin + chunk : n;
   const int gl = threadIdx.x % G;
   const int gbase = (threadIdx.x & 63) - gl;
   const uint32_t gid = threadIdx.x / G;
   constexpr uint32_t gpb = kThreads / G;
   constexpr uint32_t tile = gpb * Q;
+  // Persistent grid: each block owns one contiguous chunk of the batch; the count may live on the
+  // device (graph replays with data-dependent batch sizes).
+  if (dn) n = *dn;
+  uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  chunk = (chunk + tile - 1) / tile * tile;
+  const uint32_t vb = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t begin = vb * chunk;
+  const uint32_t end = begin + chunk < n ? begin + chunk : n;
 
   for (uint32_t qb = begin; qb < end; qb += tile) {  // uniform across the block
     uint64_t t[Q], td[Q];
@@ -119,28 +124,27 @@ __global__ __launch_bounds__(kThreads) void locate_kernel(TreeView tv, const uin
 
 template <int F, int G, int Q>
 hipError_t launch_fgq(const TreeView& tv, const SearchLaunch& sl, const uint64_t* q, const uint32_t* pos,
-                     uint64_t* out_vals, uint8_t* out_found, size_t n, SearchOut mode, hipStream_t s) {
+                     uint64_t* out_vals, uint8_t* out_found, size_t n, SearchOut mode, hipStream_t s,
+                     const uint32_t* dn) {
   const LaunchCfg& lc = launch_cfg();
   constexpr uint32_t gpb = kThreads / G * Q;
   size_t max_blocks = (size_t)lc.num_cus * (size_t)(sl.blocks_per_cu > 0 ? sl.blocks_per_cu : 8);
   size_t need = (n + gpb - 1) / gpb;
   size_t nb = need < max_blocks ? need : max_blocks;
   nb = (nb + 7) / 8 * 8;
-  size_t chunk = (n + nb - 1) / nb;
-  chunk = (chunk + gpb - 1) / gpb * gpb;
   const size_t lds = lds_bytes_for(tv, sl.lds_levels);
   switch (mode) {
     case OUT_DIRECT:
       hipLaunchKernelGGL((search_kernel<F, G, Q, OUT_DIRECT>), dim3((unsigned)nb), dim3(kThreads), lds, s, tv,
-                         sl.lds_levels, q, pos, out_vals, out_found, (uint32_t)n, (uint32_t)chunk);
+                         sl.lds_levels, q, pos, out_vals, out_found, (uint32_t)n, dn);
       break;
     case OUT_SCATTER:
       hipLaunchKernelGGL((search_kernel<F, G, Q, OUT_SCATTER>), dim3((unsigned)nb), dim3(kThreads), lds, s, tv,
-                         sl.lds_levels, q, pos, out_vals, out_found, (uint32_t)n, (uint32_t)chunk);
+                         sl.lds_levels, q, pos, out_vals, out_found, (uint32_t)n, dn);
       break;
     default:
       hipLaunchKernelGGL((search_kernel<F, G, Q, OUT_RECORDS>), dim3((unsigned)nb), dim3(kThreads), lds, s, tv,
-                         sl.lds_levels, q, pos, out_vals, out_found, (uint32_t)n, (uint32_t)chunk);
+                         sl.lds_levels, q, pos, out_vals, out_found, (uint32_t)n, dn);
       break;
   }
   return hipGetLastError();
@@ -148,23 +152,25 @@ hipError_t launch_fgq(const TreeView& tv, const SearchLaunch& sl, const uint64_t
 
 template <int F, int G>
 hipError_t launch_fg(const TreeView& tv, const SearchLaunch& sl, const uint64_t* q, const uint32_t* pos,
-                     uint64_t* out_vals, uint8_t* out_found, size_t n, SearchOut mode, hipStream_t s) {
+                     uint64_t* out_vals, uint8_t* out_found, size_t n, SearchOut mode, hipStream_t s,
+                     const uint32_t* dn) {
   switch (sl.queries_per_group) {
-    case 1: return launch_fgq<F, G, 1>(tv, sl, q, pos, out_vals, out_found, n, mode, s);
-    case 2: return launch_fgq<F, G, 2>(tv, sl, q, pos, out_vals, out_found, n, mode, s);
-    case 4: return launch_fgq<F, G, 4>(tv, sl, q, pos, out_vals, out_found, n, mode, s);
+    case 1: return launch_fgq<F, G, 1>(tv, sl, q, pos, out_vals, out_found, n, mode, s, dn);
+    case 2: return launch_fgq<F, G, 2>(tv, sl, q, pos, out_vals, out_found, n, mode, s, dn);
+    case 4: return launch_fgq<F, G, 4>(tv, sl, q, pos, out_vals, out_found, n, mode, s, dn);
     default: return hipErrorInvalidValue;
   }
 }
 
 template <int F>
 hipError_t launch_f(const TreeView& tv, const SearchLaunch& sl, const uint64_t* q, const uint32_t* pos,
-                    uint64_t* out_vals, uint8_t* out_found, size_t n, SearchOut mode, hipStream_t s) {
+                    uint64_t* out_vals, uint8_t* out_found, size_t n, SearchOut mode, hipStream_t s,
+                     const uint32_t* dn) {
   switch (sl.group) {
-    case 1: return launch_fg<F, 1>(tv, sl, q, pos, out_vals, out_found, n, mode, s);
-    case 2: return launch_fg<F, 2>(tv, sl, q, pos, out_vals, out_found, n, mode, s);
-    case 4: return launch_fg<F, 4>(tv, sl, q, pos, out_vals, out_found, n, mode, s);
-    case 8: return launch_fg<F, 8>(tv, sl, q, pos, out_vals, out_found, n, mode, s);
+    case 1: return launch_fg<F, 1>(tv, sl, q, pos, out_vals, out_found, n, mode, s, dn);
+    case 2: return launch_fg<F, 2>(tv, sl, q, pos, out_vals, out_found, n, mode, s, dn);
+    case 4: return launch_fg<F, 4>(tv, sl, q, pos, out_vals, out_found, n, mode, s, dn);
+    case 8: return launch_fg<F, 8>(tv, sl, q, pos, out_vals, out_found, n, mode, s, dn);
     default: return hipErrorInvalidValue;
   }
 }
@@ -178,13 +184,14 @@ size_t lds_bytes_for(const TreeView& tv, int levels) {
 }
 
 hipError_t launch_search(const TreeView& tv, const SearchLaunch& sl, const uint64_t* q, const uint32_t* pos,
-                         uint64_t* out_vals, uint8_t* out_found, size_t n, SearchOut mode, hipStream_t s) {
+                         uint64_t* out_vals, uint8_t* out_found, size_t n, SearchOut mode, hipStream_t s,
+                     const uint32_t* dn) {
   if (n == 0) return hipSuccess;
   ProfScope prof("search", s);
   switch (tv.fanout) {
-    case 32: return launch_f<32>(tv, sl, q, pos, out_vals, out_found, n, mode, s);
-    case 64: return launch_f<64>(tv, sl, q, pos, out_vals, out_found, n, mode, s);
-    case 128: return launch_f<128>(tv, sl, q, pos, out_vals, out_found, n, mode, s);
+    case 32: return launch_f<32>(tv, sl, q, pos, out_vals, out_found, n, mode, s, dn);
+    case 64: return launch_f<64>(tv, sl, q, pos, out_vals, out_found, n, mode, s, dn);
+    case 128: return launch_f<128>(tv, sl, q, pos, out_vals, out_found, n, mode, s, dn);
     default: return hipErrorInvalidValue;
   }
 }

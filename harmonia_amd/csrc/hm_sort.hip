@@ -40,18 +40,27 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t valid) {
 template <typename K>
 __global__ __launch_bounds__(kThreads) void radix_hist_kernel(const K* __restrict__ keys, size_t n, int shift,
                                                               uint32_t mask, uint32_t* __restrict__ counts,
-                                                              uint32_t nb) {
+                                                              uint32_t nb, const uint32_t* __restrict__ dn) {
+  if (dn) n = *dn;  // device-side count (graph replays): tiles past n count nothing
   __shared__ uint32_t hist[256];
   hist[threadIdx.x] = 0;
   __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const uint64_t lt = (1ull << lane) - 1ull;
   const size_t base = (size_t)blockIdx.x * kTile + (size_t)w * 64 * kItems;
-#pragma unroll 4
+  const size_t last = n ? n - 1 : 0;
+  // issue every load of the tile before the first use, so the loads overlap
+  K kk[kItems];
+#pragma unroll
+  for (int r = 0; r < kItems; ++r) {
+    const size_t i = base + (size_t)r * 64 + lane;
+    kk[r] = keys[i < last ? i : last];
+  }
+#pragma unroll
   for (int r = 0; r < kItems; ++r) {
     const size_t i = base + (size_t)r * 64 + lane;
     const bool valid = i < n;
-    const uint32_t d = valid ? (uint32_t)(keys[i] >> shift) & mask : 0u;
+    const uint32_t d = valid ? (uint32_t)(kk[r] >> shift) & mask : 0u;
     const uint64_t peers = match_digit(d, __ballot(valid));
     if (valid && (peers & lt) == 0) atomicAdd(&hist[d], (uint32_t)__popcll(peers));
   }
@@ -67,7 +76,10 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(const K* __rest
                                                                  K* __restrict__ kout, V* __restrict__ vout,
                                                                  uint8_t* __restrict__ fout, size_t n, int shift,
                                                                  uint32_t mask, const uint32_t* __restrict__ offsets,
-                                                                 uint32_t nb, int gen_pos) {
+                                                                 uint32_t nb, int gen_pos,
+                                                                 const uint32_t* __restrict__ dn) {
+  if (dn) n = *dn;
+  if ((size_t)blockIdx.x * kTile >= n) return;  // whole tile past the end (block-uniform)
   __shared__ uint32_t wcnt[kWaves][256];
   __shared__ uint32_t goff[256];
   __shared__ uint32_t boff[256];
@@ -84,19 +96,24 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(const K* __rest
   K kk[kItems];
   V vv[kItems];
   uint32_t pk[kItems];  // (rank within wave << 8) | digit
+  const size_t last = n - 1;  // n > 0 here
+  // issue every load of the tile before the first use, so the loads overlap
+#pragma unroll
+  for (int r = 0; r < kItems; ++r) {
+    const size_t i = base + (size_t)r * 64 + lane;
+    const size_t ic = i < last ? i : last;
+    kk[r] = kin[ic];
+    vv[r] = gen_pos ? (V)i : vin[ic];
+  }
 #pragma unroll
   for (int r = 0; r < kItems; ++r) {
     const size_t i = base + (size_t)r * 64 + lane;
     const bool valid = i < n;
-    const K k = valid ? kin[i] : (K)0;
-    V v = (V)0;
-    if (valid) v = gen_pos ? (V)i : vin[i];
+    const K k = kk[r];
     const uint32_t d = (uint32_t)(k >> shift) & mask;
     const uint64_t peers = match_digit(d, __ballot(valid));
     const uint32_t prior = wcnt[w][d];
     if (valid && (peers & lt) == 0) wcnt[w][d] = prior + (uint32_t)__popcll(peers);
-    kk[r] = k;
-    vv[r] = v;
     pk[r] = ((prior + (uint32_t)__popcll(peers & lt)) << 8) | d;
   }
   __syncthreads();
@@ -157,7 +174,7 @@ size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 // One digit pass from (sk, sv) into (dk, dv) [MODE 0] or into the result arrays [MODE 1].
 template <typename K, typename V, int MODE>
-hipError_t radix_pass(const K* sk, const V* sv, K* dk, V* dv, uint8_t* df, size_t n, int shift, int bits,
+hipError_t radix_pass(const K* sk, const V* sv, K* dk, V* dv, uint8_t* df, size_t n, const uint32_t* dn, int shift, int bits,
                       bool gen, void* tmp, hipStream_t s, const char* hist_stage, const char* scatter_stage) {
   const size_t nb = (n + kTile - 1) / kTile;
   uint32_t* counts = reinterpret_cast<uint32_t*>(tmp);
@@ -166,14 +183,14 @@ hipError_t radix_pass(const K* sk, const V* sv, K* dk, V* dv, uint8_t* df, size_
   {
     ProfScope prof(hist_stage, s);
     hipLaunchKernelGGL((radix_hist_kernel<K>), dim3((unsigned)nb), dim3(kThreads), 0, s, sk, n, shift, mask, counts,
-                       (uint32_t)nb);
+                       (uint32_t)nb, dn);
   }
   hipError_t e = exclusive_scan_u32(counts, counts, 256 * nb, 0u, scan_tmp, s);
   if (e != hipSuccess) return e;
   {
     ProfScope prof(scatter_stage, s);
     hipLaunchKernelGGL((radix_scatter_kernel<K, V, MODE>), dim3((unsigned)nb), dim3(kThreads), 0, s, sk, sv, dk, dv,
-                       df, n, shift, mask, (const uint32_t*)counts, (uint32_t)nb, gen ? 1 : 0);
+                       df, n, shift, mask, (const uint32_t*)counts, (uint32_t)nb, gen ? 1 : 0, dn);
   }
   return hipGetLastError();
 }
@@ -181,7 +198,7 @@ hipError_t radix_pass(const K* sk, const V* sv, K* dk, V* dv, uint8_t* df, size_
 template <typename V>
 hipError_t radix_sort_impl(const uint64_t* keys_in, const V* vals_in, size_t n, int begin_bit, int end_bit,
                            uint64_t* k0, V* v0, uint64_t* k1, V* v1, void* tmp, hipStream_t s, bool gen_pos,
-                           int* which) {
+                           int* which, const uint32_t* dn) {
   *which = 0;
   if (n == 0) return hipSuccess;
   if (begin_bit < 0 || end_bit > 64 || begin_bit > end_bit) return hipErrorInvalidValue;
@@ -198,7 +215,7 @@ hipError_t radix_sort_impl(const uint64_t* keys_in, const V* vals_in, size_t n, 
     const int bits = end_bit - shift < 8 ? end_bit - shift : 8;
     uint64_t* dk = out == 0 ? k0 : k1;
     V* dv = out == 0 ? v0 : v1;
-    hipError_t e = radix_pass<uint64_t, V, 0>(sk, sv, dk, dv, nullptr, n, shift, bits, gen, tmp, s, "psa_hist",
+    hipError_t e = radix_pass<uint64_t, V, 0>(sk, sv, dk, dv, nullptr, n, dn, shift, bits, gen, tmp, s, "psa_hist",
                                               "psa_scatter");
     if (e != hipSuccess) return e;
     sk = dk;
@@ -218,19 +235,20 @@ size_t radix_tmp_bytes(size_t n) {
 }
 
 hipError_t radix_sort_pos(const uint64_t* keys_in, size_t n, int begin_bit, int end_bit, uint64_t* k0, uint32_t* v0,
-                          uint64_t* k1, uint32_t* v1, void* tmp, hipStream_t s, int* which) {
-  return radix_sort_impl<uint32_t>(keys_in, nullptr, n, begin_bit, end_bit, k0, v0, k1, v1, tmp, s, true, which);
+                          uint64_t* k1, uint32_t* v1, void* tmp, hipStream_t s, int* which, const uint32_t* dn) {
+  return radix_sort_impl<uint32_t>(keys_in, nullptr, n, begin_bit, end_bit, k0, v0, k1, v1, tmp, s, true, which, dn);
 }
 
 hipError_t radix_sort_u64pay(const uint64_t* keys_in, const uint64_t* vals_in, size_t n, int begin_bit, int end_bit,
                              uint64_t* k0, uint64_t* v0, uint64_t* k1, uint64_t* v1, void* tmp, hipStream_t s,
                              int* which) {
-  return radix_sort_impl<uint64_t>(keys_in, vals_in, n, begin_bit, end_bit, k0, v0, k1, v1, tmp, s, false, which);
+  return radix_sort_impl<uint64_t>(keys_in, vals_in, n, begin_bit, end_bit, k0, v0, k1, v1, tmp, s, false, which,
+                                   nullptr);
 }
 
 hipError_t unpermute_results(const uint32_t* posf, const uint64_t* vals, size_t n, uint32_t* p0, uint64_t* v0,
                              uint32_t* p1, uint64_t* v1, uint64_t* out_vals, uint8_t* out_found, void* tmp,
-                             hipStream_t s) {
+                             hipStream_t s, const uint32_t* dn) {
   if (n == 0) return hipSuccess;
   int bits = 1;
   while (bits < 31 && ((size_t)1 << bits) < n) ++bits;  // positions fit in `bits` bits
@@ -241,12 +259,12 @@ hipError_t unpermute_results(const uint32_t* posf, const uint64_t* vals, size_t 
     const int b = bits - shift < 8 ? bits - shift : 8;
     hipError_t e;
     if (shift + b >= bits) {
-      e = radix_pass<uint32_t, uint64_t, 1>(sk, sv, nullptr, out_vals, out_found, n, shift, b, false, tmp, s,
+      e = radix_pass<uint32_t, uint64_t, 1>(sk, sv, nullptr, out_vals, out_found, n, dn, shift, b, false, tmp, s,
                                             "unpermute_hist", "unpermute_scatter");
     } else {
       uint32_t* dk = out == 0 ? p0 : p1;
       uint64_t* dv = out == 0 ? v0 : v1;
-      e = radix_pass<uint32_t, uint64_t, 0>(sk, sv, dk, dv, nullptr, n, shift, b, false, tmp, s, "unpermute_hist",
+      e = radix_pass<uint32_t, uint64_t, 0>(sk, sv, dk, dv, nullptr, n, dn, shift, b, false, tmp, s, "unpermute_hist",
                                             "unpermute_scatter");
       sk = dk;
       sv = dv;

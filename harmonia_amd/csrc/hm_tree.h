@@ -1,0 +1,94 @@
+// Host-side tree object and helpers shared by the C ABI translation units (hm_api.cpp,
+// hm_batch.cpp, hm_server.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+
+#include "../../include/harmonia.h"
+#include "hm_internal.h"
+
+namespace hm {
+
+int set_err(int code, const char* fmt, ...);
+size_t align256(size_t x);
+bool is_device_ptr(const void* p);
+
+#define HM_TRY(expr)                                                                                       \
+  do {                                                                                                     \
+    hipError_t e_ = (expr);                                                                                \
+    if (e_ != hipSuccess) {                                                                                \
+      int code_ = (e_ == hipErrorOutOfMemory) ? HM_ENOMEM : HM_EDEVICE;                                    \
+      return ::hm::set_err(code_, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+    }                                                                                                      \
+  } while (0)
+
+// Launch configuration of one search path (tuned per tree).
+struct SearchCfg {
+  int group;
+  int queries_per_group;
+  int blocks_per_cu;
+  int output_mode;
+};
+
+struct Workspace {
+  void* buf = nullptr;
+  size_t bytes = 0;
+  hipEvent_t last_use = nullptr;  // the latest search/range issued on this stream
+};
+
+}  // namespace hm
+
+struct hm_tree {
+  hm_params p{};
+  int F = 64, fill = 8, NS = 72;
+  int device = 0;
+  uint64_t n_keys = 0;
+  uint64_t n_leaves = 0, leaf_cap = 0;
+  uint32_t n_inner = 0, inner_cap = 0;
+  int height = 1;
+  uint32_t level_start[hm::MAX_LEVELS + 1] = {0};
+  uint64_t* d_inner = nullptr;
+  uint32_t* d_prefix = nullptr;
+  hm::Leaf* d_leaves = nullptr;
+  int lds_levels = 0;
+  uint64_t layout_version = 0;  // bumped whenever a device array of the layout is reallocated
+  hm::SearchCfg cfg_direct{4, 4, 4, HM_OUT_UNPERMUTE};  // unsorted batches: memory-transaction bound
+  hm::SearchCfg cfg_psa{1, 2, 4, HM_OUT_UNPERMUTE};     // PSA-ordered batches: neighbours share nodes
+  std::mutex mu;
+  std::unordered_map<hipStream_t, hm::Workspace> ws;
+  hipStream_t own = nullptr;
+  hipEvent_t publish_ev = nullptr;
+  bool publish_valid = false;
+  uint64_t epoch = 0;
+  uint64_t* pend_k = nullptr;
+  uint64_t* pend_v = nullptr;
+  size_t pend_n = 0, pend_cap = 0;
+
+  hm::TreeView view() const {
+    hm::TreeView tv{};
+    tv.inner = d_inner;
+    tv.prefix = d_prefix;
+    tv.leaves = d_leaves;
+    tv.n_inner = n_inner;
+    tv.n_leaves = (uint32_t)n_leaves;
+    tv.height = height;
+    tv.fanout = F;
+    for (int i = 0; i <= hm::MAX_LEVELS; ++i) tv.level_start[i] = level_start[i];
+    return tv;
+  }
+  uint64_t device_bytes() const {
+    return (uint64_t)inner_cap * NS * 8 + (uint64_t)inner_cap * 4 + leaf_cap * sizeof(hm::Leaf) + pend_cap * 16;
+  }
+};
+
+namespace hm {
+int ensure_ws(hm_tree* t, hipStream_t s, size_t bytes, Workspace** out);
+int choose_lds_levels(const hm_tree* t, int req);
+// Resolve the launch shape of one search call (tree defaults overridden by opts).
+int resolve_search(const hm_tree* t, uint64_t n, const hm_search_opts* o, SearchLaunch* sl, int* psa_bits,
+                   int* out_mode);
+}  // namespace hm
