@@ -83,6 +83,128 @@ def cpu_baseline(tree, q_host, budget_s=12.0):
                       f"layout, median of {len(times)} runs, OpenMP static over queries"}
 
 
+def batched_uneven(dev, n_batches=10000, keys_log2=25, seed=4):
+    """Uneven batches (sizes log-uniform in [32, 2^20], per-batch distribution uniform / Zipf(0.99) /
+    sorted runs of 4096) on an autotuned 2^25-key tree: one hm_search per batch (the plain path) against
+    the hipGraph-captured Batcher that packs batches into one replay."""
+    import torch
+
+    from harmonia_amd import Batcher, BPlusTree
+    from harmonia_amd import workloads as W
+
+    cseed = W.config_seed(3) + seed
+    keys, vals = W.gen_keys(1 << keys_log2, cseed)
+    sk = np.sort(keys)
+    pool_n = 1 << 24
+    pools = [W.point_queries(sk, pool_n, cseed + 1),
+             W.zipf_queries(sk, pool_n, cseed + 2),
+             W.point_queries(sk, pool_n, cseed + 3, order="runs4096")]
+    sample = np.concatenate([p[: 1 << 19] for p in pools])
+    W.rng(cseed, 7).shuffle(sample)
+    t0 = time.perf_counter()
+    tree, tune = BPlusTree.from_keys_tuned(keys, vals, sample, quick=True)
+    tune_s = time.perf_counter() - t0
+    log("autotune:", json.dumps(tune), f"{tune_s:.1f} s")
+    dpools = [torch.from_numpy(p.view(np.int64)).to(dev) for p in pools]
+    del pools, keys, vals
+    g = W.rng(cseed, 8)
+    sizes = W.log_uniform_sizes(n_batches, 32, 1 << 20, cseed)
+    dist = g.integers(0, 3, n_batches)
+    starts = [int(g.integers(0, pool_n - int(m) + 1)) for m in sizes]
+    total = int(sizes.sum())
+    ov = torch.empty(1 << 20, dtype=torch.int64, device=dev)
+    of = torch.empty(1 << 20, dtype=torch.uint8, device=dev)
+    stream = torch.cuda.Stream()
+    check_idx = set(int(i) for i in g.integers(0, n_batches, 8))
+    plain_out = {}
+    # plain: one search call (its own launches) per batch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(n_batches):
+        m, d, s0 = int(sizes[i]), int(dist[i]), starts[i]
+        tree.search_batch(dpools[d][s0:s0 + m], out_vals=ov[:m], out_found=of[:m], stream=stream)
+        if i in check_idx:
+            stream.synchronize()
+            plain_out[i] = (ov[:m].cpu().numpy().copy(), of[:m].cpu().numpy().copy())
+    stream.synchronize()
+    plain_s = time.perf_counter() - t0
+    # batched: pack into the Batcher, one graph replay per full buffer
+    cap = 1 << 24
+    b = Batcher(tree, cap, stream=stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pending = []
+    fill = 0
+    mismatches = 0
+    for i in range(n_batches + 1):
+        if i == n_batches or fill + int(sizes[i]) > cap:
+            b.flush()
+            for j, off in pending:
+                if j in check_idx:
+                    v, f = b.copy_out(off, int(sizes[j]))
+                    pv, pf = plain_out[j]
+                    mismatches += int(not (np.array_equal(v.view(np.int64), pv) and np.array_equal(f, pf)))
+            pending = []
+            fill = 0
+            if i == n_batches:
+                break
+        m, d, s0 = int(sizes[i]), int(dist[i]), starts[i]
+        pending.append((i, b.add(dpools[d][s0:s0 + m])))
+        fill += m
+    b.sync()
+    batched_s = time.perf_counter() - t0
+    b.close()
+    res = {"workload": f"{n_batches} batches, sizes log-uniform 32..2^20 (total {total} queries), per-batch "
+                       f"uniform / Zipf(0.99) / runs of 4096, 2^{keys_log2}-key tree",
+           "plain_qps": round(total / plain_s, 1), "batched_qps": round(total / batched_s, 1),
+           "speedup": round(plain_s / batched_s, 2), "plain_s": round(plain_s, 4), "batched_s": round(batched_s, 4),
+           "checked_batches": len(check_idx), "mismatches": mismatches, "autotune": tune,
+           "autotune_s": round(tune_s, 1)}
+    tree.close()
+    return res
+
+
+def served(args):
+    """Run the served-load driver (hm_serve_bench_main in harmonia_amd/libhm_servebench.so, called
+    in-process): plain path closed-loop capacity, then the serving loop open-loop at three Poisson
+    rates."""
+    import ctypes
+
+    path = os.path.join(ROOT, "harmonia_amd", "libhm_servebench.so")
+    if not os.path.exists(path):
+        return {"error": "libhm_servebench.so not built"}
+    lib = ctypes.CDLL(path)
+    lib.hm_serve_bench_main.restype = ctypes.c_int
+    lib.hm_serve_bench_main.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), ctypes.c_char_p,
+                                        ctypes.c_uint64]
+    common = ["--keys-log2", str(args.served_keys_log2), "--seed", "5"]
+
+    def run(extra):
+        argv = [b"hm_serve_bench"] + [a.encode() for a in common + extra]
+        arr = (ctypes.c_char_p * len(argv))(*argv)
+        out = ctypes.create_string_buffer(4096)
+        rc = lib.hm_serve_bench_main(len(argv), arr, out, len(out))
+        if not out.value:
+            return {"error": f"rc={rc}"}
+        res = json.loads(out.value.decode().strip())
+        if rc != 0:
+            res["error"] = f"rc={rc}"
+        return res
+
+    plain = run(["--plain", "--closed", "--requests", str(args.served_requests // 2)])
+    log("plain", json.dumps(plain))
+    res = {"plain_closed_loop": plain, "served": []}
+    if "error" in plain:
+        return res
+    base_rps = plain["requests_per_s"]
+    for mult in (1.0, 3.0, 5.0):
+        r = run(["--rate", str(base_rps * mult), "--requests", str(args.served_requests)])
+        r["rate_vs_plain_capacity"] = mult
+        log("served", json.dumps(r))
+        res["served"].append(r)
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +214,10 @@ def main():
     ap.add_argument("--queries-log2", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--variants", default="all", help="'all' or 'headline'")
+    ap.add_argument("--no-batched", action="store_true", help="skip the uneven-batches comparison")
+    ap.add_argument("--no-served", action="store_true", help="skip the served-load runs")
+    ap.add_argument("--served-keys-log2", type=int, default=27)
+    ap.add_argument("--served-requests", type=int, default=20000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the tree is replicated per GPU "
@@ -195,6 +321,20 @@ def main():
         log("cpu baseline ...")
         cpu = cpu_baseline(tree, q_unsorted)
         log("cpu", json.dumps(cpu))
+    tree.close()
+    del dq, out_v, out_f
+    torch.cuda.empty_cache()
+
+    batched = None
+    if not args.no_batched:
+        log("uneven batches ...")
+        batched = batched_uneven(dev)
+        log("batched", json.dumps({k: v for k, v in batched.items() if k != "autotune"}))
+        torch.cuda.empty_cache()
+    serve = None
+    if not args.no_served:
+        log("served load ...")
+        serve = served(args)
 
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "Mqueries/s", "n_gpus": 1, "steps": args.steps,
@@ -209,6 +349,8 @@ def main():
         "variants": results,
         "build_ms": round(build_ms, 1),
         "tree": {"height": st.height, "n_inner": st.n_inner, "n_leaves": st.n_leaves, "lds_levels": st.lds_levels},
+        "batched_uneven": batched,
+        "served_mixed": serve,
     }
     print(json.dumps(line), flush=True)
 

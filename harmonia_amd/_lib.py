@@ -72,6 +72,63 @@ class HmSearchOpts(ctypes.Structure):
     ]
 
 
+class HmTuneResult(ctypes.Structure):
+    _fields_ = [
+        ("fanout", ctypes.c_uint32),
+        ("direct_group", ctypes.c_uint32),
+        ("direct_queries_per_group", ctypes.c_uint32),
+        ("direct_blocks_per_cu", ctypes.c_uint32),
+        ("lds_levels", ctypes.c_int32),
+        ("psa_group", ctypes.c_uint32),
+        ("psa_queries_per_group", ctypes.c_uint32),
+        ("psa_blocks_per_cu", ctypes.c_uint32),
+        ("psa_bits", ctypes.c_uint32),
+        ("output_mode", ctypes.c_uint32),
+        ("sort_threshold", ctypes.c_uint64),
+        ("direct_ms", ctypes.c_double),
+        ("psa_ms", ctypes.c_double),
+        ("best_ms", ctypes.c_double),
+        ("trials", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+HM_TUNE_QUICK = 1
+
+
+class HmServerParams(ctypes.Structure):
+    _fields_ = [
+        ("mem_budget", ctypes.c_uint64),
+        ("max_batch", ctypes.c_uint32),
+        ("max_inflight", ctypes.c_uint32),
+        ("coalesce", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
+class HmServerStats(ctypes.Structure):
+    _fields_ = [
+        ("completed", ctypes.c_uint64),
+        ("lookups", ctypes.c_uint64),
+        ("ranges", ctypes.c_uint64),
+        ("updates", ctypes.c_uint64),
+        ("queries", ctypes.c_uint64),
+        ("rejected", ctypes.c_uint64),
+        ("launches", ctypes.c_uint64),
+        ("epochs", ctypes.c_uint64),
+        ("p50_us", ctypes.c_double),
+        ("p99_us", ctypes.c_double),
+        ("mean_us", ctypes.c_double),
+        ("max_us", ctypes.c_double),
+        ("span_s", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
 # (name, restype, argtypes) for every entry point of include/harmonia.h
 SIGNATURES = [
     ("hm_params_init", ctypes.c_int, [ctypes.POINTER(HmParams)]),
@@ -83,6 +140,25 @@ SIGNATURES = [
     ("hm_range_between", ctypes.c_int, [vp, vp, vp, ctypes.c_uint64, ctypes.c_uint32, vp, vp, vp, vp]),
     ("hm_update", ctypes.c_int, [vp, vp, vp, ctypes.c_uint64, vp]),
     ("hm_epoch_publish", ctypes.c_int, [vp, vp, ctypes.POINTER(ctypes.c_int)]),
+    ("hm_batcher_create", ctypes.c_int, [vp, ctypes.c_uint64, ctypes.POINTER(HmSearchOpts), vp, ctypes.POINTER(vp)]),
+    ("hm_batcher_add", ctypes.c_int, [vp, vp, ctypes.c_uint64, u64p]),
+    ("hm_batcher_flush", ctypes.c_int, [vp]),
+    ("hm_batcher_results", ctypes.c_int, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp)]),
+    ("hm_batcher_copy_out", ctypes.c_int, [vp, ctypes.c_uint64, ctypes.c_uint64, vp, vp]),
+    ("hm_batcher_sync", ctypes.c_int, [vp]),
+    ("hm_batcher_destroy", ctypes.c_int, [vp]),
+    ("hm_server_create", ctypes.c_int, [vp, ctypes.POINTER(HmServerParams), ctypes.POINTER(vp)]),
+    ("hm_server_lookup", ctypes.c_int, [vp, vp, ctypes.c_uint32, vp, vp, u64p]),
+    ("hm_server_range", ctypes.c_int, [vp, vp, vp, ctypes.c_uint32, vp, vp, vp, ctypes.c_uint64, u64p]),
+    ("hm_server_update", ctypes.c_int, [vp, vp, vp, ctypes.c_uint32, u64p]),
+    ("hm_server_wait", ctypes.c_int, [vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int), u64p]),
+    ("hm_server_drain", ctypes.c_int, [vp]),
+    ("hm_server_get_stats", ctypes.c_int, [vp, ctypes.POINTER(HmServerStats)]),
+    ("hm_server_reset_stats", ctypes.c_int, [vp]),
+    ("hm_server_destroy", ctypes.c_int, [vp]),
+    ("hm_autotune", ctypes.c_int, [vp, vp, ctypes.c_uint64, ctypes.c_uint32, ctypes.POINTER(HmTuneResult)]),
+    ("hm_tree_build_tuned", ctypes.c_int, [vp, vp, ctypes.c_uint64, ctypes.POINTER(HmParams), vp, ctypes.c_uint64,
+                                           ctypes.c_uint32, ctypes.POINTER(vp), ctypes.POINTER(HmTuneResult)]),
     ("hm_tree_stats", ctypes.c_int, [vp, ctypes.POINTER(HmStats)]),
     ("hm_tree_export", ctypes.c_int, [vp, vp, vp, vp]),
     ("hm_psa_sort", ctypes.c_int, [vp, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, vp, vp, vp]),

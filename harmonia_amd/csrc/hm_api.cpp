@@ -154,6 +154,79 @@ int choose_lds_levels(const hm_tree* t, int req) {
   return best;
 }
 
+int resolve_search(const hm_tree* t, uint64_t n, const hm_search_opts* o, SearchLaunch* sl, int* psa_bits,
+                   int* out_mode) {
+  int bits = (o && o->psa_bits >= 0) ? o->psa_bits : (int)t->p.psa_bits;
+  if (bits < 0 || bits > 32) return set_err(HM_EINVAL, "psa_bits %d not in 0..32", bits);
+  const bool psa = bits > 0 && (n >= t->p.sort_threshold || (o && o->psa_bits > 0));
+  const SearchCfg& base = psa ? t->cfg_psa : t->cfg_direct;
+  sl->group = (o && o->group_size) ? (int)o->group_size : base.group;
+  if (sl->group != 1 && sl->group != 2 && sl->group != 4 && sl->group != 8)
+    return set_err(HM_EINVAL, "group_size %d not in {1,2,4,8}", sl->group);
+  sl->lds_levels = (o && o->lds_levels >= 0) ? choose_lds_levels(t, o->lds_levels) : t->lds_levels;
+  sl->blocks_per_cu = (o && o->blocks_per_cu) ? (int)o->blocks_per_cu : base.blocks_per_cu;
+  sl->queries_per_group = (o && o->queries_per_group) ? (int)o->queries_per_group : base.queries_per_group;
+  if (sl->queries_per_group != 1 && sl->queries_per_group != 2 && sl->queries_per_group != 4)
+    return set_err(HM_EINVAL, "queries_per_group %d not in {1,2,4}", sl->queries_per_group);
+  int mode = (o && o->output_mode) ? (int)o->output_mode : base.output_mode;
+  if (mode < HM_OUT_UNPERMUTE || mode > HM_OUT_PSA_ORDER)
+    return set_err(HM_EINVAL, "output_mode %d not in 0..3", mode);
+  if (mode == HM_OUT_PSA_ORDER && (!psa || !o || !o->out_pos))
+    return set_err(HM_EINVAL, "PSA-order output needs PSA on and out_pos");
+  *psa_bits = psa ? bits : 0;
+  *out_mode = psa ? mode : 0;
+  return HM_OK;
+}
+
+size_t pipe_bytes(size_t n, int psa_bits, int out_mode) {
+  if (!psa_bits) return 0;
+  const size_t kq = align256(n * 8), kp = align256(n * 4);
+  size_t b = 2 * kq + 2 * kp + align256(radix_tmp_bytes(n));
+  if (out_mode == HM_OUT_UNPERMUTE) b += kq + kp;
+  return b;
+}
+
+PipeBufs carve_pipe(char*& cur, size_t n, int psa_bits, int out_mode) {
+  PipeBufs b{};
+  if (!psa_bits) return b;
+  const size_t kq = align256(n * 8), kp = align256(n * 4);
+  b.k0 = (uint64_t*)cur, cur += kq;
+  b.k1 = (uint64_t*)cur, cur += kq;
+  b.p0 = (uint32_t*)cur, cur += kp;
+  b.p1 = (uint32_t*)cur, cur += kp;
+  b.tmp = cur, cur += align256(radix_tmp_bytes(n));
+  if (out_mode == HM_OUT_UNPERMUTE) {
+    b.rv = (uint64_t*)cur, cur += kq;
+    b.rp = (uint32_t*)cur, cur += kp;
+  }
+  return b;
+}
+
+int enqueue_pipeline(const TreeView& tv, const SearchLaunch& sl, int psa_bits, int out_mode, const uint64_t* dq,
+                     size_t n, const uint32_t* dn, const PipeBufs& b, uint64_t* dvals, uint8_t* dfound,
+                     uint32_t* dpos, hipStream_t s) {
+  if (!psa_bits) {
+    HM_TRY(launch_search(tv, sl, dq, nullptr, dvals, dfound, n, OUT_DIRECT, s, dn));
+    return HM_OK;
+  }
+  int which = 0;
+  HM_TRY(radix_sort_pos(dq, n, 64 - psa_bits, 64, b.k0, b.p0, b.k1, b.p1, b.tmp, s, &which, dn));
+  const uint64_t* sk = which ? b.k1 : b.k0;
+  const uint32_t* sp = which ? b.p1 : b.p0;
+  if (out_mode == HM_OUT_UNPERMUTE) {
+    // search in PSA order writing coalesced (value, pos|found) records, then sort them back by
+    // position; the sort buffers are free once the search has consumed them (stream order)
+    HM_TRY(launch_search(tv, sl, sk, sp, b.rv, (uint8_t*)b.rp, n, OUT_RECORDS, s, dn));
+    HM_TRY(unpermute_results(b.rp, b.rv, n, b.p0, b.k0, b.p1, b.k1, dvals, dfound, b.tmp, s, dn));
+  } else if (out_mode == HM_OUT_PSA_ORDER) {
+    HM_TRY(launch_search(tv, sl, sk, sp, dvals, dfound, n, OUT_DIRECT, s, dn));
+    HM_TRY(hipMemcpyAsync(dpos, sp, n * 4, hipMemcpyDeviceToDevice, s));
+  } else {
+    HM_TRY(launch_search(tv, sl, sk, sp, dvals, dfound, n, OUT_SCATTER, s, dn));
+  }
+  return HM_OK;
+}
+
 }  // namespace hm
 
 namespace {
@@ -226,6 +299,7 @@ int rebuild_inner(hm_tree* t, hipStream_t s) {
   t->n_inner = n_inner;
   for (int i = 0; i <= MAX_LEVELS; ++i) t->level_start[i] = ls[i];
   t->lds_levels = choose_lds_levels(t, t->p.lds_levels);
+  t->layout_version++;  // captured pipelines hold the old pointers and level table
   return HM_OK;
 }
 
@@ -419,85 +493,43 @@ int hm_search_ex(hm_tree* t, const uint64_t* q, uint64_t n, uint64_t* out_vals, 
   hipStream_t s = (hipStream_t)stream;
   std::lock_guard<std::mutex> g(t->mu);
 
-  int bits = (o && o->psa_bits >= 0) ? o->psa_bits : (int)t->p.psa_bits;
-  if (bits < 0 || bits > 32) return set_err(HM_EINVAL, "psa_bits %d not in 0..32", bits);
-  const bool psa = bits > 0 && (n >= t->p.sort_threshold || (o && o->psa_bits > 0));
-  const SearchCfg& base = psa ? t->cfg_psa : t->cfg_direct;
   SearchLaunch sl;
-  sl.group = (o && o->group_size) ? (int)o->group_size : base.group;
-  if (sl.group != 1 && sl.group != 2 && sl.group != 4 && sl.group != 8)
-    return set_err(HM_EINVAL, "group_size %d not in {1,2,4,8}", sl.group);
-  sl.lds_levels = (o && o->lds_levels >= 0) ? choose_lds_levels(t, o->lds_levels) : t->lds_levels;
-  sl.blocks_per_cu = (o && o->blocks_per_cu) ? (int)o->blocks_per_cu : base.blocks_per_cu;
-  sl.queries_per_group = (o && o->queries_per_group) ? (int)o->queries_per_group : base.queries_per_group;
-  if (sl.queries_per_group != 1 && sl.queries_per_group != 2 && sl.queries_per_group != 4)
-    return set_err(HM_EINVAL, "queries_per_group %d not in {1,2,4}", sl.queries_per_group);
-  const int out_mode = (o && o->output_mode) ? (int)o->output_mode : base.output_mode;
-  if (out_mode < HM_OUT_UNPERMUTE || out_mode > HM_OUT_PSA_ORDER)
-    return set_err(HM_EINVAL, "output_mode %d not in 0..3", out_mode);
+  int bits = 0, out_mode = 0;
+  int rc = resolve_search(t, n, o, &sl, &bits, &out_mode);
+  if (rc) return rc;
   const bool psa_order = out_mode == HM_OUT_PSA_ORDER;
-  if (psa_order && (!psa || !o || !o->out_pos))
-    return set_err(HM_EINVAL, "PSA-order output needs PSA on and out_pos");
-  const bool unpermute = psa && out_mode == HM_OUT_UNPERMUTE;
-
   const bool hq = !is_device_ptr(q), hv = !is_device_ptr(out_vals), hf = !is_device_ptr(out_found);
   const bool hp = psa_order && !is_device_ptr(o->out_pos);
-  size_t need = 0;
   const size_t kq = align256(n * 8), kp = align256(n * 4);
+  size_t need = pipe_bytes(n, bits, out_mode);
   if (hq) need += kq;
   if (hv) need += kq;
   if (hf) need += align256(n);
-  if (psa) need += 2 * kq + 2 * kp + align256(radix_tmp_bytes(n));
-  if (unpermute) need += kq + kp;
+  if (hp) need += kp;
   Workspace* w = nullptr;
-  int rc = ensure_ws(t, s, need, &w);
+  rc = ensure_ws(t, s, need, &w);
   if (rc) return rc;
   if (t->publish_valid) HM_TRY(hipStreamWaitEvent(s, t->publish_ev, 0));
   char* cur = (char*)w->buf;
-  auto take = [&](size_t b) {
-    char* r = cur;
-    cur += b;
-    return r;
-  };
   const uint64_t* dq = q;
   uint64_t* dvals = out_vals;
   uint8_t* dfound = out_found;
+  uint32_t* dpos = psa_order ? o->out_pos : nullptr;
   if (hq) {
-    uint64_t* b = (uint64_t*)take(kq);
+    uint64_t* b = (uint64_t*)cur;
+    cur += kq;
     HM_TRY(hipMemcpyAsync(b, q, n * 8, hipMemcpyHostToDevice, s));
     dq = b;
   }
-  if (hv) dvals = (uint64_t*)take(kq);
-  if (hf) dfound = (uint8_t*)take(align256(n));
-  const TreeView tv = t->view();
-  if (psa) {
-    uint64_t* k0 = (uint64_t*)take(kq);
-    uint64_t* k1 = (uint64_t*)take(kq);
-    uint32_t* p0 = (uint32_t*)take(kp);
-    uint32_t* p1 = (uint32_t*)take(kp);
-    void* tmp = take(align256(radix_tmp_bytes(n)));
-    int which = 0;
-    HM_TRY(radix_sort_pos(dq, n, 64 - bits, 64, k0, p0, k1, p1, tmp, s, &which));
-    const uint64_t* sk = which ? k1 : k0;
-    const uint32_t* sp = which ? p1 : p0;
-    if (unpermute) {
-      // search in PSA order writing coalesced (value, pos|found) records, then sort them back by
-      // position; the sort buffers are free once the search has consumed them (stream order)
-      uint64_t* rv = (uint64_t*)take(kq);
-      uint32_t* rp = (uint32_t*)take(kp);
-      HM_TRY(launch_search(tv, sl, sk, sp, rv, (uint8_t*)rp, n, OUT_RECORDS, s));
-      HM_TRY(unpermute_results(rp, rv, n, p0, k0, p1, k1, dvals, dfound, tmp, s));
-    } else if (psa_order) {
-      HM_TRY(launch_search(tv, sl, sk, sp, dvals, dfound, n, OUT_DIRECT, s));
-      HM_TRY(hipMemcpyAsync(o->out_pos, sp, n * 4, hp ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, s));
-    } else {
-      HM_TRY(launch_search(tv, sl, sk, sp, dvals, dfound, n, OUT_SCATTER, s));
-    }
-  } else {
-    HM_TRY(launch_search(tv, sl, dq, nullptr, dvals, dfound, n, OUT_DIRECT, s));
-  }
+  if (hv) dvals = (uint64_t*)cur, cur += kq;
+  if (hf) dfound = (uint8_t*)cur, cur += align256(n);
+  if (hp) dpos = (uint32_t*)cur, cur += kp;
+  PipeBufs pb = carve_pipe(cur, n, bits, out_mode);
+  rc = enqueue_pipeline(t->view(), sl, bits, out_mode, dq, n, nullptr, pb, dvals, dfound, dpos, s);
+  if (rc) return rc;
   if (hv) HM_TRY(hipMemcpyAsync(out_vals, dvals, n * 8, hipMemcpyDeviceToHost, s));
   if (hf) HM_TRY(hipMemcpyAsync(out_found, dfound, n, hipMemcpyDeviceToHost, s));
+  if (hp) HM_TRY(hipMemcpyAsync(o->out_pos, dpos, n * 4, hipMemcpyDeviceToHost, s));
   HM_TRY(hipEventRecord(w->last_use, s));
   if (hq || hv || hf || hp) HM_TRY(hipStreamSynchronize(s));
   return HM_OK;

@@ -1,0 +1,218 @@
+// Batched pipeline: many uneven query batches packed into one buffer, served by one hipGraph
+// replay of sort -> persistent search -> un-permute with the query count read from device memory.
+//
+// The reference drives fixed-size buckets from a host thread over two streams, one
+// H2D -> SortPairs -> kernel -> D2H sequence per bucket (H/regular_bpt/prefix-sort.cu:261-309).
+// Here the graph is captured once per (tree layout, path) and replayed for any packed count up to
+// the capacity; small flushes skip PSA (the tree's sort threshold), large ones sort first.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+
+#include "hm_tree.h"
+
+using namespace hm;
+
+namespace {
+
+struct Graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint64_t version = ~0ull;  // tree layout version it was captured for
+  void reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    version = ~0ull;
+  }
+};
+
+}  // namespace
+
+struct hm_batcher {
+  hm_tree* t = nullptr;
+  uint64_t cap = 0;
+  hipStream_t s = nullptr;
+  bool own_stream = false;
+  hm_search_opts opts{};
+  bool have_opts = false;
+  uint64_t* d_q = nullptr;
+  uint64_t* d_vals = nullptr;
+  uint8_t* d_found = nullptr;
+  uint32_t* d_n = nullptr;
+  void* scratch = nullptr;
+  Graph g_psa, g_direct;
+  uint64_t fill = 0;
+};
+
+namespace {
+
+void free_batcher(hm_batcher* b) {
+  if (!b) return;
+  if (b->s) (void)hipStreamSynchronize(b->s);
+  b->g_psa.reset();
+  b->g_direct.reset();
+  for (void* p : {(void*)b->d_q, (void*)b->d_vals, (void*)b->d_found, (void*)b->d_n, b->scratch})
+    if (p) (void)hipFree(p);
+  if (b->own_stream && b->s) (void)hipStreamDestroy(b->s);
+  delete b;
+}
+
+// Capture the pipeline for `psa` (true: tree's PSA path at full capacity) into g.
+int capture(hm_batcher* b, bool psa, Graph& g) {
+  hm_tree* t = b->t;
+  hm_search_opts o = b->have_opts ? b->opts : hm_search_opts{0, -1, -1, 0, 0, 0, nullptr};
+  if (!psa) o.psa_bits = 0;
+  // force the PSA decision for the captured capacity regardless of the threshold
+  if (psa && o.psa_bits <= 0) o.psa_bits = (int32_t)(t->p.psa_bits ? t->p.psa_bits : 16);
+  SearchLaunch sl;
+  int bits = 0, mode = 0;
+  int rc = resolve_search(t, b->cap, &o, &sl, &bits, &mode);
+  if (rc) return rc;
+  if (mode == HM_OUT_PSA_ORDER) return set_err(HM_EINVAL, "a batcher returns results in input order");
+  char* cur = (char*)b->scratch;
+  PipeBufs pb = carve_pipe(cur, b->cap, bits, mode);
+  g.reset();
+  HM_TRY(hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
+  rc = enqueue_pipeline(t->view(), sl, bits, mode, b->d_q, b->cap, b->d_n, pb, b->d_vals, b->d_found, nullptr,
+                        b->s);
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(b->s, &graph);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  HM_TRY(e);
+  g.graph = graph;
+  HM_TRY(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0));
+  g.version = t->layout_version;
+  return HM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hm_batcher_create(hm_tree* t, uint64_t capacity, const hm_search_opts* opts, void* stream, hm_batcher** out) {
+  if (!t || !out) return set_err(HM_EINVAL, "NULL argument");
+  *out = nullptr;
+  if (capacity == 0 || capacity > (1ull << 31) - 1) return set_err(HM_EINVAL, "capacity must be in 1..2^31-1");
+  if (opts && opts->output_mode == HM_OUT_PSA_ORDER)
+    return set_err(HM_EINVAL, "a batcher returns results in input order");
+  hm_batcher* b = new hm_batcher();
+  b->t = t;
+  b->cap = capacity;
+  if (opts) {
+    b->opts = *opts;
+    b->opts.out_pos = nullptr;
+    b->have_opts = true;
+  }
+  auto fail = [&](int rc) {
+    free_batcher(b);
+    return rc;
+  };
+  if (stream) {
+    b->s = (hipStream_t)stream;
+  } else {
+    // graphs cannot be captured on the null stream: use a private stream
+    if (hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking) != hipSuccess)
+      return fail(set_err(HM_EDEVICE, "hipStreamCreate failed"));
+    b->own_stream = true;
+  }
+  const size_t need = pipe_bytes(capacity, 32, HM_OUT_UNPERMUTE);  // the PSA path needs the most scratch
+  if (t->p.mem_budget && t->device_bytes() + need + capacity * 17 > t->p.mem_budget)
+    return fail(set_err(HM_EBUDGET, "batcher of %llu queries exceeds the device-memory budget",
+                        (unsigned long long)capacity));
+  if (hipMalloc(&b->d_q, capacity * 8) != hipSuccess || hipMalloc(&b->d_vals, capacity * 8) != hipSuccess ||
+      hipMalloc(&b->d_found, capacity) != hipSuccess || hipMalloc(&b->d_n, 256) != hipSuccess ||
+      hipMalloc(&b->scratch, need) != hipSuccess)
+    return fail(set_err(HM_ENOMEM, "batcher allocation failed"));
+  *out = b;
+  return HM_OK;
+}
+
+int hm_batcher_add(hm_batcher* b, const uint64_t* q, uint64_t n, uint64_t* out_offset) {
+  if (!b) return set_err(HM_EINVAL, "batcher is NULL");
+  if (n == 0) {
+    if (out_offset) *out_offset = b->fill;
+    return HM_OK;
+  }
+  if (!q) return set_err(HM_EINVAL, "NULL buffer");
+  if (b->fill + n > b->cap) return set_err(HM_EBUDGET, "batch of %llu does not fit (%llu of %llu used)",
+                                           (unsigned long long)n, (unsigned long long)b->fill,
+                                           (unsigned long long)b->cap);
+  const hipMemcpyKind k = is_device_ptr(q) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  HM_TRY(hipMemcpyAsync(b->d_q + b->fill, q, n * 8, k, b->s));
+  if (out_offset) *out_offset = b->fill;
+  b->fill += n;
+  return HM_OK;
+}
+
+int hm_batcher_flush(hm_batcher* b) {
+  if (!b) return set_err(HM_EINVAL, "batcher is NULL");
+  if (b->fill == 0) return HM_OK;
+  hm_tree* t = b->t;
+  std::lock_guard<std::mutex> g(t->mu);
+  bool psa = b->have_opts && b->opts.psa_bits >= 0 ? b->opts.psa_bits > 0
+                                                   : (t->p.psa_bits > 0 && b->fill >= t->p.sort_threshold);
+  Graph& gr = psa ? b->g_psa : b->g_direct;
+  if (!gr.exec || gr.version != t->layout_version) {
+    int rc = capture(b, psa, gr);
+    if (rc) return rc;
+  }
+  Workspace* w = nullptr;
+  int rc = ensure_ws(t, b->s, 0, &w);  // registers the stream so epoch publishes wait for it
+  if (rc) return rc;
+  if (t->publish_valid) HM_TRY(hipStreamWaitEvent(b->s, t->publish_ev, 0));
+  HM_TRY(launch_set_u32(b->d_n, (uint32_t)b->fill, b->s));
+  {
+    ProfScope prof("batcher_graph", b->s);
+    HM_TRY(hipGraphLaunch(gr.exec, b->s));
+  }
+  HM_TRY(hipEventRecord(w->last_use, b->s));
+  b->fill = 0;
+  return HM_OK;
+}
+
+int hm_batcher_results(hm_batcher* b, const uint64_t** vals, const uint8_t** found) {
+  if (!b) return set_err(HM_EINVAL, "batcher is NULL");
+  if (vals) *vals = b->d_vals;
+  if (found) *found = b->d_found;
+  return HM_OK;
+}
+
+int hm_batcher_copy_out(hm_batcher* b, uint64_t offset, uint64_t n, uint64_t* out_vals, uint8_t* out_found) {
+  if (!b) return set_err(HM_EINVAL, "batcher is NULL");
+  if (offset + n > b->cap) return set_err(HM_EINVAL, "range past the batcher capacity");
+  if (n == 0) return HM_OK;
+  bool host = false;
+  if (out_vals) {
+    const bool d = is_device_ptr(out_vals);
+    host |= !d;
+    HM_TRY(hipMemcpyAsync(out_vals, b->d_vals + offset, n * 8, d ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                          b->s));
+  }
+  if (out_found) {
+    const bool d = is_device_ptr(out_found);
+    host |= !d;
+    HM_TRY(hipMemcpyAsync(out_found, b->d_found + offset, n, d ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                          b->s));
+  }
+  if (host) HM_TRY(hipStreamSynchronize(b->s));
+  return HM_OK;
+}
+
+int hm_batcher_sync(hm_batcher* b) {
+  if (!b) return set_err(HM_EINVAL, "batcher is NULL");
+  HM_TRY(hipStreamSynchronize(b->s));
+  return HM_OK;
+}
+
+int hm_batcher_destroy(hm_batcher* b) {
+  if (!b) return set_err(HM_EINVAL, "batcher is NULL");
+  free_batcher(b);
+  return HM_OK;
+}
+
+}  // extern "C"

@@ -117,6 +117,8 @@ hipError_t launch_search(const TreeView& tv, const SearchLaunch& sl, const uint6
                          const uint32_t* dn = nullptr);
 // Bytes of LDS the top `levels` need.
 size_t lds_bytes_for(const TreeView& tv, int levels);
+// *p = v as a one-thread kernel (a graph-safe way to hand a count to the next replay).
+hipError_t launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s);
 // Locate (leaf, slot, found) per key (update path).
 hipError_t launch_locate(const TreeView& tv, const uint64_t* keys, size_t n, uint32_t* leaf_out, uint32_t* slot_out,
                          uint8_t* found_out, hipStream_t s);

@@ -224,3 +224,16 @@ hipError_t launch_locate(const TreeView& tv, const uint64_t* keys, size_t n, uin
 }
 
 }  // namespace hm
+
+namespace hm {
+namespace {
+__global__ void set_u32_kernel(uint32_t* p, uint32_t v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+}  // namespace
+
+hipError_t launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s) {
+  hipLaunchKernelGGL(set_u32_kernel, dim3(1), dim3(64), 0, s, p, v);
+  return hipGetLastError();
+}
+}  // namespace hm

@@ -1,0 +1,451 @@
+// Serving loop: a device-resident tree behind a request queue with continuous batching, admission
+// under a device-memory budget and update epochs.
+//
+// Worker threads (one HIP stream each) take every pending read request up to the next update,
+// coalesce lookups into one batched search and ranges into one batched scan, run them and write
+// each request's results back.  While one worker's batch is on the GPU the others keep admitting
+// whatever arrived meanwhile, so batches overlap on the device.  Updates act as barriers: reads
+// taken before an update are enqueued before it is published (epoch lock), and the publish orders
+// the device work (hm_epoch_publish waits for earlier searches; later searches wait for it).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <shared_mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "hm_tree.h"
+
+using namespace hm;
+using Clock = std::chrono::steady_clock;
+
+namespace {
+
+enum ReqType { REQ_LOOKUP = 0, REQ_RANGE = 1, REQ_UPDATE = 2 };
+
+struct Request {
+  uint64_t id = 0;
+  int type = REQ_LOOKUP;
+  std::vector<uint64_t> keys;  // lookup keys / range starts / update keys
+  std::vector<uint32_t> lens;  // range lengths
+  std::vector<uint64_t> vals;  // update values
+  uint64_t* out_vals = nullptr;
+  uint8_t* out_found = nullptr;
+  uint64_t* out_keys = nullptr;
+  uint32_t* out_counts = nullptr;
+  uint64_t range_total = 0;  // sum of lens
+  Clock::time_point t_submit;
+  int status = HM_OK;
+  uint64_t epoch = 0;
+  bool done = false;
+};
+using ReqPtr = std::shared_ptr<Request>;
+
+// Grow-only host (pinned) and device buffers of one worker.
+struct Buf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  bool pinned = false;
+  int ensure(size_t b) {
+    if (b <= bytes) return HM_OK;
+    release();
+    size_t want = std::max(b, (size_t)1 << 16);
+    if (pinned) {
+      HM_TRY(hipHostMalloc(&p, want, hipHostMallocDefault));
+    } else {
+      HM_TRY(hipMalloc(&p, want));
+    }
+    bytes = want;
+    return HM_OK;
+  }
+  void release() {
+    if (p) (void)(pinned ? hipHostFree(p) : hipFree(p));
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+struct Worker {
+  hipStream_t s = nullptr;
+  Buf hq{nullptr, 0, true}, hv{nullptr, 0, true}, hf{nullptr, 0, true}, hl{nullptr, 0, true},
+      hc{nullptr, 0, true}, hk{nullptr, 0, true};
+  Buf dq, dv, df, dl, dc, dk, doff;
+  std::thread th;
+};
+
+size_t lookup_bytes(uint64_t n) { return n * 17 + pipe_bytes(n, 32, HM_OUT_UNPERMUTE); }
+size_t range_bytes(uint64_t n, uint64_t total) { return n * 24 + total * 16; }
+size_t request_bytes(const Request& r) {
+  if (r.type == REQ_LOOKUP) return lookup_bytes(r.keys.size());
+  if (r.type == REQ_RANGE) return range_bytes(r.keys.size(), r.range_total);
+  return r.keys.size() * 16;
+}
+
+}  // namespace
+
+struct hm_server {
+  hm_tree* t = nullptr;
+  hm_server_params p{};
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<ReqPtr> pending;
+  std::unordered_map<uint64_t, ReqPtr> live;
+  uint64_t next_id = 1;
+  bool stop = false;
+  uint64_t inflight_bytes = 0;
+  uint64_t outstanding = 0;  // submitted, not yet completed
+  std::shared_mutex epoch_mu;
+  std::vector<std::unique_ptr<Worker>> workers;
+  // statistics
+  std::vector<double> lat_us;
+  uint64_t n_lookups = 0, n_ranges = 0, n_updates = 0, n_queries = 0, n_rejected = 0, n_launches = 0,
+           n_epochs = 0;
+  bool have_first = false;
+  Clock::time_point first_submit, last_complete;
+};
+
+namespace {
+
+void complete(hm_server* sv, const ReqPtr& r, int status, uint64_t epoch) {
+  const auto now = Clock::now();
+  std::lock_guard<std::mutex> g(sv->mu);
+  r->status = status;
+  r->epoch = epoch;
+  r->done = true;
+  sv->lat_us.push_back(std::chrono::duration<double, std::micro>(now - r->t_submit).count());
+  sv->last_complete = now;
+  if (r->type == REQ_LOOKUP) ++sv->n_lookups, sv->n_queries += r->keys.size();
+  if (r->type == REQ_RANGE) ++sv->n_ranges, sv->n_queries += r->keys.size();
+  if (r->type == REQ_UPDATE) ++sv->n_updates;
+  --sv->outstanding;
+  sv->cv_done.notify_all();
+}
+
+uint64_t tree_epoch(hm_tree* t) {
+  hm_stats st;
+  hm_tree_stats(t, &st);
+  return st.epoch;
+}
+
+// Run one coalesced read batch on worker w.  Called with the epoch lock held shared until the
+// device work is enqueued.
+int run_reads(hm_server* sv, Worker& w, std::vector<ReqPtr>& lookups, std::vector<ReqPtr>& ranges,
+              std::shared_lock<std::shared_mutex>& epoch_lock, uint64_t* epoch) {
+  hm_tree* t = sv->t;
+  *epoch = tree_epoch(t);
+  uint64_t nq = 0;
+  for (auto& r : lookups) nq += r->keys.size();
+  uint64_t nr = 0, rtot = 0;
+  for (auto& r : ranges) nr += r->keys.size(), rtot += r->range_total;
+  int rc = HM_OK;
+  if (nq) {
+    if ((rc = w.hq.ensure(nq * 8)) || (rc = w.hv.ensure(nq * 8)) || (rc = w.hf.ensure(nq)) ||
+        (rc = w.dq.ensure(nq * 8)) || (rc = w.dv.ensure(nq * 8)) || (rc = w.df.ensure(nq)))
+      return rc;
+    uint64_t o = 0;
+    for (auto& r : lookups) {
+      std::memcpy((uint64_t*)w.hq.p + o, r->keys.data(), r->keys.size() * 8);
+      o += r->keys.size();
+    }
+    HM_TRY(hipMemcpyAsync(w.dq.p, w.hq.p, nq * 8, hipMemcpyHostToDevice, w.s));
+    if ((rc = hm_search(t, (const uint64_t*)w.dq.p, nq, (uint64_t*)w.dv.p, (uint8_t*)w.df.p, w.s))) return rc;
+    HM_TRY(hipMemcpyAsync(w.hv.p, w.dv.p, nq * 8, hipMemcpyDeviceToHost, w.s));
+    HM_TRY(hipMemcpyAsync(w.hf.p, w.df.p, nq, hipMemcpyDeviceToHost, w.s));
+  }
+  if (nr) {
+    if ((rc = w.hl.ensure(nr * 12)) || (rc = w.hc.ensure(nr * 4)) || (rc = w.hk.ensure(std::max<uint64_t>(rtot, 1) * 16)) ||
+        (rc = w.dl.ensure(nr * 12)) || (rc = w.dc.ensure(nr * 4)) || (rc = w.doff.ensure((nr + 1) * 8)) ||
+        (rc = w.dk.ensure(std::max<uint64_t>(rtot, 1) * 16)))
+      return rc;
+    uint64_t* hs = (uint64_t*)w.hl.p;
+    uint32_t* hlen = (uint32_t*)((char*)w.hl.p + nr * 8);
+    uint64_t o = 0;
+    for (auto& r : ranges) {
+      std::memcpy(hs + o, r->keys.data(), r->keys.size() * 8);
+      std::memcpy(hlen + o, r->lens.data(), r->lens.size() * 4);
+      o += r->keys.size();
+    }
+    HM_TRY(hipMemcpyAsync(w.dl.p, w.hl.p, nr * 12, hipMemcpyHostToDevice, w.s));
+    uint64_t* dkeys = (uint64_t*)w.dk.p;
+    uint64_t* dvals = dkeys + std::max<uint64_t>(rtot, 1);
+    if ((rc = hm_range(t, (const uint64_t*)w.dl.p, (const uint32_t*)((char*)w.dl.p + nr * 8), nr,
+                       (uint64_t*)w.doff.p, dkeys, dvals, (uint32_t*)w.dc.p, rtot, w.s)))
+      return rc;
+    HM_TRY(hipMemcpyAsync(w.hk.p, dkeys, rtot * 8, hipMemcpyDeviceToHost, w.s));
+    HM_TRY(hipMemcpyAsync((uint64_t*)w.hk.p + rtot, dvals, rtot * 8, hipMemcpyDeviceToHost, w.s));
+    HM_TRY(hipMemcpyAsync(w.hc.p, w.dc.p, nr * 4, hipMemcpyDeviceToHost, w.s));
+  }
+  epoch_lock.unlock();  // the device work is ordered against later publishes from here on
+  HM_TRY(hipStreamSynchronize(w.s));
+  // hand results back
+  uint64_t o = 0;
+  for (auto& r : lookups) {
+    const size_t n = r->keys.size();
+    std::memcpy(r->out_vals, (uint64_t*)w.hv.p + o, n * 8);
+    std::memcpy(r->out_found, (uint8_t*)w.hf.p + o, n);
+    o += n;
+  }
+  uint64_t ro = 0, rk = 0;
+  for (auto& r : ranges) {
+    const size_t n = r->keys.size();
+    std::memcpy(r->out_counts, (uint32_t*)w.hc.p + ro, n * 4);
+    std::memcpy(r->out_keys, (uint64_t*)w.hk.p + rk, r->range_total * 8);
+    std::memcpy(r->out_vals, (uint64_t*)w.hk.p + rtot + rk, r->range_total * 8);
+    ro += n;
+    rk += r->range_total;
+  }
+  return HM_OK;
+}
+
+int run_updates(hm_server* sv, Worker& w, std::vector<ReqPtr>& ups, uint64_t* epoch) {
+  hm_tree* t = sv->t;
+  std::unique_lock<std::shared_mutex> ex(sv->epoch_mu);  // reads taken earlier are enqueued by now
+  for (auto& r : ups) {
+    int rc = hm_update(t, r->keys.data(), r->vals.data(), r->keys.size(), w.s);
+    if (rc) return rc;
+  }
+  int rebuilt = 0;
+  int rc = hm_epoch_publish(t, w.s, &rebuilt);
+  if (rc) return rc;
+  HM_TRY(hipStreamSynchronize(w.s));
+  *epoch = tree_epoch(t);
+  return HM_OK;
+}
+
+void worker_loop(hm_server* sv, Worker* w) {
+  (void)hipSetDevice(sv->t->device);
+  for (;;) {
+    std::vector<ReqPtr> lookups, ranges, ups;
+    uint64_t bytes = 0;
+    std::shared_lock<std::shared_mutex> epoch_lock(sv->epoch_mu, std::defer_lock);
+    {
+      std::unique_lock<std::mutex> lk(sv->mu);
+      sv->cv_work.wait(lk, [&] {
+        if (sv->stop) return true;
+        if (sv->pending.empty()) return false;
+        const ReqPtr& h = sv->pending.front();
+        if (h->type == REQ_UPDATE) return true;
+        return sv->inflight_bytes == 0 || sv->inflight_bytes + request_bytes(*h) <= sv->p.mem_budget;
+      });
+      if (sv->stop && sv->pending.empty()) return;
+      if (sv->pending.front()->type == REQ_UPDATE) {
+        while (!sv->pending.empty() && sv->pending.front()->type == REQ_UPDATE) {
+          ups.push_back(sv->pending.front());
+          sv->pending.pop_front();
+          if (!sv->p.coalesce) break;
+        }
+      } else {
+        uint64_t nq = 0;
+        while (!sv->pending.empty() && sv->pending.front()->type != REQ_UPDATE) {
+          const ReqPtr& h = sv->pending.front();
+          const uint64_t b = request_bytes(*h);
+          const uint64_t m = h->keys.size() + h->range_total;
+          if (!lookups.empty() || !ranges.empty()) {
+            if (!sv->p.coalesce) break;
+            if (nq + m > sv->p.max_batch) break;
+            if (sv->inflight_bytes + bytes + b > sv->p.mem_budget) break;
+          }
+          (h->type == REQ_LOOKUP ? lookups : ranges).push_back(h);
+          nq += m;
+          bytes += b;
+          sv->pending.pop_front();
+        }
+        sv->inflight_bytes += bytes;
+        epoch_lock.lock();  // before mu is released: an update behind these reads waits for them
+      }
+      ++sv->n_launches;
+      if (!sv->pending.empty()) sv->cv_work.notify_one();
+    }
+    uint64_t epoch = 0;
+    if (!ups.empty()) {
+      const int rc = run_updates(sv, *w, ups, &epoch);
+      {
+        std::lock_guard<std::mutex> g(sv->mu);
+        ++sv->n_epochs;
+      }
+      for (auto& r : ups) complete(sv, r, rc, epoch);
+    } else {
+      const int rc = run_reads(sv, *w, lookups, ranges, epoch_lock, &epoch);
+      if (epoch_lock.owns_lock()) epoch_lock.unlock();
+      {
+        std::lock_guard<std::mutex> g(sv->mu);
+        sv->inflight_bytes -= bytes;
+      }
+      sv->cv_work.notify_all();
+      for (auto& r : lookups) complete(sv, r, rc, epoch);
+      for (auto& r : ranges) complete(sv, r, rc, epoch);
+    }
+  }
+}
+
+int submit(hm_server* sv, ReqPtr r, uint64_t* ticket) {
+  r->t_submit = Clock::now();
+  std::lock_guard<std::mutex> g(sv->mu);
+  if (sv->stop) return set_err(HM_EINVAL, "server is shutting down");
+  if (!sv->have_first) {
+    sv->first_submit = r->t_submit;
+    sv->have_first = true;
+  }
+  r->id = sv->next_id++;
+  if (r->type != REQ_UPDATE && request_bytes(*r) > sv->p.mem_budget) {
+    ++sv->n_rejected;
+    return set_err(HM_EBUDGET, "request needs %llu device bytes, budget is %llu",
+                   (unsigned long long)request_bytes(*r), (unsigned long long)sv->p.mem_budget);
+  }
+  sv->live[r->id] = r;
+  sv->pending.push_back(r);
+  ++sv->outstanding;
+  if (ticket) *ticket = r->id;
+  sv->cv_work.notify_one();
+  return HM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hm_server_create(hm_tree* t, const hm_server_params* p, hm_server** out) {
+  if (!t || !out) return set_err(HM_EINVAL, "NULL argument");
+  *out = nullptr;
+  hm_server* sv = new hm_server();
+  sv->t = t;
+  if (p) sv->p = *p;
+  if (!sv->p.mem_budget) sv->p.mem_budget = 1ull << 30;
+  if (!sv->p.max_batch) sv->p.max_batch = 1u << 22;
+  if (!sv->p.max_inflight) sv->p.max_inflight = 4;
+  if (sv->p.max_inflight > 16) sv->p.max_inflight = 16;
+  for (uint32_t i = 0; i < sv->p.max_inflight; ++i) {
+    auto w = std::make_unique<Worker>();
+    if (hipStreamCreateWithFlags(&w->s, hipStreamNonBlocking) != hipSuccess) {
+      for (auto& x : sv->workers) (void)hipStreamDestroy(x->s);
+      delete sv;
+      return set_err(HM_EDEVICE, "hipStreamCreate failed");
+    }
+    sv->workers.push_back(std::move(w));
+  }
+  for (auto& w : sv->workers) w->th = std::thread(worker_loop, sv, w.get());
+  *out = sv;
+  return HM_OK;
+}
+
+int hm_server_lookup(hm_server* s, const uint64_t* keys, uint32_t n, uint64_t* out_vals, uint8_t* out_found,
+                     uint64_t* ticket) {
+  if (!s || (n && (!keys || !out_vals || !out_found))) return set_err(HM_EINVAL, "NULL argument");
+  auto r = std::make_shared<Request>();
+  r->type = REQ_LOOKUP;
+  r->keys.assign(keys, keys + n);
+  r->out_vals = out_vals;
+  r->out_found = out_found;
+  return submit(s, r, ticket);
+}
+
+int hm_server_range(hm_server* s, const uint64_t* starts, const uint32_t* lens, uint32_t n, uint64_t* out_keys,
+                    uint64_t* out_vals, uint32_t* out_counts, uint64_t capacity, uint64_t* ticket) {
+  if (!s || (n && (!starts || !lens || !out_keys || !out_vals || !out_counts)))
+    return set_err(HM_EINVAL, "NULL argument");
+  auto r = std::make_shared<Request>();
+  r->type = REQ_RANGE;
+  r->keys.assign(starts, starts + n);
+  r->lens.assign(lens, lens + n);
+  for (uint32_t i = 0; i < n; ++i) r->range_total += lens[i];
+  if (r->range_total > capacity) return set_err(HM_EINVAL, "capacity %llu < sum of lengths %llu",
+                                                (unsigned long long)capacity, (unsigned long long)r->range_total);
+  r->out_keys = out_keys;
+  r->out_vals = out_vals;
+  r->out_counts = out_counts;
+  return submit(s, r, ticket);
+}
+
+int hm_server_update(hm_server* s, const uint64_t* keys, const uint64_t* vals, uint32_t n, uint64_t* ticket) {
+  if (!s || (n && (!keys || !vals))) return set_err(HM_EINVAL, "NULL argument");
+  auto r = std::make_shared<Request>();
+  r->type = REQ_UPDATE;
+  r->keys.assign(keys, keys + n);
+  r->vals.assign(vals, vals + n);
+  return submit(s, r, ticket);
+}
+
+int hm_server_wait(hm_server* s, uint64_t ticket, int* status, uint64_t* epoch) {
+  if (!s) return set_err(HM_EINVAL, "server is NULL");
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->live.find(ticket);
+  if (it == s->live.end()) return set_err(HM_EINVAL, "unknown ticket %llu", (unsigned long long)ticket);
+  ReqPtr r = it->second;
+  s->cv_done.wait(lk, [&] { return r->done; });
+  if (status) *status = r->status;
+  if (epoch) *epoch = r->epoch;
+  s->live.erase(ticket);
+  return HM_OK;
+}
+
+int hm_server_drain(hm_server* s) {
+  if (!s) return set_err(HM_EINVAL, "server is NULL");
+  std::unique_lock<std::mutex> lk(s->mu);
+  s->cv_done.wait(lk, [&] { return s->outstanding == 0; });
+  return HM_OK;
+}
+
+int hm_server_get_stats(hm_server* s, hm_server_stats* st) {
+  if (!s || !st) return set_err(HM_EINVAL, "NULL argument");
+  std::lock_guard<std::mutex> g(s->mu);
+  std::memset(st, 0, sizeof(*st));
+  st->completed = s->lat_us.size();
+  st->lookups = s->n_lookups;
+  st->ranges = s->n_ranges;
+  st->updates = s->n_updates;
+  st->queries = s->n_queries;
+  st->rejected = s->n_rejected;
+  st->launches = s->n_launches;
+  st->epochs = s->n_epochs;
+  if (!s->lat_us.empty()) {
+    std::vector<double> v = s->lat_us;
+    std::sort(v.begin(), v.end());
+    auto pct = [&](double q) { return v[std::min(v.size() - 1, (size_t)(q * (double)(v.size() - 1) + 0.5))]; };
+    st->p50_us = pct(0.50);
+    st->p99_us = pct(0.99);
+    double sum = 0;
+    for (double x : v) sum += x;
+    st->mean_us = sum / (double)v.size();
+    st->max_us = v.back();
+    st->span_s = std::chrono::duration<double>(s->last_complete - s->first_submit).count();
+  }
+  return HM_OK;
+}
+
+int hm_server_reset_stats(hm_server* s) {
+  if (!s) return set_err(HM_EINVAL, "server is NULL");
+  std::lock_guard<std::mutex> g(s->mu);
+  s->lat_us.clear();
+  s->n_lookups = s->n_ranges = s->n_updates = s->n_queries = s->n_rejected = s->n_launches = s->n_epochs = 0;
+  s->have_first = false;
+  return HM_OK;
+}
+
+int hm_server_destroy(hm_server* s) {
+  if (!s) return set_err(HM_EINVAL, "server is NULL");
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    s->stop = true;
+  }
+  s->cv_work.notify_all();
+  for (auto& w : s->workers)
+    if (w->th.joinable()) w->th.join();
+  for (auto& w : s->workers) {
+    (void)hipStreamSynchronize(w->s);
+    for (Buf* b : {&w->hq, &w->hv, &w->hf, &w->hl, &w->hc, &w->hk, &w->dq, &w->dv, &w->df, &w->dl, &w->dc, &w->dk,
+                   &w->doff})
+      b->release();
+    (void)hipStreamDestroy(w->s);
+  }
+  delete s;
+  return HM_OK;
+}
+
+}  // extern "C"

@@ -88,7 +88,25 @@ struct hm_tree {
 namespace hm {
 int ensure_ws(hm_tree* t, hipStream_t s, size_t bytes, Workspace** out);
 int choose_lds_levels(const hm_tree* t, int req);
-// Resolve the launch shape of one search call (tree defaults overridden by opts).
+// Resolve the launch shape of one search call (tree defaults overridden by opts); psa_bits = 0
+// means no PSA (then out_mode = 0, results in input order).
 int resolve_search(const hm_tree* t, uint64_t n, const hm_search_opts* o, SearchLaunch* sl, int* psa_bits,
                    int* out_mode);
+
+// Device scratch of one sort -> search -> (un-permute) pipeline over up to n queries.
+struct PipeBufs {
+  uint64_t *k0, *k1;  // PSA-sorted keys (ping-pong); reused by the un-permute as value buffers
+  uint32_t *p0, *p1;  // positions (ping-pong)
+  void* tmp;          // radix counts + scan scratch
+  uint64_t* rv;       // PSA-order result values (un-permute input)
+  uint32_t* rp;       // PSA-order pos | found << 31 (un-permute input)
+};
+size_t pipe_bytes(size_t n, int psa_bits, int out_mode);
+PipeBufs carve_pipe(char*& cur, size_t n, int psa_bits, int out_mode);
+// Enqueue sort -> search -> un-permute on s.  n sizes every launch; dn (device) may hold a smaller
+// actual count, so the same enqueue can be captured once into a graph and replayed.  The PSA-order
+// mode copies positions to dpos with a device-to-device copy of n entries.
+int enqueue_pipeline(const TreeView& tv, const SearchLaunch& sl, int psa_bits, int out_mode, const uint64_t* dq,
+                     size_t n, const uint32_t* dn, const PipeBufs& b, uint64_t* dvals, uint8_t* dfound,
+                     uint32_t* dpos, hipStream_t s);
 }  // namespace hm

@@ -206,6 +206,29 @@ class BPlusTree:
         self.update(keys, vals)
         return self.publish()
 
+    # ---- tuning ---------------------------------------------------------------------------
+    def autotune(self, sample, quick=False):
+        """Time launch shapes on a sample batch and make the winners the tree's defaults."""
+        sample = _u64(sample)
+        r = _lib.HmTuneResult()
+        check(self._lib.hm_autotune(self._handle(), _ptr(sample), _len(sample),
+                                    _lib.HM_TUNE_QUICK if quick else 0, ctypes.byref(r)))
+        return r.as_dict()
+
+    @classmethod
+    def from_keys_tuned(cls, keys, vals, sample, quick=False, **params):
+        """Build with fanout 32, 64 and 128, autotune each on the sample, keep the fastest."""
+        t = cls(**params)
+        keys, vals, sample = _u64(keys), _u64(vals), _u64(sample)
+        h = ctypes.c_void_p()
+        r = _lib.HmTuneResult()
+        check(t._lib.hm_tree_build_tuned(_ptr(keys), _ptr(vals), _len(keys), ctypes.byref(t.params), _ptr(sample),
+                                         _len(sample), _lib.HM_TUNE_QUICK if quick else 0, ctypes.byref(h),
+                                         ctypes.byref(r)))
+        t._h = h
+        t.params.fanout = r.fanout
+        return t, r.as_dict()
+
     # ---- introspection --------------------------------------------------------------------
     def stats(self):
         s = HmStats()
@@ -240,3 +263,133 @@ def psa_sort(keys, begin_bit=48, end_bit=64, stream=None):
     po = np.empty(n, dtype=np.uint32)
     check(lib.hm_psa_sort(_ptr(keys), n, begin_bit, end_bit, _ptr(ko), _ptr(po), _stream(stream)))
     return ko, po
+
+
+class Batcher:
+    """Many uneven query batches served by one hipGraph replay (hm_batcher_*).
+
+    add() packs a batch into the batcher's device buffer and returns its offset; flush() launches
+    PSA sort -> one persistent search launch -> un-permute over everything added; copy_out() reads
+    one batch's results (or results() gives the device pointers)."""
+
+    def __init__(self, tree, capacity, stream=None, group_size=0, lds_levels=-1, psa_bits=-1, blocks_per_cu=0,
+                 queries_per_group=0, output_mode=0):
+        self._lib = _lib.load()
+        self.tree = tree
+        self.capacity = int(capacity)
+        o = HmSearchOpts(group_size, lds_levels, psa_bits, blocks_per_cu, output_mode, queries_per_group, None)
+        h = ctypes.c_void_p()
+        check(self._lib.hm_batcher_create(tree._handle(), self.capacity, ctypes.byref(o), _stream(stream),
+                                          ctypes.byref(h)))
+        self._h = h
+
+    def add(self, q):
+        q = _u64(q)
+        off = ctypes.c_uint64(0)
+        check(self._lib.hm_batcher_add(self._h, _ptr(q), _len(q), ctypes.byref(off)))
+        return int(off.value)
+
+    def flush(self):
+        check(self._lib.hm_batcher_flush(self._h))
+
+    def results(self):
+        v, f = ctypes.c_void_p(), ctypes.c_void_p()
+        check(self._lib.hm_batcher_results(self._h, ctypes.byref(v), ctypes.byref(f)))
+        return v.value, f.value
+
+    def copy_out(self, offset, n, out_vals=None, out_found=None):
+        if out_vals is None:
+            out_vals = np.empty(n, dtype=np.uint64)
+        if out_found is None:
+            out_found = np.empty(n, dtype=np.uint8)
+        check(self._lib.hm_batcher_copy_out(self._h, offset, n, _ptr(out_vals), _ptr(out_found)))
+        return out_vals, out_found
+
+    def sync(self):
+        check(self._lib.hm_batcher_sync(self._h))
+
+    def close(self):
+        if self._h is not None:
+            self._lib.hm_batcher_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """The serving loop (hm_server_*): requests are admitted as they arrive, coalesced into batches
+    while other batches run, and updates are applied between read epochs."""
+
+    def __init__(self, tree, mem_budget=0, max_batch=0, max_inflight=0, coalesce=True):
+        self._lib = _lib.load()
+        self.tree = tree
+        p = _lib.HmServerParams(mem_budget, max_batch, max_inflight, 1 if coalesce else 0, 0)
+        h = ctypes.c_void_p()
+        check(self._lib.hm_server_create(tree._handle(), ctypes.byref(p), ctypes.byref(h)))
+        self._h = h
+        self._outs = {}
+
+    def lookup(self, keys):
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        v = np.empty(len(keys), dtype=np.uint64)
+        f = np.empty(len(keys), dtype=np.uint8)
+        t = ctypes.c_uint64(0)
+        check(self._lib.hm_server_lookup(self._h, _ptr(keys), len(keys), _ptr(v), _ptr(f), ctypes.byref(t)))
+        self._outs[t.value] = (v, f)
+        return t.value
+
+    def range(self, starts, lens):
+        starts = np.ascontiguousarray(starts, dtype=np.uint64)
+        lens = np.ascontiguousarray(lens, dtype=np.uint32)
+        cap = int(lens.astype(np.uint64).sum())
+        k = np.empty(max(cap, 1), dtype=np.uint64)
+        v = np.empty(max(cap, 1), dtype=np.uint64)
+        c = np.empty(len(starts), dtype=np.uint32)
+        t = ctypes.c_uint64(0)
+        check(self._lib.hm_server_range(self._h, _ptr(starts), _ptr(lens), len(starts), _ptr(k), _ptr(v), _ptr(c),
+                                         cap, ctypes.byref(t)))
+        off = np.zeros(len(lens) + 1, dtype=np.uint64)
+        off[1:] = np.cumsum(lens.astype(np.uint64))
+        self._outs[t.value] = (off, k, v, c)
+        return t.value
+
+    def update(self, keys, vals):
+        keys = np.ascontiguousarray(keys, dtype=np.uint64)
+        vals = np.ascontiguousarray(vals, dtype=np.uint64)
+        t = ctypes.c_uint64(0)
+        check(self._lib.hm_server_update(self._h, _ptr(keys), _ptr(vals), len(keys), ctypes.byref(t)))
+        self._outs[t.value] = ()
+        return t.value
+
+    def wait(self, ticket):
+        """Block until the request is done; returns (status, epoch, outputs)."""
+        st = ctypes.c_int(0)
+        ep = ctypes.c_uint64(0)
+        check(self._lib.hm_server_wait(self._h, ticket, ctypes.byref(st), ctypes.byref(ep)))
+        return st.value, ep.value, self._outs.pop(ticket)
+
+    def drain(self):
+        check(self._lib.hm_server_drain(self._h))
+
+    def stats(self):
+        s = _lib.HmServerStats()
+        check(self._lib.hm_server_get_stats(self._h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def reset_stats(self):
+        check(self._lib.hm_server_reset_stats(self._h))
+
+    def close(self):
+        if self._h is not None:
+            self._lib.hm_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

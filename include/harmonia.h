@@ -130,6 +130,84 @@ int hm_update(hm_tree* t, const uint64_t* keys, const uint64_t* vals, uint64_t n
  * whether leaves had to be re-laid-out (leaf overflow) and the prefix array re-scanned. */
 int hm_epoch_publish(hm_tree* t, void* stream, int* out_rebuilt);
 
+/* Batched pipeline for many uneven query batches.  Batches are packed back to back into the
+ * batcher's device buffer (hm_batcher_add); hm_batcher_flush then serves all of them with one
+ * hipGraph launch: PSA radix sort -> one persistent search launch -> un-permute, every kernel reading
+ * the packed query count from device memory, so the graph is captured once and replayed for any
+ * count <= capacity.  Batch i's results sit at its offset in the batcher's result buffers.
+ * Replaces the reference's fixed-bucket, two-stream host loop (H/regular_bpt/prefix-sort.cu:261-309,
+ * H/src/full-multi-prefix-sort-8thread.cu:276-327).  opts (may be NULL) fixes the launch shape;
+ * output_mode may be HM_OUT_UNPERMUTE or HM_OUT_SCATTER.  A batcher is used from one thread. */
+typedef struct hm_batcher hm_batcher;
+int hm_batcher_create(hm_tree* t, uint64_t capacity, const hm_search_opts* opts, void* stream, hm_batcher** out);
+/* Copy one batch in (host or device pointer); *out_offset = its first slot. HM_EBUDGET if full. */
+int hm_batcher_add(hm_batcher* b, const uint64_t* q, uint64_t n, uint64_t* out_offset);
+/* Launch the pipeline over everything added since the last flush (asynchronous). */
+int hm_batcher_flush(hm_batcher* b);
+/* Device pointers of the result buffers (valid in stream order after a flush). */
+int hm_batcher_results(hm_batcher* b, const uint64_t** vals, const uint8_t** found);
+int hm_batcher_copy_out(hm_batcher* b, uint64_t offset, uint64_t n, uint64_t* out_vals, uint8_t* out_found);
+int hm_batcher_sync(hm_batcher* b);
+int hm_batcher_destroy(hm_batcher* b);
+
+/* Serving loop.  The tree stays resident; lookup, range and update requests are admitted as they
+ * arrive.  Worker threads (one HIP stream each, max_inflight of them) repeatedly take every pending
+ * read request up to the next update (continuous batching: whatever arrived while the other workers'
+ * batches were on the GPU), coalesce the lookups into one batched search and the ranges into one
+ * batched scan, subject to the device-memory budget for in-flight batches; updates are applied
+ * between read epochs (hm_epoch_publish), so a read sees the tree before or after an update, never
+ * in between.  The reference has no serving layer (SURVEY: one driver per bucket, no latency). */
+typedef struct hm_server hm_server;
+typedef struct hm_server_params {
+  uint64_t mem_budget;   /* device bytes for in-flight request batches (0 = 1 GiB) */
+  uint32_t max_batch;    /* queries (or range keys) coalesced into one launch (0 = 1 << 22) */
+  uint32_t max_inflight; /* batches on the GPU at once, one worker + stream each (0 = 4) */
+  uint32_t coalesce;     /* 1 = continuous batching; 0 = one request per launch (the plain path) */
+  uint32_t reserved;
+} hm_server_params;
+typedef struct hm_server_stats {
+  uint64_t completed, lookups, ranges, updates, queries, rejected, launches, epochs;
+  double p50_us, p99_us, mean_us, max_us; /* request latency, submit -> results written */
+  double span_s;                          /* first submit -> last completion */
+} hm_server_stats;
+int hm_server_create(hm_tree* t, const hm_server_params* p, hm_server** out);
+/* Requests: inputs are copied at submission; outputs (host memory) are written before completion. */
+int hm_server_lookup(hm_server* s, const uint64_t* keys, uint32_t n, uint64_t* out_vals, uint8_t* out_found,
+                     uint64_t* ticket);
+int hm_server_range(hm_server* s, const uint64_t* starts, const uint32_t* lens, uint32_t n, uint64_t* out_keys,
+                    uint64_t* out_vals, uint32_t* out_counts, uint64_t capacity, uint64_t* ticket);
+int hm_server_update(hm_server* s, const uint64_t* keys, const uint64_t* vals, uint32_t n, uint64_t* ticket);
+/* Block until the request completes; *status = its HM_* result, *epoch = the tree epoch it saw. */
+int hm_server_wait(hm_server* s, uint64_t ticket, int* status, uint64_t* epoch);
+int hm_server_drain(hm_server* s);
+int hm_server_get_stats(hm_server* s, hm_server_stats* st);
+int hm_server_reset_stats(hm_server* s);
+int hm_server_destroy(hm_server* s);
+
+/* Autotuner.  The reference fixes fanout / narrowing / PSA bits at compile time
+ * (H/CMakeLists.txt:55-56, H/regular_bpt/prefix-sort.cu:288-293) and the paper sizes the thread
+ * group with a static model (Harmonia.pdf p.140-141); here candidates are timed on the device with a
+ * sample batch: lanes per query {1,2,4,8} x queries per group {1,2,4} x blocks per CU {4,8} x
+ * LDS-resident levels for the unsorted path; PSA bits {12,16,20,24} x output mode x shape for the
+ * sorted path; then the sort threshold (smallest power-of-two batch at which PSA wins).  The winners
+ * become the tree's defaults. */
+#define HM_TUNE_QUICK 1u /* fewer timing repetitions */
+typedef struct hm_tune_result {
+  uint32_t fanout;
+  uint32_t direct_group, direct_queries_per_group, direct_blocks_per_cu;
+  int32_t lds_levels;
+  uint32_t psa_group, psa_queries_per_group, psa_blocks_per_cu;
+  uint32_t psa_bits, output_mode;
+  uint64_t sort_threshold; /* UINT64_MAX: PSA never won up to the sample size */
+  double direct_ms, psa_ms, best_ms; /* per call at the sample size */
+  uint32_t trials;
+  uint32_t reserved;
+} hm_tune_result;
+int hm_autotune(hm_tree* t, const uint64_t* sample, uint64_t n, uint32_t flags, hm_tune_result* out);
+/* Build with fanout 32, 64 and 128, autotune each on the sample, keep the fastest. */
+int hm_tree_build_tuned(const uint64_t* keys, const uint64_t* vals, uint64_t n, const hm_params* p,
+                        const uint64_t* sample, uint64_t m, uint32_t flags, hm_tree** out, hm_tune_result* res);
+
 int hm_tree_stats(const hm_tree* t, hm_stats* s);
 
 /* Copy the device layout to host buffers (any may be NULL): inner (n_inner * node_slots u64),
