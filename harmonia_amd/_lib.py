@@ -57,6 +57,7 @@ class HmStats(ctypes.Structure):
         ("bytes_leaves", ctypes.c_uint64),
         ("bytes_device", ctypes.c_uint64),
         ("epoch", ctypes.c_uint64),
+        ("relayouts", ctypes.c_uint64),
     ]
 
 
@@ -161,6 +162,11 @@ SIGNATURES = [
                                            ctypes.c_uint32, ctypes.POINTER(vp), ctypes.POINTER(HmTuneResult)]),
     ("hm_tree_stats", ctypes.c_int, [vp, ctypes.POINTER(HmStats)]),
     ("hm_tree_export", ctypes.c_int, [vp, vp, vp, vp]),
+    ("hm_read_keys_text", ctypes.c_int, [ctypes.c_char_p, vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
+    ("hm_read_ranges_text", ctypes.c_int,
+     [ctypes.c_char_p, vp, vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
+    ("hm_tree_save", ctypes.c_int, [vp, ctypes.c_char_p]),
+    ("hm_tree_load", ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(HmParams), ctypes.POINTER(vp)]),
     ("hm_psa_sort", ctypes.c_int, [vp, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, vp, vp, vp]),
     ("hm_profile_enable", ctypes.c_int, [ctypes.c_int]),
     ("hm_profile_reset", ctypes.c_int, []),

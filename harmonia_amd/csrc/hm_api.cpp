@@ -127,13 +127,15 @@ int ensure_ws(hm_tree* t, hipStream_t s, size_t bytes, Workspace** out) {
   Workspace& w = t->ws[s];
   if (!w.last_use) HM_TRY(hipEventCreateWithFlags(&w.last_use, hipEventDisableTiming));
   if (w.bytes < bytes) {
+    const size_t old = w.bytes;
     if (w.buf) {
       HM_TRY(hipStreamSynchronize(s));
       HM_TRY(hipFree(w.buf));
       w.buf = nullptr;
       w.bytes = 0;
     }
-    size_t want = std::max(bytes, (size_t)1 << 20);
+    // geometric growth: a hipFree synchronizes the whole device
+    size_t want = std::max(std::max(bytes, 2 * old), (size_t)1 << 20);
     HM_TRY(hipMalloc(&w.buf, want));
     w.bytes = want;
   }
@@ -180,24 +182,31 @@ int resolve_search(const hm_tree* t, uint64_t n, const hm_search_opts* o, Search
 
 size_t pipe_bytes(size_t n, int psa_bits, int out_mode) {
   if (!psa_bits) return 0;
-  const size_t kq = align256(n * 8), kp = align256(n * 4);
-  size_t b = 2 * kq + 2 * kp + align256(radix_tmp_bytes(n));
-  if (out_mode == HM_OUT_UNPERMUTE) b += kq + kp;
+  const size_t kq = align256(n * 8), kp = align256(n * 4), kf = align256(n);
+  size_t b = 2 * kq + align256(radix_tmp_bytes(n));
+  if (out_mode == HM_OUT_UNPERMUTE)
+    b += align256(replay_bytes(n, psa_bits)) + 2 * (kq + kf);
+  else
+    b += 2 * kp;
   return b;
 }
 
 PipeBufs carve_pipe(char*& cur, size_t n, int psa_bits, int out_mode) {
   PipeBufs b{};
   if (!psa_bits) return b;
-  const size_t kq = align256(n * 8), kp = align256(n * 4);
+  const size_t kq = align256(n * 8), kp = align256(n * 4), kf = align256(n);
   b.k0 = (uint64_t*)cur, cur += kq;
   b.k1 = (uint64_t*)cur, cur += kq;
-  b.p0 = (uint32_t*)cur, cur += kp;
-  b.p1 = (uint32_t*)cur, cur += kp;
   b.tmp = cur, cur += align256(radix_tmp_bytes(n));
   if (out_mode == HM_OUT_UNPERMUTE) {
+    b.replay = cur, cur += align256(replay_bytes(n, psa_bits));
     b.rv = (uint64_t*)cur, cur += kq;
-    b.rp = (uint32_t*)cur, cur += kp;
+    b.rf = (uint8_t*)cur, cur += kf;
+    b.vt = (uint64_t*)cur, cur += kq;
+    b.ft = (uint8_t*)cur, cur += kf;
+  } else {
+    b.p0 = (uint32_t*)cur, cur += kp;
+    b.p1 = (uint32_t*)cur, cur += kp;
   }
   return b;
 }
@@ -210,15 +219,19 @@ int enqueue_pipeline(const TreeView& tv, const SearchLaunch& sl, int psa_bits, i
     return HM_OK;
   }
   int which = 0;
+  if (out_mode == HM_OUT_UNPERMUTE) {
+    // keys-only sort recording its passes; search in PSA order (coalesced result writes); then the
+    // passes run backwards to put the results in input order
+    HM_TRY(radix_sort_keys_replay(dq, n, 64 - psa_bits, 64, b.k0, b.k1, b.replay, b.tmp, s, &which, dn));
+    const uint64_t* sk = which ? b.k1 : b.k0;
+    HM_TRY(launch_search(tv, sl, sk, nullptr, b.rv, b.rf, n, OUT_DIRECT, s, dn));
+    HM_TRY(unsort_results(b.rv, b.rf, n, psa_bits, b.replay, b.vt, b.ft, dvals, dfound, s, dn));
+    return HM_OK;
+  }
   HM_TRY(radix_sort_pos(dq, n, 64 - psa_bits, 64, b.k0, b.p0, b.k1, b.p1, b.tmp, s, &which, dn));
   const uint64_t* sk = which ? b.k1 : b.k0;
   const uint32_t* sp = which ? b.p1 : b.p0;
-  if (out_mode == HM_OUT_UNPERMUTE) {
-    // search in PSA order writing coalesced (value, pos|found) records, then sort them back by
-    // position; the sort buffers are free once the search has consumed them (stream order)
-    HM_TRY(launch_search(tv, sl, sk, sp, b.rv, (uint8_t*)b.rp, n, OUT_RECORDS, s, dn));
-    HM_TRY(unpermute_results(b.rp, b.rv, n, b.p0, b.k0, b.p1, b.k1, dvals, dfound, b.tmp, s, dn));
-  } else if (out_mode == HM_OUT_PSA_ORDER) {
+  if (out_mode == HM_OUT_PSA_ORDER) {
     HM_TRY(launch_search(tv, sl, sk, sp, dvals, dfound, n, OUT_DIRECT, s, dn));
     HM_TRY(hipMemcpyAsync(dpos, sp, n * 4, hipMemcpyDeviceToDevice, s));
   } else {
@@ -315,6 +328,8 @@ void free_tree(hm_tree* t) {
   if (t->d_leaves) (void)hipFree(t->d_leaves);
   if (t->pend_k) (void)hipFree(t->pend_k);
   if (t->pend_v) (void)hipFree(t->pend_v);
+  if (t->upd_buf) (void)hipFree(t->upd_buf);
+  if (t->upd_host) (void)hipHostFree(t->upd_host);
   if (t->publish_ev) (void)hipEventDestroy(t->publish_ev);
   if (t->own) (void)hipStreamDestroy(t->own);
   delete t;
@@ -353,6 +368,29 @@ int hm_params_init(hm_params* p) {
   return HM_OK;
 }
 
+// An empty tree object (no layout yet) on the current device.
+static int create_tree(const hm_params* p, hm_tree** out) {
+  hm_tree* t = new hm_tree();
+  t->p = *p;
+  t->F = (int)p->fanout;
+  t->fill = (int)p->leaf_fill;
+  t->NS = node_slots(t->F);
+  if (p->group_size) t->cfg_direct.group = t->cfg_psa.group = (int)p->group_size;
+  int rc = HM_OK;
+  if (hipGetDevice(&t->device) != hipSuccess)
+    rc = set_err(HM_EDEVICE, "no HIP device");
+  else if (hipStreamCreateWithFlags(&t->own, hipStreamNonBlocking) != hipSuccess)
+    rc = set_err(HM_EDEVICE, "hipStreamCreate failed");
+  else if (hipEventCreateWithFlags(&t->publish_ev, hipEventDisableTiming) != hipSuccess)
+    rc = set_err(HM_EDEVICE, "hipEventCreate failed");
+  if (rc) {
+    free_tree(t);
+    return rc;
+  }
+  *out = t;
+  return HM_OK;
+}
+
 int hm_tree_build(const uint64_t* keys, const uint64_t* vals, uint64_t n, const hm_params* p, hm_tree** out) {
   if (!out) return set_err(HM_EINVAL, "out is NULL");
   *out = nullptr;
@@ -361,21 +399,13 @@ int hm_tree_build(const uint64_t* keys, const uint64_t* vals, uint64_t n, const 
   if (n > 0 && (!keys || !vals)) return set_err(HM_EINVAL, "keys/vals is NULL with n = %llu", (unsigned long long)n);
   if (n > (1ull << 34)) return set_err(HM_EINVAL, "n = %llu too large", (unsigned long long)n);
 
-  hm_tree* t = new hm_tree();
-  t->p = *p;
-  t->F = (int)p->fanout;
-  t->fill = (int)p->leaf_fill;
-  t->NS = node_slots(t->F);
-  if (p->group_size) t->cfg_direct.group = t->cfg_psa.group = (int)p->group_size;
+  hm_tree* t = nullptr;
+  rc = create_tree(p, &t);
+  if (rc) return rc;
   auto fail = [&](int code) {
     free_tree(t);
     return code;
   };
-  if (hipGetDevice(&t->device) != hipSuccess) return fail(set_err(HM_EDEVICE, "no HIP device"));
-  if (hipStreamCreateWithFlags(&t->own, hipStreamNonBlocking) != hipSuccess)
-    return fail(set_err(HM_EDEVICE, "hipStreamCreate failed"));
-  if (hipEventCreateWithFlags(&t->publish_ev, hipEventDisableTiming) != hipSuccess)
-    return fail(set_err(HM_EDEVICE, "hipEventCreate failed"));
   hipStream_t s = t->own;
 
   t->n_keys = n;
@@ -628,17 +658,80 @@ int hm_range_between(hm_tree* t, const uint64_t* lo, const uint64_t* hi, uint64_
   return range_common(t, lo, hi, nullptr, cap, n, nullptr, out_keys, out_vals, out_counts, n * (uint64_t)cap, stream);
 }
 
+// Device scratch of one publish over n staged updates (carved from t->upd_buf).
+struct UpdBufs {
+  uint64_t *uk, *uv;          // sorted unique batch
+  uint64_t *k0, *v0, *k1, *v1;  // radix ping-pong (large batches)
+  void* rtmp;
+  uint32_t *keep, *keep_scan;
+  void* stmp;
+  uint32_t *leaf, *slot;
+  uint8_t* found;
+  uint32_t* summary;  // [0] overflow, [1] new keys, [2] unique count, [3] validation flags
+};
+
+static size_t upd_need(size_t n) {
+  const size_t k8 = align256(n * 8), k4 = align256((n + 1) * 4);
+  return 6 * k8 + align256(radix_tmp_bytes(n)) + 2 * k4 + align256(scan_tmp_bytes(n + 1)) + 2 * k4 + align256(n) +
+         256;
+}
+
+static UpdBufs carve_upd(void* base, size_t n) {
+  char* cur = (char*)base;
+  const size_t k8 = align256(n * 8), k4 = align256((n + 1) * 4);
+  UpdBufs b;
+  b.summary = (uint32_t*)cur, cur += 256;
+  b.uk = (uint64_t*)cur, cur += k8;
+  b.uv = (uint64_t*)cur, cur += k8;
+  b.k0 = (uint64_t*)cur, cur += k8;
+  b.v0 = (uint64_t*)cur, cur += k8;
+  b.k1 = (uint64_t*)cur, cur += k8;
+  b.v1 = (uint64_t*)cur, cur += k8;
+  b.rtmp = cur, cur += align256(radix_tmp_bytes(n));
+  b.keep = (uint32_t*)cur, cur += k4;
+  b.keep_scan = (uint32_t*)cur, cur += k4;
+  b.stmp = cur, cur += align256(scan_tmp_bytes(n + 1));
+  b.leaf = (uint32_t*)cur, cur += k4;
+  b.slot = (uint32_t*)cur, cur += k4;
+  b.found = (uint8_t*)cur;
+  return b;
+}
+
+// Grow-only: a publish in steady state allocates nothing (hipFree would synchronize the device).
+static int ensure_upd(hm_tree* t, size_t n, hipStream_t s) {
+  if (!t->upd_host) HM_TRY(hipHostMalloc((void**)&t->upd_host, 256, hipHostMallocDefault));
+  const size_t need = upd_need(std::max<size_t>(n, 1));
+  if (t->upd_bytes >= need) return HM_OK;
+  const size_t want = std::max(need, std::min<size_t>(2 * t->upd_bytes, upd_need(kMaxBatch)));
+  if (t->p.mem_budget && t->device_bytes() - t->upd_bytes + want > t->p.mem_budget)
+    return set_err(HM_EBUDGET, "update scratch for %llu keys exceeds the device-memory budget", (unsigned long long)n);
+  if (t->upd_buf) {
+    HM_TRY(hipStreamSynchronize(s));
+    HM_TRY(hipFree(t->upd_buf));
+    t->upd_buf = nullptr;
+    t->upd_bytes = 0;
+  }
+  HM_TRY(hipMalloc(&t->upd_buf, want));
+  t->upd_bytes = want;
+  return HM_OK;
+}
+
 int hm_update(hm_tree* t, const uint64_t* keys, const uint64_t* vals, uint64_t n, void* stream) {
   if (!t) return set_err(HM_EINVAL, "tree is NULL");
   if (n == 0) return HM_OK;
   if (!keys || !vals) return set_err(HM_EINVAL, "NULL buffer");
   if (n > kMaxBatch) return set_err(HM_EINVAL, "update batch too large");
   hipStream_t s = (hipStream_t)stream;
+  const bool dk = is_device_ptr(keys), dv = is_device_ptr(vals);
+  if (!dk) {  // host keys: check for the reserved key before anything touches the device
+    for (uint64_t i = 0; i < n; ++i)
+      if (keys[i] == HM_KEY_MAX) return set_err(HM_EINVAL, "key 0xFFFFFFFFFFFFFFFF is reserved for padding");
+  }
   std::lock_guard<std::mutex> g(t->mu);
   if (t->pend_n + n > kMaxBatch) return set_err(HM_EINVAL, "too many staged updates; publish first");
   if (t->pend_n + n > t->pend_cap) {
     size_t cap = std::max<size_t>(t->pend_n + n, 2 * t->pend_cap);
-    if (cap < 4096) cap = 4096;
+    if (cap < 65536) cap = 65536;
     if (t->p.mem_budget && t->device_bytes() + (cap - t->pend_cap) * 16 > t->p.mem_budget)
       return set_err(HM_EBUDGET, "staging %llu updates exceeds the device-memory budget", (unsigned long long)n);
     uint64_t *nk = nullptr, *nv = nullptr;
@@ -655,37 +748,78 @@ int hm_update(hm_tree* t, const uint64_t* keys, const uint64_t* vals, uint64_t n
     t->pend_v = nv;
     t->pend_cap = cap;
   }
-  const hipMemcpyKind kk = is_device_ptr(keys) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-  const hipMemcpyKind kv = is_device_ptr(vals) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-  HM_TRY(hipMemcpyAsync(t->pend_k + t->pend_n, keys, n * 8, kk, s));
-  HM_TRY(hipMemcpyAsync(t->pend_v + t->pend_n, vals, n * 8, kv, s));
-  // reject the reserved key before anything is staged
-  uint32_t* d_flags = nullptr;
-  uint32_t flags = 0;
-  HM_TRY(hipMalloc(&d_flags, 4));
-  HM_TRY(hipMemsetAsync(d_flags, 0, 4, s));
-  // validate_sorted flags bit 1 for KEY_MAX; bit 2 (order) is ignored here
-  HM_TRY(validate_sorted(t->pend_k + t->pend_n, n, d_flags, s));
-  HM_TRY(hipMemcpyAsync(&flags, d_flags, 4, hipMemcpyDeviceToHost, s));
-  HM_TRY(hipStreamSynchronize(s));
-  HM_TRY(hipFree(d_flags));
-  if (flags & 1u) return set_err(HM_EINVAL, "key 0xFFFFFFFFFFFFFFFF is reserved for padding");
+  HM_TRY(hipMemcpyAsync(t->pend_k + t->pend_n, keys, n * 8, dk ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+  HM_TRY(hipMemcpyAsync(t->pend_v + t->pend_n, vals, n * 8, dv ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+  if (dk) {  // device keys: check on the device (validate_sorted sets bit 0 for KEY_MAX; bit 1 is ignored)
+    int rc = ensure_upd(t, t->pend_n + n, s);
+    if (rc) return rc;
+    UpdBufs b = carve_upd(t->upd_buf, t->pend_n + n);
+    HM_TRY(hipMemsetAsync(b.summary + 3, 0, 4, s));
+    HM_TRY(validate_sorted(t->pend_k + t->pend_n, n, b.summary + 3, s));
+    HM_TRY(hipMemcpyAsync(t->upd_host + 3, b.summary + 3, 4, hipMemcpyDeviceToHost, s));
+    HM_TRY(hipStreamSynchronize(s));
+    if (t->upd_host[3] & 1u) return set_err(HM_EINVAL, "key 0xFFFFFFFFFFFFFFFF is reserved for padding");
+  }
   t->pend_n += n;
   return HM_OK;
 }
 
+// Make s wait for every search / range issued so far (before leaves change in place).
+static int wait_for_readers(hm_tree* t, hipStream_t s) {
+  for (auto& kv : t->ws)
+    if (kv.second.last_use) HM_TRY(hipStreamWaitEvent(s, kv.second.last_use, 0));
+  return HM_OK;
+}
+
+// Apply the staged batch on s.  Small batches (the serving case) cost a handful of launches and one
+// host round trip for the plan; only a batch that overflows some leaf re-lays out the leaves.
 static int apply_pending(hm_tree* t, hipStream_t s, int* rebuilt) {
   const size_t n = t->pend_n;
   if (!n) return HM_OK;
-  // sort the staged batch by key (stable: a later write of a key follows the earlier one)
+  int rc = ensure_upd(t, n, s);
+  if (rc) return rc;
+  const UpdBufs b = carve_upd(t->upd_buf, n);
+  uint32_t* dm = b.summary + 2;
+  HM_TRY(hipMemsetAsync(b.summary, 0, 16, s));
+  // sort by key (stable: a later write of a key follows the earlier one), keep the last write
+  if (n <= kSmallUpdate) {
+    HM_TRY(small_sort_dedupe(t->pend_k, t->pend_v, (uint32_t)n, b.uk, b.uv, dm, s));
+  } else {
+    int which = 0;
+    HM_TRY(radix_sort_u64pay(t->pend_k, t->pend_v, n, 0, 64, b.k0, b.v0, b.k1, b.v1, b.rtmp, s, &which));
+    const uint64_t* sk = which ? b.k1 : b.k0;
+    const uint64_t* sv = which ? b.v1 : b.v0;
+    HM_TRY(mark_last_of_runs(sk, n, b.keep, s));
+    HM_TRY(exclusive_scan_u32(b.keep, b.keep_scan, n, 0u, b.stmp, s));
+    HM_TRY(compact_pairs(sk, sv, b.keep, b.keep_scan, n, b.uk, b.uv, s));
+    HM_TRY(unique_count(b.keep, b.keep_scan, n, dm, s));
+  }
+  const TreeView tv = t->view();
+  HM_TRY(launch_locate(tv, b.uk, n, b.leaf, b.slot, b.found, s, dm));
+  HM_TRY(plan_runs(t->d_leaves, b.leaf, b.found, n, dm, b.summary, s));
+  HM_TRY(hipMemcpyAsync(t->upd_host, b.summary, 16, hipMemcpyDeviceToHost, s));
+  HM_TRY(hipStreamSynchronize(s));
+  const bool overflow = t->upd_host[0] & 1u;
+  const uint64_t n_new = t->upd_host[1];
+  const size_t m = t->upd_host[2];
+  rc = wait_for_readers(t, s);  // every write below waits for the searches issued before
+  if (rc) return rc;
+  if (!overflow) {
+    HM_TRY(apply_runs(t->d_leaves, b.leaf, b.slot, b.found, b.uk, b.uv, n, dm, s));
+    t->n_keys += n_new;
+    if (rebuilt) *rebuilt = 0;
+    return HM_OK;
+  }
+
+  // overflow: re-lay out all leaves, then rebuild the inner levels and prefix array
   std::vector<void*> tmps;
   auto cleanup = [&]() {
     (void)hipStreamSynchronize(s);
     for (void* p : tmps) (void)hipFree(p);
   };
-  auto alloc = [&](size_t b) -> void* {
+  auto alloc = [&](size_t bytes) -> void* {
     void* p = nullptr;
-    if (hipMalloc(&p, std::max<size_t>(b, 256)) != hipSuccess) return nullptr;
+    if (hipMalloc(&p, std::max<size_t>(bytes, 256)) != hipSuccess) return nullptr;
     tmps.push_back(p);
     return p;
   };
@@ -704,64 +838,12 @@ static int apply_pending(hm_tree* t, hipStream_t s, int* rebuilt) {
     cleanup();                                                       \
     return set_err(HM_ENOMEM, "update scratch allocation failed");   \
   }
-  UPD_ALLOC(k0, uint64_t*, n * 8);
-  UPD_ALLOC(v0, uint64_t*, n * 8);
-  UPD_ALLOC(k1, uint64_t*, n * 8);
-  UPD_ALLOC(v1, uint64_t*, n * 8);
-  UPD_ALLOC(rtmp, void*, radix_tmp_bytes(n));
-  int which = 0;
-  UPD_TRY(radix_sort_u64pay(t->pend_k, t->pend_v, n, 0, 64, k0, v0, k1, v1, rtmp, s, &which));
-  uint64_t* sk = which ? k1 : k0;
-  uint64_t* sv = which ? v1 : v0;
-  UPD_ALLOC(keep, uint32_t*, n * 4);
-  UPD_ALLOC(keep_scan, uint32_t*, (n + 1) * 4);
-  UPD_ALLOC(stmp, void*, scan_tmp_bytes(n + 1));
-  UPD_TRY(mark_last_of_runs(sk, n, keep, s));
-  UPD_TRY(exclusive_scan_u32(keep, keep_scan, n, 0u, stmp, s));
-  uint64_t* uk = which ? k0 : k1;  // the other ping-pong buffers are free now
-  uint64_t* uv = which ? v0 : v1;
-  UPD_TRY(compact_pairs(sk, sv, keep, keep_scan, n, uk, uv, s));
-  uint32_t last_scan = 0, last_keep = 0;
-  UPD_TRY(hipMemcpyAsync(&last_scan, keep_scan + n - 1, 4, hipMemcpyDeviceToHost, s));
-  UPD_TRY(hipMemcpyAsync(&last_keep, keep + n - 1, 4, hipMemcpyDeviceToHost, s));
-  UPD_TRY(hipStreamSynchronize(s));
-  const size_t m = last_scan + last_keep;
-
-  // locate every unique key
-  UPD_ALLOC(leaf, uint32_t*, m * 4);
-  UPD_ALLOC(slot, uint32_t*, m * 4);
-  UPD_ALLOC(found, uint8_t*, m);
-  const TreeView tv = t->view();
-  UPD_TRY(launch_locate(tv, uk, m, leaf, slot, found, s));
-  // inserts per leaf and the first batch index of each leaf's inserts
   UPD_ALLOC(ins, uint32_t*, t->n_leaves * 4);
   UPD_ALLOC(istart, uint32_t*, t->n_leaves * 4);
-  UPD_ALLOC(flags, uint32_t*, 256);
   UPD_TRY(hipMemsetAsync(ins, 0, t->n_leaves * 4, s));
   UPD_TRY(hipMemsetAsync(istart, 0xFF, t->n_leaves * 4, s));
-  UPD_TRY(hipMemsetAsync(flags, 0, 256, s));
-  UPD_TRY(count_inserts(leaf, found, m, ins, s));
-  UPD_TRY(insert_starts(leaf, found, m, istart, s));
-  UPD_TRY(overflow_check(t->d_leaves, ins, t->n_leaves, flags, s));
-  // number of inserted keys = m - found count; computed via a scan of found flags on the host side
-  std::vector<uint8_t> hfound(m);
-  uint32_t hflags = 0;
-  UPD_TRY(hipMemcpyAsync(hfound.data(), found, m, hipMemcpyDeviceToHost, s));
-  UPD_TRY(hipMemcpyAsync(&hflags, flags, 4, hipMemcpyDeviceToHost, s));
-  UPD_TRY(hipStreamSynchronize(s));
-  size_t n_new = 0;
-  for (uint8_t f : hfound) n_new += f ? 0 : 1;
-
-  if (!(hflags & 1u)) {
-    UPD_TRY(apply_value_updates(t->d_leaves, leaf, slot, found, uv, m, s));
-    if (n_new) UPD_TRY(merge_in_place(t->d_leaves, ins, istart, uk, uv, found, t->n_leaves, s));
-    UPD_TRY(hipStreamSynchronize(s));
-    t->n_keys += n_new;
-    if (rebuilt) *rebuilt = 0;
-    cleanup();
-    return HM_OK;
-  }
-  // overflow: re-lay out all leaves, then rebuild the inner levels and prefix array
+  UPD_TRY(count_inserts(b.leaf, b.found, m, ins, s));
+  UPD_TRY(insert_starts(b.leaf, b.found, m, istart, s));
   UPD_ALLOC(cnt, uint32_t*, t->n_leaves * 4);
   UPD_ALLOC(nstart, uint32_t*, (t->n_leaves + 1) * 4);
   UPD_ALLOC(ntmp, void*, scan_tmp_bytes(t->n_leaves + 1));
@@ -793,8 +875,8 @@ static int apply_pending(hm_tree* t, hipStream_t s, int* rebuilt) {
   }
   Leaf* nl = nullptr;
   UPD_TRY(hipMalloc(&nl, new_cap * sizeof(Leaf)));
-  UPD_TRY(apply_value_updates(t->d_leaves, leaf, slot, found, uv, m, s));
-  UPD_TRY(relayout_leaves(t->d_leaves, t->n_leaves, ins, istart, uk, uv, found, nstart, t->fill, nl, s));
+  UPD_TRY(apply_value_updates(t->d_leaves, b.leaf, b.slot, b.found, b.uv, m, s));
+  UPD_TRY(relayout_leaves(t->d_leaves, t->n_leaves, ins, istart, b.uk, b.uv, b.found, nstart, t->fill, nl, s));
   UPD_TRY(hipStreamSynchronize(s));
   UPD_TRY(hipFree(t->d_leaves));
   t->d_leaves = nl;
@@ -802,8 +884,9 @@ static int apply_pending(hm_tree* t, hipStream_t s, int* rebuilt) {
   t->leaf_cap = new_cap;
   t->n_keys += n_new;
   cleanup();
-  int r = rebuild_inner(t, s);
-  if (r) return r;
+  rc = rebuild_inner(t, s);
+  if (rc) return rc;
+  t->relayouts++;
   if (rebuilt) *rebuilt = 1;
   return HM_OK;
 #undef UPD_TRY
@@ -815,8 +898,8 @@ int hm_epoch_publish(hm_tree* t, void* stream, int* out_rebuilt) {
   if (out_rebuilt) *out_rebuilt = 0;
   hipStream_t s = (hipStream_t)stream;
   std::lock_guard<std::mutex> g(t->mu);
-  for (auto& kv : t->ws)
-    if (kv.second.last_use) HM_TRY(hipStreamWaitEvent(s, kv.second.last_use, 0));
+  // the previous publish's leaf writes come first; searches still running only read the tree, so
+  // the sort / locate / plan steps run beside them and only the writes wait (inside apply_pending)
   if (t->publish_valid) HM_TRY(hipStreamWaitEvent(s, t->publish_ev, 0));
   int rc = apply_pending(t, s, out_rebuilt);
   if (rc) return rc;
@@ -824,6 +907,150 @@ int hm_epoch_publish(hm_tree* t, void* stream, int* out_rebuilt) {
   HM_TRY(hipEventRecord(t->publish_ev, s));
   t->publish_valid = true;
   t->epoch++;
+  return HM_OK;
+}
+
+// ---- binary snapshot ------------------------------------------------------------------------
+// File = header, leaves (n_leaves x 128 B), inner nodes (n_inner x node_slots u64), prefix
+// (n_inner u32), all little-endian.  Loading uploads the leaves, rebuilds the inner levels and
+// prefix array on the device and requires them to hash to the stored sections: the inner layout
+// is a function of the leaves, so a file that disagrees is rejected instead of searched.
+namespace {
+constexpr char kSnapMagic[8] = {'H', 'M', 'S', 'N', 'A', 'P', '0', '1'};
+constexpr int kSnapLevels = 17;
+struct SnapHeader {
+  char magic[8];
+  uint32_t version, fanout, leaf_fill, node_slots;
+  uint32_t height, n_inner;
+  uint64_t n_leaves, n_keys, epoch;
+  uint32_t level_start[kSnapLevels];
+  uint32_t reserved;
+  uint64_t fnv_leaves, fnv_inner, fnv_prefix;
+};
+static_assert(sizeof(SnapHeader) == 152, "snapshot header layout");
+
+uint64_t fnv1a(const void* p, size_t n, uint64_t h = 0xCBF29CE484222325ull) {
+  const unsigned char* b = (const unsigned char*)p;
+  for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 0x100000001B3ull;
+  return h;
+}
+
+struct File {
+  FILE* f = nullptr;
+  ~File() {
+    if (f) fclose(f);
+  }
+};
+}  // namespace
+
+int hm_tree_save(const hm_tree* t, const char* path) {
+  if (!t || !path) return set_err(HM_EINVAL, "NULL argument");
+  if (t->pend_n) return set_err(HM_EINVAL, "%llu staged updates: publish before saving", (unsigned long long)t->pend_n);
+  std::vector<uint64_t> leaves(t->n_leaves * 16), inner((size_t)t->n_inner * t->NS);
+  std::vector<uint32_t> prefix(t->n_inner);
+  int rc = hm_tree_export(t, inner.data(), prefix.data(), leaves.data());
+  if (rc) return rc;
+  SnapHeader h{};
+  std::memcpy(h.magic, kSnapMagic, 8);
+  h.version = 1;
+  h.fanout = (uint32_t)t->F;
+  h.leaf_fill = (uint32_t)t->fill;
+  h.node_slots = (uint32_t)t->NS;
+  h.height = (uint32_t)t->height;
+  h.n_inner = t->n_inner;
+  h.n_leaves = t->n_leaves;
+  h.n_keys = t->n_keys;
+  h.epoch = t->epoch;
+  for (int i = 0; i < kSnapLevels; ++i) h.level_start[i] = t->level_start[i];
+  h.fnv_leaves = fnv1a(leaves.data(), leaves.size() * 8);
+  h.fnv_inner = fnv1a(inner.data(), inner.size() * 8);
+  h.fnv_prefix = fnv1a(prefix.data(), prefix.size() * 4);
+  File f;
+  f.f = fopen(path, "wb");
+  if (!f.f) return set_err(HM_EINVAL, "cannot create %s", path);
+  if (fwrite(&h, sizeof h, 1, f.f) != 1 || fwrite(leaves.data(), 8, leaves.size(), f.f) != leaves.size() ||
+      fwrite(inner.data(), 8, inner.size(), f.f) != inner.size() ||
+      fwrite(prefix.data(), 4, prefix.size(), f.f) != prefix.size() || fflush(f.f) != 0)
+    return set_err(HM_EINVAL, "write to %s failed", path);
+  return HM_OK;
+}
+
+int hm_tree_load(const char* path, const hm_params* p, hm_tree** out) {
+  if (!path || !out) return set_err(HM_EINVAL, "NULL argument");
+  *out = nullptr;
+  hm_params q;
+  hm_params_init(&q);
+  if (p) q = *p;
+  File f;
+  f.f = fopen(path, "rb");
+  if (!f.f) return set_err(HM_EINVAL, "cannot open %s", path);
+  SnapHeader h{};
+  if (fread(&h, sizeof h, 1, f.f) != 1 || std::memcmp(h.magic, kSnapMagic, 8) != 0 || h.version != 1)
+    return set_err(HM_EINVAL, "%s is not a harmonia snapshot", path);
+  q.fanout = h.fanout;  // the layout decides these
+  q.leaf_fill = h.leaf_fill;
+  int rc = check_params(&q);
+  if (rc) return rc;
+  if (h.node_slots != (uint32_t)node_slots((int)h.fanout) || h.n_leaves == 0 || h.n_leaves >= (1ull << 32) ||
+      h.n_keys > h.n_leaves * LEAF_KEYS)
+    return set_err(HM_EINVAL, "%s: inconsistent snapshot header", path);
+  std::vector<uint64_t> leaves(h.n_leaves * 16);
+  if (fread(leaves.data(), 8, leaves.size(), f.f) != leaves.size() ||
+      fnv1a(leaves.data(), leaves.size() * 8) != h.fnv_leaves)
+    return set_err(HM_EINVAL, "%s: leaf section truncated or corrupt", path);
+  // the leaf chain must hold the keys in strictly ascending order, padding at the end of each leaf
+  uint64_t n_keys = 0, prev = 0;
+  bool any = false;
+  for (uint64_t l = 0; l < h.n_leaves; ++l) {
+    const uint64_t* k = &leaves[l * 16];
+    int c = 0;
+    while (c < LEAF_KEYS && k[c] != HM_KEY_MAX) ++c;
+    for (int i = c; i < LEAF_KEYS; ++i)
+      if (k[i] != HM_KEY_MAX) return set_err(HM_EINVAL, "%s: leaf %llu has a key after padding", path, (unsigned long long)l);
+    if (c == 0 && h.n_leaves > 1) return set_err(HM_EINVAL, "%s: empty leaf %llu", path, (unsigned long long)l);
+    for (int i = 0; i < c; ++i) {
+      if (any && k[i] <= prev) return set_err(HM_EINVAL, "%s: keys out of order in leaf %llu", path, (unsigned long long)l);
+      prev = k[i];
+      any = true;
+    }
+    n_keys += (uint64_t)c;
+  }
+  if (n_keys != h.n_keys) return set_err(HM_EINVAL, "%s: key count mismatch", path);
+  const uint64_t need = h.n_leaves * sizeof(Leaf) + (uint64_t)h.n_inner * ((uint64_t)h.node_slots * 8 + 4);
+  if (q.mem_budget && need > q.mem_budget)
+    return set_err(HM_EBUDGET, "tree needs %llu device bytes, budget is %llu", (unsigned long long)need,
+                   (unsigned long long)q.mem_budget);
+  hm_tree* t = nullptr;
+  rc = create_tree(&q, &t);
+  if (rc) return rc;
+  t->n_keys = n_keys;
+  t->n_leaves = h.n_leaves;
+  t->leaf_cap = h.n_leaves;
+  auto fail = [&](int code) {
+    free_tree(t);
+    return code;
+  };
+  if (hipMalloc(&t->d_leaves, t->leaf_cap * sizeof(Leaf)) != hipSuccess)
+    return fail(set_err(HM_ENOMEM, "leaf allocation failed"));
+  if (hipMemcpy(t->d_leaves, leaves.data(), leaves.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+    return fail(set_err(HM_EDEVICE, "leaf upload failed"));
+  rc = rebuild_inner(t, t->own);
+  if (rc) return fail(rc);
+  // the stored inner levels and prefix must be the ones the leaves define
+  std::vector<uint64_t> inner((size_t)t->n_inner * t->NS), finner(inner.size());
+  std::vector<uint32_t> prefix(t->n_inner), fprefix(prefix.size());
+  if (t->n_inner != h.n_inner || (uint32_t)t->height != h.height ||
+      fread(finner.data(), 8, finner.size(), f.f) != finner.size() ||
+      fread(fprefix.data(), 4, fprefix.size(), f.f) != fprefix.size())
+    return fail(set_err(HM_EINVAL, "%s: inner section truncated or inconsistent", path));
+  rc = hm_tree_export(t, inner.data(), prefix.data(), nullptr);
+  if (rc) return fail(rc);
+  if (fnv1a(finner.data(), finner.size() * 8) != h.fnv_inner || fnv1a(fprefix.data(), fprefix.size() * 4) != h.fnv_prefix ||
+      std::memcmp(inner.data(), finner.data(), inner.size() * 8) != 0 ||
+      std::memcmp(prefix.data(), fprefix.data(), prefix.size() * 4) != 0)
+    return fail(set_err(HM_EINVAL, "%s: stored inner levels do not match the leaves", path));
+  t->epoch = h.epoch;
+  *out = t;
   return HM_OK;
 }
 
@@ -843,6 +1070,7 @@ int hm_tree_stats(const hm_tree* t, hm_stats* st) {
   st->bytes_leaves = t->n_leaves * sizeof(Leaf);
   st->bytes_device = t->device_bytes();
   st->epoch = t->epoch;
+  st->relayouts = t->relayouts;
   return HM_OK;
 }
 

@@ -78,12 +78,20 @@ hipError_t radix_sort_pos(const uint64_t* keys_in, size_t n, int begin_bit, int 
 hipError_t radix_sort_u64pay(const uint64_t* keys_in, const uint64_t* vals_in, size_t n, int begin_bit, int end_bit,
                              uint64_t* k0, uint64_t* v0, uint64_t* k1, uint64_t* v1, void* tmp, hipStream_t s,
                              int* which);
-// Return PSA-ordered results to input order: records (posf[i] = pos | found << 31, vals[i]) are
-// radix-sorted by position (LSD, 8-bit digits, p/v ping-pong buffers), the last pass writing
-// out_vals[pos] / out_found[pos] directly.  n < 2^31.
-hipError_t unpermute_results(const uint32_t* posf, const uint64_t* vals, size_t n, uint32_t* p0, uint64_t* v0,
-                             uint32_t* p1, uint64_t* v1, uint64_t* out_vals, uint8_t* out_found, void* tmp,
-                             hipStream_t s, const uint32_t* dn = nullptr);
+// Keys-only stable LSD sort over bits [begin_bit, end_bit) that records how to undo it: per pass,
+// the scanned (digit, tile) offsets and each element's index in its tile's digit-ordered staging
+// (16 bits).  `replay` must hold replay_bytes(n, end_bit - begin_bit); tmp holds radix_tmp_bytes(n).
+size_t replay_bytes(size_t n, int bits);
+hipError_t radix_sort_keys_replay(const uint64_t* keys_in, size_t n, int begin_bit, int end_bit, uint64_t* k0,
+                                  uint64_t* k1, void* replay, void* tmp, hipStream_t s, int* which,
+                                  const uint32_t* dn = nullptr);
+// Route per-element results (vals, found) of the sorted order back to input order by running the
+// recorded passes backwards: each pass gathers its tile's digit runs (coalesced) into LDS and writes
+// the tile in its input order.  (v_tmp, f_tmp) hold n entries; (vals, found) are clobbered when
+// there are three or more passes.
+hipError_t unsort_results(uint64_t* vals, uint8_t* found, size_t n, int bits, const void* replay, uint64_t* v_tmp,
+                          uint8_t* f_tmp, uint64_t* out_vals, uint8_t* out_found, hipStream_t s,
+                          const uint32_t* dn = nullptr);
 
 // ---- build (hm_build.hip) --------------------------------------------------------------
 // flags |= 1 if a key is KEY_MAX, |= 2 if keys are not strictly ascending.
@@ -109,7 +117,6 @@ struct SearchLaunch {
 enum SearchOut {
   OUT_DIRECT = 0,   // out_vals[i], out_found[i]
   OUT_SCATTER = 1,  // out_vals[pos[i]], out_found[pos[i]]
-  OUT_RECORDS = 2,  // out_vals[i] and ((uint32_t*)out_found)[i] = pos[i] | found << 31 (for unpermute_results)
 };
 // n sizes the grid; with dn != nullptr the kernel reads the actual count (<= n) from device memory.
 hipError_t launch_search(const TreeView& tv, const SearchLaunch& sl, const uint64_t* q, const uint32_t* pos,
@@ -119,9 +126,10 @@ hipError_t launch_search(const TreeView& tv, const SearchLaunch& sl, const uint6
 size_t lds_bytes_for(const TreeView& tv, int levels);
 // *p = v as a one-thread kernel (a graph-safe way to hand a count to the next replay).
 hipError_t launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s);
-// Locate (leaf, slot, found) per key (update path).
+// Locate (leaf, slot, found) per key (update path); n sizes the grid, dn (device) may hold the
+// actual count.
 hipError_t launch_locate(const TreeView& tv, const uint64_t* keys, size_t n, uint32_t* leaf_out, uint32_t* slot_out,
-                         uint8_t* found_out, hipStream_t s);
+                         uint8_t* found_out, hipStream_t s, const uint32_t* dn = nullptr);
 
 // ---- range (hm_range.hip) --------------------------------------------------------------
 hipError_t launch_range(const TreeView& tv, const uint64_t* start, const uint64_t* hi, const uint32_t* len,
@@ -140,12 +148,20 @@ hipError_t count_inserts(const uint32_t* leaf, const uint8_t* found, size_t n, u
                          hipStream_t s);
 // ins_start[leaf] = first index (into the sorted batch) of the leaf's inserts (valid where count > 0).
 hipError_t insert_starts(const uint32_t* leaf, const uint8_t* found, size_t n, uint32_t* ins_start, hipStream_t s);
-// In place: merge inserts into leaves that stay within 8 keys. flags |= 1 if any leaf overflows.
-hipError_t overflow_check(const Leaf* leaves, const uint32_t* ins_per_leaf, size_t n_leaves, uint32_t* d_flags,
-                          hipStream_t s);
-hipError_t merge_in_place(Leaf* leaves, const uint32_t* ins_per_leaf, const uint32_t* ins_start,
-                          const uint64_t* bkeys, const uint64_t* bvals, const uint8_t* found, size_t n_leaves,
-                          hipStream_t s);
+// Small batches (n <= kSmallUpdate): one block sorts the staged (key, value) pairs by key in LDS,
+// keeps the last write of each key and writes the m unique pairs to (uk, uv), m to *dm.
+constexpr uint32_t kSmallUpdate = 4096;
+hipError_t small_sort_dedupe(const uint64_t* pk, const uint64_t* pv, uint32_t n, uint64_t* uk, uint64_t* uv,
+                             uint32_t* dm, hipStream_t s);
+// *dm = keep_scan[n - 1] + keep[n - 1] (unique count after compact_pairs).
+hipError_t unique_count(const uint32_t* keep, const uint32_t* keep_scan, size_t n, uint32_t* dm, hipStream_t s);
+// Over the m located unique keys (sorted, so each leaf's keys form one run):
+// summary[0] |= 1 if some leaf would exceed 8 keys, summary[1] += number of new keys.  n sizes the grid.
+hipError_t plan_runs(const Leaf* leaves, const uint32_t* leaf, const uint8_t* found, size_t n, const uint32_t* dm,
+                     uint32_t* summary, hipStream_t s);
+// No leaf overflows: per run, overwrite the values of existing keys and merge the new keys in place.
+hipError_t apply_runs(Leaf* leaves, const uint32_t* leaf, const uint32_t* slot, const uint8_t* found,
+                      const uint64_t* uk, const uint64_t* uv, size_t n, const uint32_t* dm, hipStream_t s);
 // Out of place: new leaf count per old leaf (1 if it fits, else ceil(total / fill)).
 hipError_t new_leaf_counts(const Leaf* leaves, const uint32_t* ins_per_leaf, size_t n_leaves, int fill,
                            uint32_t* cnt, hipStream_t s);

@@ -79,13 +79,10 @@ __global__ __launch_bounds__(kThreads) void search_kernel(TreeView tv, int lds_l
         if constexpr (OUT == OUT_DIRECT) {
           out_vals[qi] = val;
           out_found[qi] = eq ? 1 : 0;
-        } else if constexpr (OUT == OUT_SCATTER) {
+        } else {
           const uint32_t o = pos[qi];
           out_vals[o] = val;
           out_found[o] = eq ? 1 : 0;
-        } else {
-          out_vals[qi] = val;
-          reinterpret_cast<uint32_t*>(out_found)[qi] = pos[qi] | (eq ? 0x80000000u : 0u);
         }
       }
     }
@@ -97,7 +94,9 @@ template <int F, int G>
 __global__ __launch_bounds__(kThreads) void locate_kernel(TreeView tv, const uint64_t* __restrict__ keys, uint32_t n,
                                                           uint32_t* __restrict__ leaf_out,
                                                           uint32_t* __restrict__ slot_out,
-                                                          uint8_t* __restrict__ found_out) {
+                                                          uint8_t* __restrict__ found_out,
+                                                          const uint32_t* __restrict__ dn) {
+  if (dn) n = *dn;  // device-side count (<= the launch size)
   constexpr int W = 8 / G;
   const int gl = threadIdx.x % G;
   const int gbase = (threadIdx.x & 63) - gl;
@@ -138,12 +137,8 @@ hipError_t launch_fgq(const TreeView& tv, const SearchLaunch& sl, const uint64_t
       hipLaunchKernelGGL((search_kernel<F, G, Q, OUT_DIRECT>), dim3((unsigned)nb), dim3(kThreads), lds, s, tv,
                          sl.lds_levels, q, pos, out_vals, out_found, (uint32_t)n, dn);
       break;
-    case OUT_SCATTER:
-      hipLaunchKernelGGL((search_kernel<F, G, Q, OUT_SCATTER>), dim3((unsigned)nb), dim3(kThreads), lds, s, tv,
-                         sl.lds_levels, q, pos, out_vals, out_found, (uint32_t)n, dn);
-      break;
     default:
-      hipLaunchKernelGGL((search_kernel<F, G, Q, OUT_RECORDS>), dim3((unsigned)nb), dim3(kThreads), lds, s, tv,
+      hipLaunchKernelGGL((search_kernel<F, G, Q, OUT_SCATTER>), dim3((unsigned)nb), dim3(kThreads), lds, s, tv,
                          sl.lds_levels, q, pos, out_vals, out_found, (uint32_t)n, dn);
       break;
   }
@@ -197,7 +192,7 @@ hipError_t launch_search(const TreeView& tv, const SearchLaunch& sl, const uint6
 }
 
 hipError_t launch_locate(const TreeView& tv, const uint64_t* keys, size_t n, uint32_t* leaf_out, uint32_t* slot_out,
-                         uint8_t* found_out, hipStream_t s) {
+                         uint8_t* found_out, hipStream_t s, const uint32_t* dn) {
   if (n == 0) return hipSuccess;
   ProfScope prof("update", s);
   constexpr int G = 8;
@@ -207,15 +202,15 @@ hipError_t launch_locate(const TreeView& tv, const uint64_t* keys, size_t n, uin
   switch (tv.fanout) {
     case 32:
       hipLaunchKernelGGL((locate_kernel<32, G>), dim3((unsigned)nb), dim3(kThreads), 0, s, tv, keys, (uint32_t)n,
-                         leaf_out, slot_out, found_out);
+                         leaf_out, slot_out, found_out, dn);
       break;
     case 64:
       hipLaunchKernelGGL((locate_kernel<64, G>), dim3((unsigned)nb), dim3(kThreads), 0, s, tv, keys, (uint32_t)n,
-                         leaf_out, slot_out, found_out);
+                         leaf_out, slot_out, found_out, dn);
       break;
     case 128:
       hipLaunchKernelGGL((locate_kernel<128, G>), dim3((unsigned)nb), dim3(kThreads), 0, s, tv, keys, (uint32_t)n,
-                         leaf_out, slot_out, found_out);
+                         leaf_out, slot_out, found_out, dn);
       break;
     default:
       return hipErrorInvalidValue;

@@ -10,7 +10,7 @@
 // number of ascending keys with matching values.  Prints one JSON object per run.
 //
 // Usage: hm_serve_bench --keys-log2 27 --requests 20000 --rate 5000 [--plain] [--closed]
-//                       [--inflight 4] [--budget-mb 4096] [--seed 1]
+//                       [--inflight 4] [--budget-mb 4096] [--seed 1] [--leaf-fill 6]
 //   --plain   one request per launch, one worker (the baseline path)
 //   --closed  submit back to back (throughput probe) instead of Poisson arrivals
 #include <algorithm>
@@ -111,7 +111,7 @@ extern "C" int hm_serve_bench_main(int argc, char** argv, char* out_json, uint64
   }
   hm_params p;
   CHECK(hm_params_init(&p));
-  p.leaf_fill = 6;  // slack for the insert stream
+  p.leaf_fill = (uint32_t)arg_d(argc, argv, "--leaf-fill", 6);  // slack for the insert stream
   hm_tree* t = nullptr;
   auto tb = std::chrono::steady_clock::now();
   CHECK(hm_tree_build(keys.data(), vals.data(), n, &p, &t));
@@ -221,18 +221,21 @@ extern "C" int hm_serve_bench_main(int argc, char** argv, char* out_json, uint64
     }
   }
   CHECK(hm_server_destroy(sv));
+  hm_stats ts;
+  CHECK(hm_tree_stats(t, &ts));
   CHECK(hm_tree_destroy(t));
   std::snprintf(out_json, out_len,
       "{\"mode\": \"%s\", \"arrivals\": \"%s\", \"rate_req_s\": %.1f, \"requests\": %llu, \"queries\": %llu, "
       "\"span_s\": %.4f, \"submit_s\": %.4f, \"queries_per_s\": %.1f, \"requests_per_s\": %.1f, \"p50_us\": %.1f, "
       "\"p99_us\": %.1f, \"mean_us\": %.1f, \"max_us\": %.1f, \"launches\": %llu, \"epochs\": %llu, "
       "\"lookups\": %llu, \"ranges\": %llu, \"updates\": %llu, \"bad_results\": %llu, \"keys\": %llu, "
-      "\"inflight\": %u, \"build_s\": %.2f}\n",
+      "\"inflight\": %u, \"build_s\": %.2f, \"leaf_fill\": %u, \"relayouts\": %llu}\n",
       plain ? "plain" : "served", closed ? "closed" : "poisson", rate, (unsigned long long)n_req,
       (unsigned long long)st.queries, st.span_s, submit_s, st.queries / st.span_s, st.completed / st.span_s,
       st.p50_us, st.p99_us, st.mean_us, st.max_us, (unsigned long long)st.launches, (unsigned long long)st.epochs,
       (unsigned long long)st.lookups, (unsigned long long)st.ranges, (unsigned long long)st.updates,
-      (unsigned long long)bad, (unsigned long long)n, sp.max_inflight, build_s);
+      (unsigned long long)bad, (unsigned long long)n, sp.max_inflight, build_s, p.leaf_fill,
+      (unsigned long long)ts.relayouts);
   return bad ? 3 : 0;
 }
 

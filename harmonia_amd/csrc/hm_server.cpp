@@ -1,12 +1,19 @@
 // Serving loop: a device-resident tree behind a request queue with continuous batching, admission
 // under a device-memory budget and update epochs.
 //
-// Worker threads (one HIP stream each) take every pending read request up to the next update,
-// coalesce lookups into one batched search and ranges into one batched scan, run them and write
-// each request's results back.  While one worker's batch is on the GPU the others keep admitting
-// whatever arrived meanwhile, so batches overlap on the device.  Updates act as barriers: reads
-// taken before an update are enqueued before it is published (epoch lock), and the publish orders
-// the device work (hm_epoch_publish waits for earlier searches; later searches wait for it).
+// Worker threads (one HIP stream each) take every pending read request (within the batch and memory
+// limits), coalesce lookups into one batched search and ranges into one batched scan, run them and
+// write each request's results back.  While one worker's batch is on the GPU the others keep
+// admitting whatever arrived meanwhile, so batches overlap on the device.
+//
+// Updates: when the oldest pending request is an update, a worker takes every pending update and
+// applies them as one epoch (in submission order, so the last write of a key wins).  Reads and
+// updates that are pending at the same time are concurrent operations and may be ordered either
+// way; a request submitted after another one completed always sees its effect.  A read batch holds
+// the epoch lock shared until its device work is enqueued; the update worker takes it exclusively
+// (while holding the queue lock, so no new read batch can start) and releases it once the publish
+// is enqueued: hm_epoch_publish makes the device wait for earlier searches, and later searches wait
+// for the publish.  Every read therefore sees one whole epoch, reported with its result.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -30,12 +37,15 @@ namespace {
 
 enum ReqType { REQ_LOOKUP = 0, REQ_RANGE = 1, REQ_UPDATE = 2 };
 
+// The caller's buffers (inputs included) stay valid until the request completes, so a request
+// holds pointers only: the keys are copied once, straight into the worker's pinned staging buffer.
 struct Request {
   uint64_t id = 0;
   int type = REQ_LOOKUP;
-  std::vector<uint64_t> keys;  // lookup keys / range starts / update keys
-  std::vector<uint32_t> lens;  // range lengths
-  std::vector<uint64_t> vals;  // update values
+  const uint64_t* keys = nullptr;  // lookup keys / range starts / update keys
+  const uint32_t* lens = nullptr;  // range lengths
+  const uint64_t* vals = nullptr;  // update values
+  uint32_t n = 0;
   uint64_t* out_vals = nullptr;
   uint8_t* out_found = nullptr;
   uint64_t* out_keys = nullptr;
@@ -55,8 +65,9 @@ struct Buf {
   bool pinned = false;
   int ensure(size_t b) {
     if (b <= bytes) return HM_OK;
-    release();
-    size_t want = std::max(b, (size_t)1 << 16);
+    const size_t old = bytes;
+    release();  // grows geometrically: a hipFree synchronizes the whole device
+    size_t want = std::max(std::max(b, 2 * old), (size_t)1 << 20);
     if (pinned) {
       HM_TRY(hipHostMalloc(&p, want, hipHostMallocDefault));
     } else {
@@ -83,9 +94,9 @@ struct Worker {
 size_t lookup_bytes(uint64_t n) { return n * 17 + pipe_bytes(n, 32, HM_OUT_UNPERMUTE); }
 size_t range_bytes(uint64_t n, uint64_t total) { return n * 24 + total * 16; }
 size_t request_bytes(const Request& r) {
-  if (r.type == REQ_LOOKUP) return lookup_bytes(r.keys.size());
-  if (r.type == REQ_RANGE) return range_bytes(r.keys.size(), r.range_total);
-  return r.keys.size() * 16;
+  if (r.type == REQ_LOOKUP) return lookup_bytes(r.n);
+  if (r.type == REQ_RANGE) return range_bytes(r.n, r.range_total);
+  return (uint64_t)r.n * 16;
 }
 
 }  // namespace
@@ -101,6 +112,7 @@ struct hm_server {
   bool stop = false;
   uint64_t inflight_bytes = 0;
   uint64_t outstanding = 0;  // submitted, not yet completed
+  uint32_t idle = 0;         // workers waiting for work
   std::shared_mutex epoch_mu;
   std::vector<std::unique_ptr<Worker>> workers;
   // statistics
@@ -113,18 +125,25 @@ struct hm_server {
 
 namespace {
 
-void complete(hm_server* sv, const ReqPtr& r, int status, uint64_t epoch) {
+// Mark a launch's requests done under one lock acquisition and wake the waiters once.
+void complete_all(hm_server* sv, const std::vector<ReqPtr>& rs, const int* status, int status_all, uint64_t epoch) {
+  if (rs.empty()) return;
   const auto now = Clock::now();
-  std::lock_guard<std::mutex> g(sv->mu);
-  r->status = status;
-  r->epoch = epoch;
-  r->done = true;
-  sv->lat_us.push_back(std::chrono::duration<double, std::micro>(now - r->t_submit).count());
-  sv->last_complete = now;
-  if (r->type == REQ_LOOKUP) ++sv->n_lookups, sv->n_queries += r->keys.size();
-  if (r->type == REQ_RANGE) ++sv->n_ranges, sv->n_queries += r->keys.size();
-  if (r->type == REQ_UPDATE) ++sv->n_updates;
-  --sv->outstanding;
+  {
+    std::lock_guard<std::mutex> g(sv->mu);
+    for (size_t i = 0; i < rs.size(); ++i) {
+      Request& r = *rs[i];
+      r.status = status ? status[i] : status_all;
+      r.epoch = epoch;
+      r.done = true;
+      sv->lat_us.push_back(std::chrono::duration<double, std::micro>(now - r.t_submit).count());
+      if (r.type == REQ_LOOKUP) ++sv->n_lookups, sv->n_queries += r.n;
+      if (r.type == REQ_RANGE) ++sv->n_ranges, sv->n_queries += r.n;
+      if (r.type == REQ_UPDATE) ++sv->n_updates;
+    }
+    sv->last_complete = now;
+    sv->outstanding -= rs.size();
+  }
   sv->cv_done.notify_all();
 }
 
@@ -141,9 +160,9 @@ int run_reads(hm_server* sv, Worker& w, std::vector<ReqPtr>& lookups, std::vecto
   hm_tree* t = sv->t;
   *epoch = tree_epoch(t);
   uint64_t nq = 0;
-  for (auto& r : lookups) nq += r->keys.size();
+  for (auto& r : lookups) nq += r->n;
   uint64_t nr = 0, rtot = 0;
-  for (auto& r : ranges) nr += r->keys.size(), rtot += r->range_total;
+  for (auto& r : ranges) nr += r->n, rtot += r->range_total;
   int rc = HM_OK;
   if (nq) {
     if ((rc = w.hq.ensure(nq * 8)) || (rc = w.hv.ensure(nq * 8)) || (rc = w.hf.ensure(nq)) ||
@@ -151,8 +170,8 @@ int run_reads(hm_server* sv, Worker& w, std::vector<ReqPtr>& lookups, std::vecto
       return rc;
     uint64_t o = 0;
     for (auto& r : lookups) {
-      std::memcpy((uint64_t*)w.hq.p + o, r->keys.data(), r->keys.size() * 8);
-      o += r->keys.size();
+      std::memcpy((uint64_t*)w.hq.p + o, r->keys, (size_t)r->n * 8);
+      o += r->n;
     }
     HM_TRY(hipMemcpyAsync(w.dq.p, w.hq.p, nq * 8, hipMemcpyHostToDevice, w.s));
     if ((rc = hm_search(t, (const uint64_t*)w.dq.p, nq, (uint64_t*)w.dv.p, (uint8_t*)w.df.p, w.s))) return rc;
@@ -168,9 +187,9 @@ int run_reads(hm_server* sv, Worker& w, std::vector<ReqPtr>& lookups, std::vecto
     uint32_t* hlen = (uint32_t*)((char*)w.hl.p + nr * 8);
     uint64_t o = 0;
     for (auto& r : ranges) {
-      std::memcpy(hs + o, r->keys.data(), r->keys.size() * 8);
-      std::memcpy(hlen + o, r->lens.data(), r->lens.size() * 4);
-      o += r->keys.size();
+      std::memcpy(hs + o, r->keys, (size_t)r->n * 8);
+      std::memcpy(hlen + o, r->lens, (size_t)r->n * 4);
+      o += r->n;
     }
     HM_TRY(hipMemcpyAsync(w.dl.p, w.hl.p, nr * 12, hipMemcpyHostToDevice, w.s));
     uint64_t* dkeys = (uint64_t*)w.dk.p;
@@ -187,14 +206,14 @@ int run_reads(hm_server* sv, Worker& w, std::vector<ReqPtr>& lookups, std::vecto
   // hand results back
   uint64_t o = 0;
   for (auto& r : lookups) {
-    const size_t n = r->keys.size();
+    const size_t n = r->n;
     std::memcpy(r->out_vals, (uint64_t*)w.hv.p + o, n * 8);
     std::memcpy(r->out_found, (uint8_t*)w.hf.p + o, n);
     o += n;
   }
   uint64_t ro = 0, rk = 0;
   for (auto& r : ranges) {
-    const size_t n = r->keys.size();
+    const size_t n = r->n;
     std::memcpy(r->out_counts, (uint32_t*)w.hc.p + ro, n * 4);
     std::memcpy(r->out_keys, (uint64_t*)w.hk.p + rk, r->range_total * 8);
     std::memcpy(r->out_vals, (uint64_t*)w.hk.p + rtot + rk, r->range_total * 8);
@@ -204,19 +223,31 @@ int run_reads(hm_server* sv, Worker& w, std::vector<ReqPtr>& lookups, std::vecto
   return HM_OK;
 }
 
-int run_updates(hm_server* sv, Worker& w, std::vector<ReqPtr>& ups, uint64_t* epoch) {
+// Apply a coalesced update batch as one epoch.  Called with the epoch lock held exclusively; it is
+// released once the publish is enqueued.
+// A request that cannot be staged (e.g. it holds the reserved key) fails alone; status[i] is the
+// outcome of ups[i].
+void run_updates(hm_server* sv, Worker& w, std::vector<ReqPtr>& ups, std::unique_lock<std::shared_mutex>& ex,
+                 std::vector<int>& status, uint64_t* epoch) {
   hm_tree* t = sv->t;
-  std::unique_lock<std::shared_mutex> ex(sv->epoch_mu);  // reads taken earlier are enqueued by now
-  for (auto& r : ups) {
-    int rc = hm_update(t, r->keys.data(), r->vals.data(), r->keys.size(), w.s);
-    if (rc) return rc;
-  }
+  status.assign(ups.size(), HM_OK);
+  for (size_t i = 0; i < ups.size(); ++i)
+    status[i] = hm_update(t, ups[i]->keys, ups[i]->vals, ups[i]->n, w.s);
   int rebuilt = 0;
   int rc = hm_epoch_publish(t, w.s, &rebuilt);
-  if (rc) return rc;
-  HM_TRY(hipStreamSynchronize(w.s));
   *epoch = tree_epoch(t);
-  return HM_OK;
+  ex.unlock();
+  if (!rc && hipStreamSynchronize(w.s) != hipSuccess) rc = set_err(HM_EDEVICE, "update stream failed");
+  if (rc)
+    for (int& st : status)
+      if (!st) st = rc;
+}
+
+// The first pending read, or nullptr.
+Request* first_read(hm_server* sv) {
+  for (auto& r : sv->pending)
+    if (r->type != REQ_UPDATE) return r.get();
+  return nullptr;
 }
 
 void worker_loop(hm_server* sv, Worker* w) {
@@ -225,52 +256,77 @@ void worker_loop(hm_server* sv, Worker* w) {
     std::vector<ReqPtr> lookups, ranges, ups;
     uint64_t bytes = 0;
     std::shared_lock<std::shared_mutex> epoch_lock(sv->epoch_mu, std::defer_lock);
+    std::unique_lock<std::shared_mutex> ex_lock(sv->epoch_mu, std::defer_lock);
     {
       std::unique_lock<std::mutex> lk(sv->mu);
+      ++sv->idle;
       sv->cv_work.wait(lk, [&] {
         if (sv->stop) return true;
         if (sv->pending.empty()) return false;
-        const ReqPtr& h = sv->pending.front();
-        if (h->type == REQ_UPDATE) return true;
-        return sv->inflight_bytes == 0 || sv->inflight_bytes + request_bytes(*h) <= sv->p.mem_budget;
+        if (sv->pending.front()->type == REQ_UPDATE) return true;
+        if (!sv->p.coalesce) return sv->inflight_bytes == 0;
+        const Request* h = first_read(sv);
+        return h && (sv->inflight_bytes == 0 || sv->inflight_bytes + request_bytes(*h) <= sv->p.mem_budget);
       });
+      --sv->idle;
       if (sv->stop && sv->pending.empty()) return;
-      if (sv->pending.front()->type == REQ_UPDATE) {
-        while (!sv->pending.empty() && sv->pending.front()->type == REQ_UPDATE) {
-          ups.push_back(sv->pending.front());
-          sv->pending.pop_front();
-          if (!sv->p.coalesce) break;
+      if (!sv->p.coalesce) {  // one request per launch, in arrival order
+        ReqPtr h = sv->pending.front();
+        sv->pending.pop_front();
+        if (h->type == REQ_UPDATE) {
+          ups.push_back(h);
+          ex_lock.lock();
+        } else {
+          (h->type == REQ_LOOKUP ? lookups : ranges).push_back(h);
+          bytes = request_bytes(*h);
+          sv->inflight_bytes += bytes;
+          epoch_lock.lock();
         }
+      } else if (sv->pending.front()->type == REQ_UPDATE) {
+        // every pending update, in submission order, as one epoch
+        std::deque<ReqPtr> rest;
+        for (auto& r : sv->pending) (r->type == REQ_UPDATE ? ups.push_back(r) : rest.push_back(r));
+        sv->pending.swap(rest);
+        ex_lock.lock();  // waits only for read batches still enqueueing; none can start meanwhile
       } else {
+        // every pending read that fits the batch and memory limits
         uint64_t nq = 0;
-        while (!sv->pending.empty() && sv->pending.front()->type != REQ_UPDATE) {
-          const ReqPtr& h = sv->pending.front();
+        std::deque<ReqPtr> rest;
+        bool full = false;
+        for (auto& h : sv->pending) {
+          if (h->type == REQ_UPDATE || full) {
+            rest.push_back(h);
+            continue;
+          }
           const uint64_t b = request_bytes(*h);
-          const uint64_t m = h->keys.size() + h->range_total;
-          if (!lookups.empty() || !ranges.empty()) {
-            if (!sv->p.coalesce) break;
-            if (nq + m > sv->p.max_batch) break;
-            if (sv->inflight_bytes + bytes + b > sv->p.mem_budget) break;
+          const uint64_t m = h->n + h->range_total;
+          if ((!lookups.empty() || !ranges.empty()) &&
+              (nq + m > sv->p.max_batch || sv->inflight_bytes + bytes + b > sv->p.mem_budget)) {
+            full = true;  // keep arrival order among the reads left behind
+            rest.push_back(h);
+            continue;
           }
           (h->type == REQ_LOOKUP ? lookups : ranges).push_back(h);
           nq += m;
           bytes += b;
-          sv->pending.pop_front();
         }
+        sv->pending.swap(rest);
         sv->inflight_bytes += bytes;
-        epoch_lock.lock();  // before mu is released: an update behind these reads waits for them
+        epoch_lock.lock();  // before mu is released: an update taken later waits for these reads
       }
       ++sv->n_launches;
       if (!sv->pending.empty()) sv->cv_work.notify_one();
     }
     uint64_t epoch = 0;
     if (!ups.empty()) {
-      const int rc = run_updates(sv, *w, ups, &epoch);
+      std::vector<int> status;
+      run_updates(sv, *w, ups, ex_lock, status, &epoch);
       {
         std::lock_guard<std::mutex> g(sv->mu);
         ++sv->n_epochs;
       }
-      for (auto& r : ups) complete(sv, r, rc, epoch);
+      complete_all(sv, ups, status.data(), HM_OK, epoch);
+      sv->cv_work.notify_all();
     } else {
       const int rc = run_reads(sv, *w, lookups, ranges, epoch_lock, &epoch);
       if (epoch_lock.owns_lock()) epoch_lock.unlock();
@@ -279,8 +335,8 @@ void worker_loop(hm_server* sv, Worker* w) {
         sv->inflight_bytes -= bytes;
       }
       sv->cv_work.notify_all();
-      for (auto& r : lookups) complete(sv, r, rc, epoch);
-      for (auto& r : ranges) complete(sv, r, rc, epoch);
+      complete_all(sv, lookups, nullptr, rc, epoch);
+      complete_all(sv, ranges, nullptr, rc, epoch);
     }
   }
 }
@@ -303,7 +359,7 @@ int submit(hm_server* sv, ReqPtr r, uint64_t* ticket) {
   sv->pending.push_back(r);
   ++sv->outstanding;
   if (ticket) *ticket = r->id;
-  sv->cv_work.notify_one();
+  if (sv->idle) sv->cv_work.notify_one();  // busy workers look at the queue when they finish
   return HM_OK;
 }
 
@@ -340,7 +396,8 @@ int hm_server_lookup(hm_server* s, const uint64_t* keys, uint32_t n, uint64_t* o
   if (!s || (n && (!keys || !out_vals || !out_found))) return set_err(HM_EINVAL, "NULL argument");
   auto r = std::make_shared<Request>();
   r->type = REQ_LOOKUP;
-  r->keys.assign(keys, keys + n);
+  r->keys = keys;
+  r->n = n;
   r->out_vals = out_vals;
   r->out_found = out_found;
   return submit(s, r, ticket);
@@ -352,8 +409,9 @@ int hm_server_range(hm_server* s, const uint64_t* starts, const uint32_t* lens, 
     return set_err(HM_EINVAL, "NULL argument");
   auto r = std::make_shared<Request>();
   r->type = REQ_RANGE;
-  r->keys.assign(starts, starts + n);
-  r->lens.assign(lens, lens + n);
+  r->keys = starts;
+  r->lens = lens;
+  r->n = n;
   for (uint32_t i = 0; i < n; ++i) r->range_total += lens[i];
   if (r->range_total > capacity) return set_err(HM_EINVAL, "capacity %llu < sum of lengths %llu",
                                                 (unsigned long long)capacity, (unsigned long long)r->range_total);
@@ -367,8 +425,9 @@ int hm_server_update(hm_server* s, const uint64_t* keys, const uint64_t* vals, u
   if (!s || (n && (!keys || !vals))) return set_err(HM_EINVAL, "NULL argument");
   auto r = std::make_shared<Request>();
   r->type = REQ_UPDATE;
-  r->keys.assign(keys, keys + n);
-  r->vals.assign(vals, vals + n);
+  r->keys = keys;
+  r->vals = vals;
+  r->n = n;
   return submit(s, r, ticket);
 }
 

@@ -1,4 +1,4 @@
-// Hand-written stable LSD radix sort (8-bit digits) for u64 or u32 keys with a payload.
+// Hand-written stable LSD radix sort (8-bit digits) for u64 keys, with or without a payload.
 //
 // The PSA step of the reference sorts (key, position) pairs with
 // cub::DeviceRadixSort::SortPairs(keys, pos, n, 48, 64) before every search launch
@@ -6,14 +6,22 @@
 // That call is a stable LSD radix sort over key bits [begin_bit, end_bit); for keys in
 // [0, 2^63) CUB's signed-key bit twiddle (H/cub/cub/util_type.cuh:1015-1022) does not change the
 // order, so the result equals a stable sort of (key, pos) by key >> begin_bit.  The same kernels
-// sort the bulk-build input (u64 value payload over every varying bit) and, keyed by the original
-// position, return PSA-ordered search results to input order ("un-permute"; the last pass writes
-// the caller's value and found arrays directly).
+// sort the bulk-build input (u64 value payload over every varying bit).
 //
 // One pass = histogram kernel (per-tile digit counts, digit-major) -> exclusive scan over
 // (digit, tile) -> scatter kernel (wave-level digit match via 8 ballots gives each item its
 // stable rank inside its wave; per-wave counters are scanned across the block; the tile is then
-// staged in LDS in digit order so the global writes of each digit run are coalesced).
+// staged in LDS in digit order so the global writes of each digit run are coalesced).  Tiles are
+// mapped to blocks so that each XCD processes a contiguous range of tiles: the digit runs of
+// neighbouring tiles share cache lines at their ends, and those partial lines then meet in one L2.
+//
+// Returning results to input order: a search in PSA order produces one result per sorted query.
+// Instead of sorting the results by position (three more passes over 24 position bits) or
+// writing each result to its position (one random 9-byte write per query, bound by cache-line
+// transactions), the keys-only sort records, per pass, the scanned (digit, tile) offsets and each
+// element's index in its tile's LDS staging.  Running the passes backwards, a block reads its
+// tile's digit runs from the pass output (contiguous runs: coalesced), stages them in LDS, and
+// writes the tile back in its input order (coalesced).
 #include "hm_block.h"
 #include "hm_internal.h"
 
@@ -24,6 +32,13 @@ constexpr int kThreads = 256;
 constexpr int kItems = 16;
 constexpr int kTile = kThreads * kItems;
 constexpr int kWaves = kThreads / 64;
+static_assert(kTile <= 65536, "staging indices are 16-bit");
+
+// Block -> tile: blocks are dispatched round-robin over the 8 XCDs; XCD x gets tiles
+// [x * grid / 8, (x + 1) * grid / 8).  grid % 8 == 0; tiles >= nb are idle.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t grid) { return (b & 7u) * (grid >> 3) + (b >> 3); }
+unsigned grid_for(size_t nb) { return (unsigned)((nb + 7) / 8 * 8); }
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 // Lanes (among `valid`) whose digit equals this lane's digit.
 __device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t valid) {
@@ -41,13 +56,15 @@ template <typename K>
 __global__ __launch_bounds__(kThreads) void radix_hist_kernel(const K* __restrict__ keys, size_t n, int shift,
                                                               uint32_t mask, uint32_t* __restrict__ counts,
                                                               uint32_t nb, const uint32_t* __restrict__ dn) {
+  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  if (tile >= nb) return;  // block-uniform
   if (dn) n = *dn;  // device-side count (graph replays): tiles past n count nothing
   __shared__ uint32_t hist[256];
   hist[threadIdx.x] = 0;
   __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const uint64_t lt = (1ull << lane) - 1ull;
-  const size_t base = (size_t)blockIdx.x * kTile + (size_t)w * 64 * kItems;
+  const size_t base = (size_t)tile * kTile + (size_t)w * 64 * kItems;
   const size_t last = n ? n - 1 : 0;
   // issue every load of the tile before the first use, so the loads overlap
   K kk[kItems];
@@ -65,36 +82,36 @@ __global__ __launch_bounds__(kThreads) void radix_hist_kernel(const K* __restric
     if (valid && (peers & lt) == 0) atomicAdd(&hist[d], (uint32_t)__popcll(peers));
   }
   __syncthreads();
-  counts[(size_t)threadIdx.x * nb + blockIdx.x] = hist[threadIdx.x];
+  counts[(size_t)threadIdx.x * nb + tile] = hist[threadIdx.x];
 }
 
-// MODE 0: write (key, value) to kout/vout.
-// MODE 1: final un-permute pass: keys are pos | found << 31, the destination index is the position
-//         itself; write out_vals[p] = value and out_found[p] = found.
-template <typename K, typename V, int MODE>
+// Write (key, value) to (kout, vout); without HAS_VAL keys only.  With SAVE_LP, lp_out[i] = the
+// index of element i in its tile's digit-ordered staging (what un-sorting needs).
+template <typename K, typename V, bool HAS_VAL, bool SAVE_LP>
 __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(const K* __restrict__ kin, const V* __restrict__ vin,
                                                                  K* __restrict__ kout, V* __restrict__ vout,
-                                                                 uint8_t* __restrict__ fout, size_t n, int shift,
+                                                                 uint16_t* __restrict__ lp_out, size_t n, int shift,
                                                                  uint32_t mask, const uint32_t* __restrict__ offsets,
                                                                  uint32_t nb, int gen_pos,
                                                                  const uint32_t* __restrict__ dn) {
+  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   if (dn) n = *dn;
-  if ((size_t)blockIdx.x * kTile >= n) return;  // whole tile past the end (block-uniform)
+  if (tile >= nb || (size_t)tile * kTile >= n) return;  // whole tile past the end (block-uniform)
   __shared__ uint32_t wcnt[kWaves][256];
   __shared__ uint32_t goff[256];
   __shared__ uint32_t boff[256];
   __shared__ uint32_t scan_sh[kWaves + 1];
   __shared__ K skeys[kTile];
-  __shared__ V svals[kTile];
+  __shared__ V svals[HAS_VAL ? kTile : 1];
   for (int i = threadIdx.x; i < kWaves * 256; i += kThreads) (&wcnt[0][0])[i] = 0;
-  goff[threadIdx.x] = offsets[(size_t)threadIdx.x * nb + blockIdx.x];
+  goff[threadIdx.x] = offsets[(size_t)threadIdx.x * nb + tile];
   __syncthreads();
 
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const uint64_t lt = (1ull << lane) - 1ull;
-  const size_t base = (size_t)blockIdx.x * kTile + (size_t)w * 64 * kItems;
+  const size_t base = (size_t)tile * kTile + (size_t)w * 64 * kItems;
   K kk[kItems];
-  V vv[kItems];
+  V vv[HAS_VAL ? kItems : 1];
   uint32_t pk[kItems];  // (rank within wave << 8) | digit
   const size_t last = n - 1;  // n > 0 here
   // issue every load of the tile before the first use, so the loads overlap
@@ -103,7 +120,7 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(const K* __rest
     const size_t i = base + (size_t)r * 64 + lane;
     const size_t ic = i < last ? i : last;
     kk[r] = kin[ic];
-    vv[r] = gen_pos ? (V)i : vin[ic];
+    if constexpr (HAS_VAL) vv[r] = gen_pos ? (V)i : vin[ic];
   }
 #pragma unroll
   for (int r = 0; r < kItems; ++r) {
@@ -139,23 +156,92 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(const K* __rest
       const uint32_t d = pk[r] & 255u;
       const uint32_t lp = boff[d] + wcnt[w][d] + (pk[r] >> 8);
       skeys[lp] = kk[r];
-      svals[lp] = vv[r];
+      if constexpr (HAS_VAL) svals[lp] = vv[r];
+      if constexpr (SAVE_LP) lp_out[i] = (uint16_t)lp;
     }
   }
   __syncthreads();
   // ... and write it out with consecutive lanes on consecutive addresses of each digit run
-  const size_t tile0 = (size_t)blockIdx.x * kTile;
+  const size_t tile0 = (size_t)tile * kTile;
   const uint32_t cnt = (uint32_t)(n - tile0 < (size_t)kTile ? n - tile0 : (size_t)kTile);
   for (uint32_t i = threadIdx.x; i < cnt; i += kThreads) {
     const K k = skeys[i];
     const uint32_t d = (uint32_t)(k >> shift) & mask;
     const uint32_t p = goff[d] + (i - boff[d]);
-    if constexpr (MODE == 0) {
-      kout[p] = k;
-      vout[p] = svals[i];
-    } else {
-      vout[p] = svals[i];
-      fout[p] = (uint8_t)((uint32_t)k >> 31);
+    kout[p] = k;
+    if constexpr (HAS_VAL) vout[p] = svals[i];
+  }
+}
+
+// Undo one recorded pass for per-element results: (vin, fin) are in the pass's output order,
+// (vout, fout) receive them in its input order.
+__global__ __launch_bounds__(kThreads) void radix_unscatter_kernel(const uint64_t* __restrict__ vin,
+                                                                   const uint8_t* __restrict__ fin,
+                                                                   uint64_t* __restrict__ vout,
+                                                                   uint8_t* __restrict__ fout,
+                                                                   const uint16_t* __restrict__ lp, size_t n,
+                                                                   const uint32_t* __restrict__ offsets, uint32_t nb,
+                                                                   const uint32_t* __restrict__ dn) {
+  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  if (dn) n = *dn;
+  if (tile >= nb || (size_t)tile * kTile >= n) return;  // block-uniform
+  __shared__ uint32_t goff[256];
+  __shared__ uint32_t boff[257];
+  __shared__ uint32_t scan_sh[kWaves + 1];
+  __shared__ uint64_t sv[kTile];
+  __shared__ uint8_t sf[kTile];
+  {
+    // this tile's count of digit d = next scanned offset - this one (digit-major, exclusive scan)
+    const uint32_t d = threadIdx.x;
+    const size_t idx = (size_t)d * nb + tile;
+    const uint32_t g = offsets[idx];
+    const uint32_t next = idx + 1 < (size_t)256 * nb ? offsets[idx + 1] : (uint32_t)n;
+    goff[d] = g;
+    uint32_t total;
+    boff[d] = block_exclusive_scan<uint32_t, kThreads>(next - g, scan_sh, total);
+    if (d == 0) boff[256] = total;
+  }
+  __syncthreads();
+  const size_t tile0 = (size_t)tile * kTile;
+  const uint32_t cnt = boff[256];
+  // gather the tile's digit runs in digit order (consecutive i of one digit read consecutive
+  // addresses); the digit of staging index i is the last d with boff[d] <= i.  Every load of the
+  // tile is issued before the first one is used, so their latencies overlap.
+  uint32_t src[kItems];
+  uint16_t lpv[kItems];
+#pragma unroll
+  for (int r = 0; r < kItems; ++r) {
+    const uint32_t i = threadIdx.x + r * kThreads;
+    uint32_t lo = 0;
+#pragma unroll
+    for (uint32_t step = 128; step; step >>= 1)
+      if (boff[lo + step] <= i) lo += step;
+    src[r] = goff[lo] + (i - boff[lo]);
+    lpv[r] = i < cnt ? lp[tile0 + i] : (uint16_t)0;
+  }
+  uint64_t v[kItems];
+  uint8_t f[kItems];
+#pragma unroll
+  for (int r = 0; r < kItems; ++r) {
+    const uint32_t i = threadIdx.x + r * kThreads;
+    v[r] = i < cnt ? vin[src[r]] : 0ull;
+    f[r] = i < cnt ? fin[src[r]] : (uint8_t)0;
+  }
+#pragma unroll
+  for (int r = 0; r < kItems; ++r) {
+    const uint32_t i = threadIdx.x + r * kThreads;
+    if (i < cnt) {
+      sv[i] = v[r];
+      sf[i] = f[r];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kItems; ++r) {
+    const uint32_t j = threadIdx.x + r * kThreads;
+    if (j < cnt) {
+      vout[tile0 + j] = sv[lpv[r]];
+      fout[tile0 + j] = sf[lpv[r]];
     }
   }
 }
@@ -170,30 +256,30 @@ __global__ void copy_with_pos_kernel(const uint64_t* __restrict__ kin, const V* 
   }
 }
 
-size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
-// One digit pass from (sk, sv) into (dk, dv) [MODE 0] or into the result arrays [MODE 1].
-template <typename K, typename V, int MODE>
-hipError_t radix_pass(const K* sk, const V* sv, K* dk, V* dv, uint8_t* df, size_t n, const uint32_t* dn, int shift, int bits,
-                      bool gen, void* tmp, hipStream_t s, const char* hist_stage, const char* scatter_stage) {
+// One digit pass from (sk, sv) into (dk, dv).  counts (256 * nb u32) receive the scanned
+// (digit, tile) offsets; scan_tmp holds scan_tmp_bytes(256 * nb).
+template <typename V, bool HAS_VAL, bool SAVE_LP>
+hipError_t radix_pass(const uint64_t* sk, const V* sv, uint64_t* dk, V* dv, uint16_t* lp, size_t n, const uint32_t* dn,
+                      int shift, int bits, bool gen, uint32_t* counts, void* scan_tmp, hipStream_t s) {
   const size_t nb = (n + kTile - 1) / kTile;
-  uint32_t* counts = reinterpret_cast<uint32_t*>(tmp);
-  void* scan_tmp = reinterpret_cast<char*>(tmp) + align_up(256 * nb * sizeof(uint32_t), 256);
   const uint32_t mask = (1u << bits) - 1u;
   {
-    ProfScope prof(hist_stage, s);
-    hipLaunchKernelGGL((radix_hist_kernel<K>), dim3((unsigned)nb), dim3(kThreads), 0, s, sk, n, shift, mask, counts,
-                       (uint32_t)nb, dn);
+    ProfScope prof("psa_hist", s);
+    hipLaunchKernelGGL((radix_hist_kernel<uint64_t>), dim3(grid_for(nb)), dim3(kThreads), 0, s, sk, n, shift, mask,
+                       counts, (uint32_t)nb, dn);
   }
   hipError_t e = exclusive_scan_u32(counts, counts, 256 * nb, 0u, scan_tmp, s);
   if (e != hipSuccess) return e;
   {
-    ProfScope prof(scatter_stage, s);
-    hipLaunchKernelGGL((radix_scatter_kernel<K, V, MODE>), dim3((unsigned)nb), dim3(kThreads), 0, s, sk, sv, dk, dv,
-                       df, n, shift, mask, (const uint32_t*)counts, (uint32_t)nb, gen ? 1 : 0, dn);
+    ProfScope prof("psa_scatter", s);
+    hipLaunchKernelGGL((radix_scatter_kernel<uint64_t, V, HAS_VAL, SAVE_LP>), dim3(grid_for(nb)), dim3(kThreads), 0, s,
+                       sk, sv, dk, dv, lp, n, shift, mask, (const uint32_t*)counts, (uint32_t)nb, gen ? 1 : 0, dn);
   }
   return hipGetLastError();
 }
+
+size_t counts_bytes(size_t n) { return align_up(256 * ((n + kTile - 1) / kTile + 1) * sizeof(uint32_t), 256); }
 
 template <typename V>
 hipError_t radix_sort_impl(const uint64_t* keys_in, const V* vals_in, size_t n, int begin_bit, int end_bit,
@@ -207,6 +293,8 @@ hipError_t radix_sort_impl(const uint64_t* keys_in, const V* vals_in, size_t n, 
                        gen_pos ? 1 : 0);
     return hipGetLastError();
   }
+  uint32_t* counts = reinterpret_cast<uint32_t*>(tmp);
+  void* scan_tmp = reinterpret_cast<char*>(tmp) + counts_bytes(n);
   const uint64_t* sk = keys_in;
   const V* sv = vals_in;
   int out = 0;
@@ -215,8 +303,7 @@ hipError_t radix_sort_impl(const uint64_t* keys_in, const V* vals_in, size_t n, 
     const int bits = end_bit - shift < 8 ? end_bit - shift : 8;
     uint64_t* dk = out == 0 ? k0 : k1;
     V* dv = out == 0 ? v0 : v1;
-    hipError_t e = radix_pass<uint64_t, V, 0>(sk, sv, dk, dv, nullptr, n, dn, shift, bits, gen, tmp, s, "psa_hist",
-                                              "psa_scatter");
+    hipError_t e = radix_pass<V, true, false>(sk, sv, dk, dv, nullptr, n, dn, shift, bits, gen, counts, scan_tmp, s);
     if (e != hipSuccess) return e;
     sk = dk;
     sv = dv;
@@ -227,12 +314,11 @@ hipError_t radix_sort_impl(const uint64_t* keys_in, const V* vals_in, size_t n, 
   return hipGetLastError();
 }
 
+int n_passes(int bits) { return (bits + 7) / 8; }
+
 }  // namespace
 
-size_t radix_tmp_bytes(size_t n) {
-  const size_t nb = (n + kTile - 1) / kTile + 1;
-  return align_up(256 * nb * sizeof(uint32_t), 256) + scan_tmp_bytes(256 * nb) + 256;
-}
+size_t radix_tmp_bytes(size_t n) { return counts_bytes(n) + scan_tmp_bytes(256 * ((n + kTile - 1) / kTile + 1)) + 256; }
 
 hipError_t radix_sort_pos(const uint64_t* keys_in, size_t n, int begin_bit, int end_bit, uint64_t* k0, uint32_t* v0,
                           uint64_t* k1, uint32_t* v1, void* tmp, hipStream_t s, int* which, const uint32_t* dn) {
@@ -246,33 +332,63 @@ hipError_t radix_sort_u64pay(const uint64_t* keys_in, const uint64_t* vals_in, s
                                    nullptr);
 }
 
-hipError_t unpermute_results(const uint32_t* posf, const uint64_t* vals, size_t n, uint32_t* p0, uint64_t* v0,
-                             uint32_t* p1, uint64_t* v1, uint64_t* out_vals, uint8_t* out_found, void* tmp,
-                             hipStream_t s, const uint32_t* dn) {
+// Replay record of pass p: counts at replay + p * stride, staging indices right after them.
+size_t replay_bytes(size_t n, int bits) {
+  return (size_t)n_passes(bits) * (counts_bytes(n) + align_up(n * sizeof(uint16_t), 256));
+}
+
+hipError_t radix_sort_keys_replay(const uint64_t* keys_in, size_t n, int begin_bit, int end_bit, uint64_t* k0,
+                                  uint64_t* k1, void* replay, void* tmp, hipStream_t s, int* which,
+                                  const uint32_t* dn) {
+  *which = 0;
   if (n == 0) return hipSuccess;
-  int bits = 1;
-  while (bits < 31 && ((size_t)1 << bits) < n) ++bits;  // positions fit in `bits` bits
-  const uint32_t* sk = posf;
-  const uint64_t* sv = vals;
-  int out = 0;
-  for (int shift = 0; shift < bits; shift += 8) {
-    const int b = bits - shift < 8 ? bits - shift : 8;
-    hipError_t e;
-    if (shift + b >= bits) {
-      e = radix_pass<uint32_t, uint64_t, 1>(sk, sv, nullptr, out_vals, out_found, n, dn, shift, b, false, tmp, s,
-                                            "unpermute_hist", "unpermute_scatter");
-    } else {
-      uint32_t* dk = out == 0 ? p0 : p1;
-      uint64_t* dv = out == 0 ? v0 : v1;
-      e = radix_pass<uint32_t, uint64_t, 0>(sk, sv, dk, dv, nullptr, n, dn, shift, b, false, tmp, s, "unpermute_hist",
-                                            "unpermute_scatter");
-      sk = dk;
-      sv = dv;
-      out ^= 1;
-    }
+  if (begin_bit < 0 || end_bit > 64 || begin_bit >= end_bit) return hipErrorInvalidValue;
+  const size_t stride = counts_bytes(n) + align_up(n * sizeof(uint16_t), 256);
+  const uint64_t* sk = keys_in;
+  int out = 0, p = 0;
+  for (int shift = begin_bit; shift < end_bit; shift += 8, ++p) {
+    const int bits = end_bit - shift < 8 ? end_bit - shift : 8;
+    char* rec = reinterpret_cast<char*>(replay) + (size_t)p * stride;
+    uint64_t* dk = out == 0 ? k0 : k1;
+    hipError_t e = radix_pass<uint32_t, false, true>(sk, nullptr, dk, nullptr,
+                                                     reinterpret_cast<uint16_t*>(rec + counts_bytes(n)), n, dn, shift,
+                                                     bits, false, reinterpret_cast<uint32_t*>(rec), tmp, s);
     if (e != hipSuccess) return e;
+    sk = dk;
+    *which = out;
+    out ^= 1;
   }
-  return hipSuccess;
+  return hipGetLastError();
+}
+
+hipError_t unsort_results(uint64_t* vals, uint8_t* found, size_t n, int bits, const void* replay, uint64_t* v_tmp,
+                          uint8_t* f_tmp, uint64_t* out_vals, uint8_t* out_found, hipStream_t s,
+                          const uint32_t* dn) {
+  if (n == 0) return hipSuccess;
+  const int P = n_passes(bits);
+  const size_t nb = (n + kTile - 1) / kTile;
+  const size_t stride = counts_bytes(n) + align_up(n * sizeof(uint16_t), 256);
+  const uint64_t* sv = vals;
+  const uint8_t* sf = found;
+  for (int p = P - 1; p >= 0; --p) {
+    const char* rec = reinterpret_cast<const char*>(replay) + (size_t)p * stride;
+    uint64_t* dv;
+    uint8_t* df;
+    if (p == 0) {
+      dv = out_vals, df = out_found;
+    } else if (sv == vals) {
+      dv = v_tmp, df = f_tmp;
+    } else {
+      dv = vals, df = found;
+    }
+    ProfScope prof("unsort", s);
+    hipLaunchKernelGGL(radix_unscatter_kernel, dim3(grid_for(nb)), dim3(kThreads), 0, s, sv, sf, dv, df,
+                       reinterpret_cast<const uint16_t*>(rec + counts_bytes(n)), n,
+                       reinterpret_cast<const uint32_t*>(rec), (uint32_t)nb, dn);
+    sv = dv;
+    sf = df;
+  }
+  return hipGetLastError();
 }
 
 }  // namespace hm

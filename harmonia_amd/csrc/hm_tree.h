@@ -64,9 +64,13 @@ struct hm_tree {
   hipEvent_t publish_ev = nullptr;
   bool publish_valid = false;
   uint64_t epoch = 0;
+  uint64_t relayouts = 0;
   uint64_t* pend_k = nullptr;
   uint64_t* pend_v = nullptr;
   size_t pend_n = 0, pend_cap = 0;
+  void* upd_buf = nullptr;       // grow-only device scratch of hm_update / hm_epoch_publish
+  size_t upd_bytes = 0;
+  uint32_t* upd_host = nullptr;  // pinned: the publish plan summary read back by the host
 
   hm::TreeView view() const {
     hm::TreeView tv{};
@@ -81,7 +85,8 @@ struct hm_tree {
     return tv;
   }
   uint64_t device_bytes() const {
-    return (uint64_t)inner_cap * NS * 8 + (uint64_t)inner_cap * 4 + leaf_cap * sizeof(hm::Leaf) + pend_cap * 16;
+    return (uint64_t)inner_cap * NS * 8 + (uint64_t)inner_cap * 4 + leaf_cap * sizeof(hm::Leaf) + pend_cap * 16 +
+           upd_bytes;
   }
 };
 
@@ -95,11 +100,12 @@ int resolve_search(const hm_tree* t, uint64_t n, const hm_search_opts* o, Search
 
 // Device scratch of one sort -> search -> (un-permute) pipeline over up to n queries.
 struct PipeBufs {
-  uint64_t *k0, *k1;  // PSA-sorted keys (ping-pong); reused by the un-permute as value buffers
-  uint32_t *p0, *p1;  // positions (ping-pong)
+  uint64_t *k0, *k1;  // PSA-sorted keys (ping-pong)
+  uint32_t *p0, *p1;  // positions (ping-pong; scatter and PSA-order outputs)
   void* tmp;          // radix counts + scan scratch
-  uint64_t* rv;       // PSA-order result values (un-permute input)
-  uint32_t* rp;       // PSA-order pos | found << 31 (un-permute input)
+  void* replay;       // per-pass offsets and staging indices (input-order output)
+  uint64_t *rv, *vt;  // PSA-order result values and the un-sort ping-pong buffer
+  uint8_t *rf, *ft;   // PSA-order found flags and their ping-pong buffer
 };
 size_t pipe_bytes(size_t n, int psa_bits, int out_mode);
 PipeBufs carve_pipe(char*& cur, size_t n, int psa_bits, int out_mode);

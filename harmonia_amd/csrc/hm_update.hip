@@ -8,6 +8,11 @@
 // are merged into their leaf's free slots.  When some leaf would overflow, every leaf is re-laid
 // out (overflowing leaves split evenly into ceil(total / fill) leaves) at offsets given by a device
 // scan, and the caller rebuilds the inner levels and re-scans the prefix array on the device.
+//
+// The common serving case (a few thousand keys, no leaf overflows) stays O(batch): one block sorts
+// and de-duplicates the batch in LDS, the keys are located, each leaf's keys form one run of the
+// sorted batch, and one thread per run plans (overflow? how many new keys?) and then applies it.
+#include "hm_block.h"
 #include "hm_internal.h"
 
 namespace hm {
@@ -68,15 +73,6 @@ __global__ void insert_starts_kernel(const uint32_t* __restrict__ leaf, const ui
     if (!found[i]) atomicMin(&start[leaf[i]], (uint32_t)i);
 }
 
-__global__ void overflow_kernel(const Leaf* __restrict__ leaves, const uint32_t* __restrict__ ins, size_t n_leaves,
-                                uint32_t* __restrict__ flags) {
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_leaves; i += stride) {
-    const uint32_t a = ins[i];
-    if (a && (uint32_t)leaf_count(leaves[i]) + a > (uint32_t)LEAF_KEYS) atomicOr(flags, 1u);
-  }
-}
-
 // Stream the merge of a leaf's keys with its inserts (the batch keys from `start` on that are not
 // found and map to this leaf; they are contiguous among the not-found keys because the batch is
 // sorted) in key order, calling emit(i, key, value) for i = 0, 1, ...; returns the merged count.
@@ -101,32 +97,6 @@ __device__ int merge_leaf(const Leaf& l, uint32_t n_ins, uint32_t start, const u
     ++c;
   }
   return c;
-}
-
-__global__ void merge_in_place_kernel(Leaf* __restrict__ leaves, const uint32_t* __restrict__ ins,
-                                      const uint32_t* __restrict__ start, const uint64_t* __restrict__ bk,
-                                      const uint64_t* __restrict__ bv, const uint8_t* __restrict__ found,
-                                      size_t n_leaves) {
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_leaves; i += stride) {
-    const uint32_t a = ins[i];
-    if (!a) continue;
-    const Leaf l = leaves[i];
-    Leaf o;
-#pragma unroll
-    for (int s = 0; s < LEAF_KEYS; ++s) {
-      o.key[s] = KEY_MAX;
-      o.val[s] = 0ull;
-    }
-    // the overflow check guarantees at most LEAF_KEYS merged keys here
-    merge_leaf(l, a, start[i], bk, bv, found, [&](int idx, uint64_t k, uint64_t v) {
-      if (idx < LEAF_KEYS) {
-        o.key[idx] = k;
-        o.val[idx] = v;
-      }
-    });
-    leaves[i] = o;
-  }
 }
 
 __global__ void new_counts_kernel(const Leaf* __restrict__ leaves, const uint32_t* __restrict__ ins, size_t n_leaves,
@@ -177,7 +147,166 @@ __global__ void relayout_kernel(const Leaf* __restrict__ old_leaves, size_t n_ol
   }
 }
 
+constexpr int kSortThreads = 1024;
+
+// Bitonic sort of (key, staging index) in LDS; the index breaks ties, so the last write of a key
+// ends its run of equal keys.  Padding (KEY_MAX, ~0) sorts after every real key (KEY_MAX is never
+// staged).
+__global__ __launch_bounds__(kSortThreads) void small_sort_dedupe_kernel(const uint64_t* __restrict__ pk,
+                                                                         const uint64_t* __restrict__ pv, uint32_t n,
+                                                                         uint64_t* __restrict__ uk,
+                                                                         uint64_t* __restrict__ uv,
+                                                                         uint32_t* __restrict__ dm) {
+  __shared__ uint64_t sk[kSmallUpdate];
+  __shared__ uint32_t si[kSmallUpdate];
+  __shared__ uint32_t scan_sh[kSortThreads / 64 + 1];
+  uint32_t N = 1;
+  while (N < n) N <<= 1;
+  for (uint32_t i = threadIdx.x; i < N; i += kSortThreads) {
+    sk[i] = i < n ? pk[i] : KEY_MAX;
+    si[i] = i < n ? i : 0xFFFFFFFFu;
+  }
+  __syncthreads();
+  for (uint32_t k = 2; k <= N; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = threadIdx.x; i < N; i += kSortThreads) {
+        const uint32_t p = i ^ j;
+        if (p > i) {
+          const uint64_t a = sk[i], b = sk[p];
+          const uint32_t ia = si[i], ib = si[p];
+          const bool gt = a > b || (a == b && ia > ib);
+          if (gt == ((i & k) == 0)) {
+            sk[i] = b;
+            sk[p] = a;
+            si[i] = ib;
+            si[p] = ia;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  constexpr int kPer = kSmallUpdate / kSortThreads;  // consecutive elements per thread
+  const uint32_t b0 = threadIdx.x * kPer;
+  bool keep[kPer];
+  uint32_t cnt = 0;
+#pragma unroll
+  for (int r = 0; r < kPer; ++r) {
+    const uint32_t i = b0 + r;
+    keep[r] = i < n && (i + 1 == n || sk[i] != sk[i + 1]);
+    cnt += keep[r] ? 1u : 0u;
+  }
+  uint32_t total = 0;
+  uint32_t off = block_exclusive_scan<uint32_t, kSortThreads>(cnt, scan_sh, total);
+#pragma unroll
+  for (int r = 0; r < kPer; ++r)
+    if (keep[r]) {
+      uk[off] = sk[b0 + r];
+      uv[off] = pv[si[b0 + r]];
+      ++off;
+    }
+  if (threadIdx.x == 0) *dm = total;
+}
+
+__global__ void unique_count_kernel(const uint32_t* __restrict__ keep, const uint32_t* __restrict__ scan, size_t n,
+                                    uint32_t* __restrict__ dm) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dm = scan[n - 1] + keep[n - 1];
+}
+
+__device__ __forceinline__ bool run_head(const uint32_t* __restrict__ leaf, uint32_t i) {
+  return i == 0 || leaf[i] != leaf[i - 1];
+}
+
+__global__ void plan_runs_kernel(const Leaf* __restrict__ leaves, const uint32_t* __restrict__ leaf,
+                                 const uint8_t* __restrict__ found, size_t n_max, const uint32_t* __restrict__ dm,
+                                 uint32_t* __restrict__ summary) {
+  const uint32_t m = dm ? *dm : (uint32_t)n_max;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  // grid-stride loop with a wave-uniform trip count, so the ballot below sees the whole wave
+  const uint32_t m_up = (m + 63u) & ~63u;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m_up; i += stride) {
+    const bool valid = i < m;
+    const bool is_new = valid && !found[i];
+    const uint64_t nb = __ballot(is_new);
+    if ((threadIdx.x & 63) == 0 && nb) atomicAdd(&summary[1], (uint32_t)__popcll(nb));
+    if (valid && run_head(leaf, i)) {
+      // count this leaf's new keys, stopping once it cannot fit
+      const uint32_t L = leaf[i];
+      const uint32_t room = (uint32_t)(LEAF_KEYS - leaf_count(leaves[L]));
+      uint32_t ins = 0;
+      for (uint32_t j = i; j < m && leaf[j] == L && ins <= room; ++j) ins += found[j] ? 0u : 1u;
+      if (ins > room) atomicOr(&summary[0], 1u);
+    }
+  }
+}
+
+__global__ void apply_runs_kernel(Leaf* __restrict__ leaves, const uint32_t* __restrict__ leaf,
+                                  const uint32_t* __restrict__ slot, const uint8_t* __restrict__ found,
+                                  const uint64_t* __restrict__ uk, const uint64_t* __restrict__ uv, size_t n_max,
+                                  const uint32_t* __restrict__ dm) {
+  const uint32_t m = dm ? *dm : (uint32_t)n_max;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
+    if (!run_head(leaf, i)) continue;
+    const uint32_t L = leaf[i];
+    uint32_t ins = 0;
+    for (uint32_t j = i; j < m && leaf[j] == L; ++j) {
+      if (found[j])
+        leaves[L].val[slot[j]] = uv[j];
+      else
+        ++ins;
+    }
+    if (!ins) continue;
+    const Leaf l = leaves[L];  // this thread's value writes above are visible to its own load
+    Leaf o;
+#pragma unroll
+    for (int s = 0; s < LEAF_KEYS; ++s) {
+      o.key[s] = KEY_MAX;
+      o.val[s] = 0ull;
+    }
+    // the plan guarantees at most LEAF_KEYS merged keys
+    merge_leaf(l, ins, i, uk, uv, found, [&](int idx, uint64_t k, uint64_t v) {
+      if (idx < LEAF_KEYS) {
+        o.key[idx] = k;
+        o.val[idx] = v;
+      }
+    });
+    leaves[L] = o;
+  }
+}
+
 }  // namespace
+
+hipError_t small_sort_dedupe(const uint64_t* pk, const uint64_t* pv, uint32_t n, uint64_t* uk, uint64_t* uv,
+                             uint32_t* dm, hipStream_t s) {
+  if (n == 0 || n > kSmallUpdate) return hipErrorInvalidValue;
+  ProfScope prof("update", s);
+  hipLaunchKernelGGL(small_sort_dedupe_kernel, dim3(1), dim3(kSortThreads), 0, s, pk, pv, n, uk, uv, dm);
+  return hipGetLastError();
+}
+
+hipError_t unique_count(const uint32_t* keep, const uint32_t* keep_scan, size_t n, uint32_t* dm, hipStream_t s) {
+  if (!n) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(unique_count_kernel, dim3(1), dim3(64), 0, s, keep, keep_scan, n, dm);
+  return hipGetLastError();
+}
+
+hipError_t plan_runs(const Leaf* leaves, const uint32_t* leaf, const uint8_t* found, size_t n, const uint32_t* dm,
+                     uint32_t* summary, hipStream_t s) {
+  if (!n) return hipSuccess;
+  ProfScope prof("update", s);
+  hipLaunchKernelGGL(plan_runs_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, leaves, leaf, found, n, dm, summary);
+  return hipGetLastError();
+}
+
+hipError_t apply_runs(Leaf* leaves, const uint32_t* leaf, const uint32_t* slot, const uint8_t* found,
+                      const uint64_t* uk, const uint64_t* uv, size_t n, const uint32_t* dm, hipStream_t s) {
+  if (!n) return hipSuccess;
+  ProfScope prof("update", s);
+  hipLaunchKernelGGL(apply_runs_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, leaves, leaf, slot, found, uk, uv, n,
+                     dm);
+  return hipGetLastError();
+}
 
 hipError_t mark_last_of_runs(const uint64_t* sorted_keys, size_t n, uint32_t* keep, hipStream_t s) {
   if (!n) return hipSuccess;
@@ -214,22 +343,6 @@ hipError_t insert_starts(const uint32_t* leaf, const uint8_t* found, size_t n, u
   if (!n) return hipSuccess;
   ProfScope prof("update", s);
   hipLaunchKernelGGL(insert_starts_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, leaf, found, n, ins_start);
-  return hipGetLastError();
-}
-
-hipError_t overflow_check(const Leaf* leaves, const uint32_t* ins_per_leaf, size_t n_leaves, uint32_t* d_flags,
-                          hipStream_t s) {
-  ProfScope prof("update", s);
-  hipLaunchKernelGGL(overflow_kernel, dim3(grid_for(n_leaves, 256)), dim3(256), 0, s, leaves, ins_per_leaf, n_leaves,
-                     d_flags);
-  return hipGetLastError();
-}
-
-hipError_t merge_in_place(Leaf* leaves, const uint32_t* ins_per_leaf, const uint32_t* ins_start, const uint64_t* bkeys,
-                          const uint64_t* bvals, const uint8_t* found, size_t n_leaves, hipStream_t s) {
-  ProfScope prof("update", s);
-  hipLaunchKernelGGL(merge_in_place_kernel, dim3(grid_for(n_leaves, 256)), dim3(256), 0, s, leaves, ins_per_leaf,
-                     ins_start, bkeys, bvals, found, n_leaves);
   return hipGetLastError();
 }
 

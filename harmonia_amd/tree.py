@@ -10,6 +10,7 @@ Arrays may be numpy arrays (host memory, staged by the library) or torch tensors
 (device pointers, used in place).
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -77,6 +78,20 @@ class BPlusTree:
         t = cls(**params)
         t.build(keys, vals, keys_sorted=keys_sorted)
         return t
+
+    @classmethod
+    def load(cls, path, **params):
+        """Reload a snapshot written by save() (fanout and leaf fill come from the file)."""
+        t = cls(**params)
+        p = HmParams.from_buffer_copy(t.params)
+        h = ctypes.c_void_p()
+        check(t._lib.hm_tree_load(os.fsencode(path), ctypes.byref(p), ctypes.byref(h)))
+        t._h = h
+        return t
+
+    def save(self, path):
+        """Binary snapshot of the published tree."""
+        check(self._lib.hm_tree_save(self._handle(), os.fsencode(path)))
 
     def build(self, keys=None, vals=None, keys_sorted=False):
         """Bulk build (replaces the reference's insert loop + BPlusTree::sort)."""
@@ -254,6 +269,27 @@ class BPlusTree:
         return self.export()["prefix"]
 
 
+def read_keys_text(path):
+    """Keys of a text dataset in the reference's format (one decimal key per line)."""
+    lib = _lib.load()
+    n = ctypes.c_uint64(0)
+    check(lib.hm_read_keys_text(os.fsencode(path), None, 0, ctypes.byref(n)))
+    out = np.empty(max(n.value, 1), dtype=np.uint64)
+    check(lib.hm_read_keys_text(os.fsencode(path), _ptr(out), n.value, ctypes.byref(n)))
+    return out[:n.value]
+
+
+def read_ranges_text(path):
+    """(starts, ends) of a range file: a start line and an end line per range."""
+    lib = _lib.load()
+    n = ctypes.c_uint64(0)
+    check(lib.hm_read_ranges_text(os.fsencode(path), None, None, 0, ctypes.byref(n)))
+    a = np.empty(max(n.value, 1), dtype=np.uint64)
+    b = np.empty(max(n.value, 1), dtype=np.uint64)
+    check(lib.hm_read_ranges_text(os.fsencode(path), _ptr(a), _ptr(b), n.value, ctypes.byref(n)))
+    return a[:n.value], b[:n.value]
+
+
 def psa_sort(keys, begin_bit=48, end_bit=64, stream=None):
     """Stable radix sort of (key, position) over key bits [begin_bit, end_bit) on the GPU."""
     lib = _lib.load()
@@ -332,6 +368,7 @@ class Server:
         check(self._lib.hm_server_create(tree._handle(), ctypes.byref(p), ctypes.byref(h)))
         self._h = h
         self._outs = {}
+        self._inputs = {}  # the library reads inputs when a request joins a batch: keep them alive
 
     def lookup(self, keys):
         keys = np.ascontiguousarray(keys, dtype=np.uint64)
@@ -340,6 +377,7 @@ class Server:
         t = ctypes.c_uint64(0)
         check(self._lib.hm_server_lookup(self._h, _ptr(keys), len(keys), _ptr(v), _ptr(f), ctypes.byref(t)))
         self._outs[t.value] = (v, f)
+        self._inputs[t.value] = keys
         return t.value
 
     def range(self, starts, lens):
@@ -355,6 +393,7 @@ class Server:
         off = np.zeros(len(lens) + 1, dtype=np.uint64)
         off[1:] = np.cumsum(lens.astype(np.uint64))
         self._outs[t.value] = (off, k, v, c)
+        self._inputs[t.value] = (starts, lens)
         return t.value
 
     def update(self, keys, vals):
@@ -363,6 +402,7 @@ class Server:
         t = ctypes.c_uint64(0)
         check(self._lib.hm_server_update(self._h, _ptr(keys), _ptr(vals), len(keys), ctypes.byref(t)))
         self._outs[t.value] = ()
+        self._inputs[t.value] = (keys, vals)
         return t.value
 
     def wait(self, ticket):
@@ -370,10 +410,12 @@ class Server:
         st = ctypes.c_int(0)
         ep = ctypes.c_uint64(0)
         check(self._lib.hm_server_wait(self._h, ticket, ctypes.byref(st), ctypes.byref(ep)))
+        self._inputs.pop(ticket, None)
         return st.value, ep.value, self._outs.pop(ticket)
 
     def drain(self):
         check(self._lib.hm_server_drain(self._h))
+        self._inputs.clear()  # every request has completed
 
     def stats(self):
         s = _lib.HmServerStats()
@@ -385,8 +427,9 @@ class Server:
 
     def close(self):
         if self._h is not None:
-            self._lib.hm_server_destroy(self._h)
+            self._lib.hm_server_destroy(self._h)  # finishes every pending request first
             self._h = None
+            self._inputs.clear()
 
     def __del__(self):
         try:

@@ -66,6 +66,7 @@ typedef struct hm_stats {
   uint64_t bytes_leaves;
   uint64_t bytes_device;   /* everything the tree holds on the device */
   uint64_t epoch;          /* number of published update epochs */
+  uint64_t relayouts;      /* epochs that re-laid out every leaf (some leaf overflowed) */
 } hm_stats;
 
 /* hm_search_opts.output_mode (with PSA on; without PSA results are always in input order) */
@@ -171,7 +172,11 @@ typedef struct hm_server_stats {
   double span_s;                          /* first submit -> last completion */
 } hm_server_stats;
 int hm_server_create(hm_tree* t, const hm_server_params* p, hm_server** out);
-/* Requests: inputs are copied at submission; outputs (host memory) are written before completion. */
+/* Requests: every buffer (inputs and outputs, host memory) must stay valid and unchanged until the
+ * request completes (hm_server_wait returns or hm_server_drain returns); inputs are read once, when
+ * the request joins a batch, and outputs are written before completion.  Requests pending at the
+ * same time may be ordered either way (reads may run before or after a pending update); a request
+ * submitted after another one completed sees its effect. */
 int hm_server_lookup(hm_server* s, const uint64_t* keys, uint32_t n, uint64_t* out_vals, uint8_t* out_found,
                      uint64_t* ticket);
 int hm_server_range(hm_server* s, const uint64_t* starts, const uint32_t* lens, uint32_t n, uint64_t* out_keys,
@@ -209,6 +214,22 @@ int hm_tree_build_tuned(const uint64_t* keys, const uint64_t* vals, uint64_t n, 
                         const uint64_t* sample, uint64_t m, uint32_t flags, hm_tree** out, hm_tune_result* res);
 
 int hm_tree_stats(const hm_tree* t, hm_stats* s);
+
+/* Datasets in the reference's text format: one decimal key per line, read by the reference with
+ * getline + sscanf("%lld") (H/src/main.cpp:50-68, H/src/bpt.h:32-35).  Every non-empty line must
+ * hold one integer in [0, 2^63); anything else fails with the line number.  keys = NULL, cap = 0
+ * counts only. */
+int hm_read_keys_text(const char* path, uint64_t* keys, uint64_t cap, uint64_t* n);
+/* Range files: a start line and an end line per range (H/src/range.cu:348-357); *n = ranges. */
+int hm_read_ranges_text(const char* path, uint64_t* starts, uint64_t* ends, uint64_t cap, uint64_t* n);
+
+/* Binary snapshot for a fast reload (no counterpart in the reference, which rebuilds the tree from
+ * the text file on every run, H/src/main.cpp:50-68).  Save needs no staged updates.  Load takes
+ * fanout and leaf fill from the file and the rest of p (NULL = defaults); it uploads the leaves,
+ * rebuilds the inner levels and prefix array on the device and rejects a file whose stored inner
+ * sections differ from them. */
+int hm_tree_save(const hm_tree* t, const char* path);
+int hm_tree_load(const char* path, const hm_params* p, hm_tree** out);
 
 /* Copy the device layout to host buffers (any may be NULL): inner (n_inner * node_slots u64),
  * prefix (n_inner u32), leaves (n_leaves * 16 u64: 8 keys then 8 values per leaf). */

@@ -264,6 +264,45 @@ def test_update_in_place_and_relayout(gpu, fill):
         assert rebuilt_any  # full leaves force the re-layout path
 
 
+@pytest.mark.parametrize("size", [1, 4095, 4096, 4097, 30000])
+@pytest.mark.parametrize("fill", [8, 5])
+def test_update_batch_sizes(gpu, size, fill):
+    """Batches on both sides of the one-block sort (<= 4096 staged keys) and the radix path; each
+    mixes new keys, existing keys and repeats (the last write wins), staged in two calls."""
+    t, keys, vals, sk, sv = make(60000, 64, fill, seed=71 + size)
+    g = W.rng(71, size)
+    n_new = (size + 1) // 2
+    new = W.gen_misses(len(keys), n_new, 71 + size) ^ U64(3 << 52)
+    new = new[~np.isin(new, sk)]
+    old = sk[g.integers(0, len(sk), size - len(new))]
+    bk = np.concatenate([new, old])
+    g.shuffle(bk)
+    bk[len(bk) // 3:len(bk) // 3 + len(bk) // 10] = bk[:len(bk) // 10]  # repeats
+    bv = g.integers(0, 1 << 64, len(bk), dtype=np.uint64)
+    half = len(bk) // 2
+    t.update(bk[:half], bv[:half])
+    t.update(bk[half:], bv[half:])
+    t.publish()
+    cur_k, cur_v = oracle.upsert_sorted(sk, sv, bk, bv)
+    full_check(t, cur_k, cur_v, 90 + size)
+
+
+def test_update_device_buffers(gpu):
+    import torch
+
+    t, keys, vals, sk, sv = make(30000, 64, 6, seed=81)
+    new = W.gen_misses(len(keys), 3000, 81)
+    nv = W.splitmix64_mix(new)
+    t.update(torch.from_numpy(new.view(np.int64)).to(gpu), torch.from_numpy(nv.view(np.int64)).to(gpu))
+    t.publish()
+    cur_k, cur_v = oracle.upsert_sorted(sk, sv, new, nv)
+    full_check(t, cur_k, cur_v, 82)
+    bad = torch.from_numpy(np.array([5, KEY_MAX], dtype=U64).view(np.int64)).to(gpu)
+    with pytest.raises(HarmoniaError):
+        t.update(bad, bad)
+    assert t.stats().epoch == 1
+
+
 def test_update_epoch_snapshot(gpu):
     t, keys, vals, sk, sv = make(20000, 64, 6, seed=61)
     bk = sk[:100]
@@ -298,3 +337,31 @@ def test_update_empty_tree_and_single_inserts(gpu):
         sk = np.array(sorted(ks), dtype=U64)
         full_check(t, sk, sk * U64(3), i)
     assert t.search(30) == 90 and t.search(31) is None
+
+
+@pytest.mark.parametrize("F", [32, 64, 128])
+def test_snapshot_roundtrip(gpu, tmp_path, F):
+    """save -> load gives the identical layout and answers; a corrupted file is rejected."""
+    t, keys, vals, sk, sv = make(50000, F, 6, seed=91)
+    new = W.gen_misses(len(keys), 7000, 91)
+    t.insert_parallel(new, W.splitmix64_mix(new))  # includes a publish
+    cur_k, cur_v = oracle.upsert_sorted(sk, sv, new, W.splitmix64_mix(new))
+    path = tmp_path / "tree.hmsnap"
+    t.save(path)
+    t2 = BPlusTree.load(path)
+    e1, e2 = t.export(), t2.export()
+    for k in ("inner", "prefix", "leaves"):
+        assert np.array_equal(e1[k], e2[k])
+    assert t2.stats().fanout == F and t2.stats().epoch == t.stats().epoch
+    full_check(t2, cur_k, cur_v, 92)
+    raw = bytearray(path.read_bytes())
+    for off in (200, len(raw) - 3):  # a leaf byte; a prefix byte
+        bad = bytearray(raw)
+        bad[off] ^= 0x5A
+        p2 = tmp_path / f"bad{off}.hmsnap"
+        p2.write_bytes(bytes(bad))
+        with pytest.raises(HarmoniaError):
+            BPlusTree.load(p2)
+    (tmp_path / "short.hmsnap").write_bytes(bytes(raw[:1000]))
+    with pytest.raises(HarmoniaError):
+        BPlusTree.load(tmp_path / "short.hmsnap")

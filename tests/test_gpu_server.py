@@ -75,12 +75,28 @@ def test_alone_equals_mixed(tree_and_keys, coalesce):
     srv.drain()
     s = srv.stats()
     assert s["completed"] >= 3 * len(reqs) - 5 and s["p99_us"] >= s["p50_us"] > 0
-    if coalesce:
-        assert s["launches"] < s["completed"]
+    srv.close()
+
+
+def test_coalescing_under_load(tree_and_keys):
+    """With one worker busy, requests that arrive meanwhile join the next batch: fewer launches than
+    requests, and every result is still exact."""
+    t, sk, sv = tree_and_keys
+    srv = Server(t, max_inflight=1)
+    qs = [W.point_queries(sk, 5000 + 37 * i, 300 + i) for i in range(200)]
+    tickets = [srv.lookup(q) for q in qs]
+    for q, tk in zip(qs, tickets):
+        st, ep, out = srv.wait(tk)
+        assert st == 0 and check_lookup(sk, sv, q, out)
+    s = srv.stats()
+    assert s["completed"] == 200 and s["launches"] < 200
     srv.close()
 
 
 def test_updates_between_epochs(gpu):
+    """Reads interleaved with update batches: every read matches the whole state of the epoch it
+    reports (initial keys plus, in submission order, every update published up to that epoch), and
+    a read submitted after an update completed sees it."""
     keys, vals = W.gen_keys(100000, 31)
     t = BPlusTree.from_keys(keys, vals, leaf_fill=6)
     sk = np.sort(keys)
@@ -88,23 +104,32 @@ def test_updates_between_epochs(gpu):
     srv = Server(t, max_inflight=3)
     new = W.gen_misses(100000, 20000, 31)
     nv = W.splitmix64_mix(new ^ U64(5))
-    states = {0: (sk, sv)}
     probe = np.concatenate([sk[:2000], new])
     tickets = []
-    cur = (sk, sv)
     for step in range(5):
         part = slice(step * 4000, (step + 1) * 4000)
         tickets.append(("read", srv.lookup(probe)))
-        tickets.append(("update", srv.update(new[part], nv[part])))
-        cur = oracle.upsert_sorted(cur[0], cur[1], new[part], nv[part])
-        states[step + 1] = cur
+        tickets.append(("update", srv.update(new[part], nv[part]), part))
     tickets.append(("read", srv.lookup(probe)))
-    for kind, tk in tickets:
-        st, ep, out = srv.wait(tk)
+    reads, ups = [], []
+    for item in tickets:
+        st, ep, out = srv.wait(item[1])
         assert st == 0
-        if kind == "read":
-            k2, v2 = states[ep]  # the whole snapshot of the epoch the read ran in
-            assert check_lookup(k2, v2, probe, out), ep
+        (reads if item[0] == "read" else ups).append((ep, out if item[0] == "read" else item[2]))
+    states = {0: (sk, sv)}
+    cur = (sk, sv)
+    for ep in sorted({e for e, _ in ups}):
+        for e, part in ups:  # submission order within the epoch
+            if e == ep:
+                cur = oracle.upsert_sorted(cur[0], cur[1], new[part], nv[part])
+        states[ep] = cur
+    for ep, out in reads:
+        k2, v2 = states[max(e for e in states if e <= ep)]
+        assert check_lookup(k2, v2, probe, out), ep
+    # after every update completed, a new read sees all of them
+    final = oracle.upsert_sorted(sk, sv, new, nv)
+    st, ep, out = srv.wait(srv.lookup(probe))
+    assert st == 0 and check_lookup(final[0], final[1], probe, out)
     srv.drain()
     assert srv.stats()["epochs"] >= 1
     srv.close()
