@@ -99,7 +99,8 @@ def batched_uneven(dev, n_batches=10000, keys_log2=25, seed=4):
     pools = [W.point_queries(sk, pool_n, cseed + 1),
              W.zipf_queries(sk, pool_n, cseed + 2),
              W.point_queries(sk, pool_n, cseed + 3, order="runs4096")]
-    sample = np.concatenate([p[: 1 << 19] for p in pools])
+    # a sample as large as the batcher's buffer, so the tuner sees the batch size a flush runs at
+    sample = np.concatenate([p[: (1 << 24) // 3] for p in pools])
     W.rng(cseed, 7).shuffle(sample)
     t0 = time.perf_counter()
     tree, tune = BPlusTree.from_keys_tuned(keys, vals, sample, quick=True)
@@ -218,6 +219,9 @@ def main():
     ap.add_argument("--no-served", action="store_true", help="skip the served-load runs")
     ap.add_argument("--served-keys-log2", type=int, default=27)
     ap.add_argument("--served-requests", type=int, default=20000)
+    ap.add_argument("--calibrate", action="store_true",
+                    help="only run a plain 1-GiB streaming kernel (read 1 GiB, write 1 GiB), the reference point "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the tree is replicated per GPU "
@@ -225,6 +229,17 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.calibrate:
+        n = (1 << 30) // 8
+        x = torch.arange(n, dtype=torch.int64, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.bitwise_xor(x, 1, out=y)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "bitwise_xor over 2^27 int64", "bytes_read": n * 8, "bytes_written": n * 8,
+                          "launches": 3}))
+        return
 
     import harmonia_amd
     from harmonia_amd import BPlusTree
@@ -259,33 +274,43 @@ def main():
           "runs4096": torch.from_numpy(q_runs.view(np.int64)).to(dev)}
     torch.cuda.synchronize()
 
-    variants = [("unsorted", 16), ("unsorted", 0), ("runs4096", 16), ("runs4096", 0)]
+    # (order, PSA bits, output mode): mode 1 = results in input order (the headline), 3 = PSA order plus
+    # positions (the reference's own contract)
+    variants = [("unsorted", 16, 1), ("unsorted", 0, 1), ("runs4096", 16, 1), ("runs4096", 0, 1),
+                ("unsorted", 16, 3)]
     if args.variants == "headline":
         variants = variants[:1]
+    out_pos = torch.empty(n_q, dtype=torch.int32, device=dev)
     results = {}
-    for order, psa in variants:
-        fn = lambda o=order, p=psa: tree.search_batch(dq[o], out_vals=out_v, out_found=out_f, stream=stream,
-                                                       psa_bits=p)
+    for order, psa, mode in variants:
+        fn = lambda o=order, p=psa, m=mode: tree.search_batch(
+            dq[o], out_vals=out_v, out_found=out_f, stream=stream, psa_bits=p, output_mode=m if p else 0,
+            out_pos=out_pos if m == 3 else None)
         ms, wall_ms = time_steps(fn, args.steps, args.warmup, stream)
         # kernel-level timing of the same call, HIP events on the launch stream
         lib.hm_profile_reset()
         lib.hm_profile_enable(1)
-        for _ in range(max(3, args.steps // 4)):
+        reps = max(3, args.steps // 4)
+        for _ in range(reps):
             fn()
         torch.cuda.synchronize()
-        search_ms, launches = stage_avg(lib, "search")
-        hist_ms, _ = stage_avg(lib, "psa_hist")
-        scat_ms, _ = stage_avg(lib, "psa_scatter")
-        scan_ms, _ = stage_avg(lib, "scan")
+        search_ms, launches = stage_avg(lib, "search")  # one search launch per step
+
+        def per_step(stage):
+            avg, cnt = stage_avg(lib, stage)
+            return avg * cnt / reps if avg else None
+
+        hist_ms, scat_ms, scan_ms, unsort_ms = (per_step(x) for x in ("psa_hist", "psa_scatter", "scan", "unsort"))
         lib.hm_profile_enable(0)
         lib.hm_profile_reset()
-        key = f"{order}_psa{psa}"
+        key = f"{order}_psa{psa}" + ("_psa_order" if mode == 3 else "")
         results[key] = {"ms_per_step": round(ms, 4), "Mq_per_s": round(n_q / ms / 1e3, 1),
                         "search_kernel_ms": round(search_ms, 4) if search_ms else None,
                         "search_kernel_Mq_per_s": round(n_q / search_ms / 1e3, 1) if search_ms else None,
                         "psa_hist_ms": round(hist_ms, 4) if hist_ms else None,
                         "psa_scatter_ms": round(scat_ms, 4) if scat_ms else None,
-                        "scan_ms": round(scan_ms, 4) if scan_ms else None}
+                        "scan_ms": round(scan_ms, 4) if scan_ms else None,
+                        "unsort_ms": round(unsort_ms, 4) if unsort_ms else None}
         log(key, json.dumps(results[key]))
 
     # correctness spot check of the headline output (size-independent property)
@@ -341,7 +366,8 @@ def main():
         "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "u64", "data": "synthetic",
         "config": {"workload": "plain batched GPU search: fanout-64 tree of 2^26 uniform uint64 keys, 2^24 point "
-                               "queries (90% hits), unsorted, PSA 16 bits", "keys": n_keys, "queries": n_q,
+                               "queries (90% hits), unsorted, PSA 16 bits, results in input order", "keys": n_keys,
+                   "queries": n_q,
                    "fanout": 64, "leaf_keys": 8, "order": "unsorted", "psa_bits": 16, "hit_frac": 0.9},
         "roofline": roofline,
         "e2e_roofline_frac_psa192B": round(e2e_achieved / PEAK_HBM_GBS, 4),

@@ -143,6 +143,12 @@ int ensure_ws(hm_tree* t, hipStream_t s, size_t bytes, Workspace** out) {
   return HM_OK;
 }
 
+int reserve_workspace(hm_tree* t, hipStream_t s, size_t bytes) {
+  std::lock_guard<std::mutex> g(t->mu);
+  Workspace* w = nullptr;
+  return ensure_ws(t, s, bytes, &w);
+}
+
 int choose_lds_levels(const hm_tree* t, int req) {
   const TreeView tv = t->view();
   int best = 0;
@@ -264,14 +270,71 @@ int plan_levels(uint64_t n_leaves, int F, uint32_t* level_start, uint32_t* n_inn
   return (int)sizes.size();
 }
 
-// Rebuild every inner level and the prefix array from the current leaves (device only).
+// Grow-only scratch of re-layouts and inner rebuilds (a hipFree synchronizes the device).
+int ensure_rl(hm_tree* t, size_t bytes, hipStream_t s) {
+  if (t->rl_bytes >= bytes) return HM_OK;
+  const size_t want = std::max(bytes, t->rl_bytes + t->rl_bytes / 2);
+  if (t->p.mem_budget && t->device_bytes() - t->rl_bytes + want > t->p.mem_budget)
+    return set_err(HM_EBUDGET, "re-layout scratch of %llu bytes exceeds the device-memory budget",
+                   (unsigned long long)want);
+  if (t->rl_buf) {
+    HM_TRY(hipStreamSynchronize(s));
+    HM_TRY(hipFree(t->rl_buf));
+    t->rl_buf = nullptr;
+    t->rl_bytes = 0;
+  }
+  HM_TRY(hipMalloc(&t->rl_buf, want));
+  t->rl_bytes = want;
+  return HM_OK;
+}
+
+size_t inner_scratch_bytes(uint64_t n_leaves, int F, uint32_t n_inner) {
+  return align256(std::max<uint64_t>(n_leaves, 1) * 8) + align256(std::max<uint64_t>((n_leaves + F - 1) / F, 1) * 8) +
+         align256((size_t)n_inner * 4) + align256(scan_tmp_bytes(n_inner));
+}
+
+// Build every inner level and the prefix array of `leaves` into (inner, prefix), which must hold
+// the n_inner nodes plan_levels gives; scratch (inner_scratch_bytes) comes from rl_buf at offset
+// `scratch_off`.  Enqueued on s, no host synchronization.
+int build_inner_into(hm_tree* t, const Leaf* leaves, uint64_t n_leaves, uint64_t* inner, uint32_t* prefix,
+                     const Levels& lv, size_t scratch_off, hipStream_t s) {
+  char* cur = (char*)t->rl_buf + scratch_off;
+  uint64_t* minA = (uint64_t*)cur;
+  cur += align256(std::max<uint64_t>(n_leaves, 1) * 8);
+  uint64_t* minB = (uint64_t*)cur;
+  cur += align256(std::max<uint64_t>((n_leaves + t->F - 1) / t->F, 1) * 8);
+  uint32_t* nchild = (uint32_t*)cur;
+  cur += align256((size_t)lv.n_inner * 4);
+  void* scan_tmp = cur;
+  HM_TRY(leaf_min_keys(leaves, n_leaves, minA, s));
+  uint64_t m_child = n_leaves;
+  uint64_t *cmin = minA, *nmin = minB;
+  for (int l = lv.height - 1; l >= 0; --l) {
+    const uint64_t m = lv.level_start[l + 1] - lv.level_start[l];
+    HM_TRY(build_level(cmin, m_child, t->F, inner + (size_t)lv.level_start[l] * t->NS, nmin,
+                       nchild + lv.level_start[l], m, s));
+    std::swap(cmin, nmin);
+    m_child = m;
+  }
+  HM_TRY(exclusive_scan_u32(nchild, prefix, lv.n_inner, 1u, scan_tmp, s));
+  return HM_OK;
+}
+
+void set_levels(hm_tree* t, const Levels& lv) {
+  t->height = lv.height;
+  t->n_inner = lv.n_inner;
+  for (int i = 0; i <= MAX_LEVELS; ++i) t->level_start[i] = lv.level_start[i];
+  t->lds_levels = choose_lds_levels(t, t->p.lds_levels);
+  t->layout_version++;  // captured pipelines hold the old pointers and level table
+}
+
+// Rebuild every inner level and the prefix array of the active leaves in place (build / load).
 int rebuild_inner(hm_tree* t, hipStream_t s) {
-  uint32_t ls[MAX_LEVELS + 1];
-  uint32_t n_inner = 0;
-  const int h = plan_levels(t->n_leaves, t->F, ls, &n_inner);
-  if (h < 0) return set_err(HM_EINVAL, "tree would exceed %d levels", MAX_LEVELS);
-  if (n_inner > t->inner_cap) {
-    const uint64_t extra = (uint64_t)(n_inner - t->inner_cap) * (t->NS * 8 + 4);
+  Levels lv;
+  lv.height = plan_levels(t->n_leaves, t->F, lv.level_start, &lv.n_inner);
+  if (lv.height < 0) return set_err(HM_EINVAL, "tree would exceed %d levels", MAX_LEVELS);
+  if (lv.n_inner > t->inner_cap) {
+    const uint64_t extra = (uint64_t)(lv.n_inner - t->inner_cap) * (t->NS * 8 + 4);
     if (t->p.mem_budget && t->device_bytes() + extra > t->p.mem_budget)
       return set_err(HM_EBUDGET, "inner levels need %llu more bytes than the budget allows",
                      (unsigned long long)extra);
@@ -280,39 +343,17 @@ int rebuild_inner(hm_tree* t, hipStream_t s) {
     if (t->d_prefix) HM_TRY(hipFree(t->d_prefix));
     t->d_inner = nullptr;
     t->d_prefix = nullptr;
-    const uint32_t cap = n_inner + n_inner / 8 + 8;
+    const uint32_t cap = lv.n_inner + lv.n_inner / 8 + 8;
     HM_TRY(hipMalloc(&t->d_inner, (size_t)cap * t->NS * 8));
     HM_TRY(hipMalloc(&t->d_prefix, (size_t)cap * 4));
     t->inner_cap = cap;
   }
-  // temporaries: two min-key arrays and the child counts
-  uint64_t *minA = nullptr, *minB = nullptr;
-  uint32_t* nchild = nullptr;
-  void* scan_tmp = nullptr;
-  HM_TRY(hipMalloc(&minA, std::max<uint64_t>(t->n_leaves, 1) * 8));
-  HM_TRY(hipMalloc(&minB, std::max<uint64_t>((t->n_leaves + t->F - 1) / t->F, 1) * 8));
-  HM_TRY(hipMalloc(&nchild, (size_t)n_inner * 4));
-  HM_TRY(hipMalloc(&scan_tmp, scan_tmp_bytes(n_inner)));
-  HM_TRY(leaf_min_keys(t->d_leaves, t->n_leaves, minA, s));
-  uint64_t m_child = t->n_leaves;
-  uint64_t *cmin = minA, *nmin = minB;
-  for (int l = h - 1; l >= 0; --l) {
-    const uint64_t m = ls[l + 1] - ls[l];
-    HM_TRY(build_level(cmin, m_child, t->F, t->d_inner + (size_t)ls[l] * t->NS, nmin, nchild + ls[l], m, s));
-    std::swap(cmin, nmin);
-    m_child = m;
-  }
-  HM_TRY(exclusive_scan_u32(nchild, t->d_prefix, n_inner, 1u, scan_tmp, s));
+  int rc = ensure_rl(t, inner_scratch_bytes(t->n_leaves, t->F, lv.n_inner), s);
+  if (rc) return rc;
+  rc = build_inner_into(t, t->d_leaves, t->n_leaves, t->d_inner, t->d_prefix, lv, 0, s);
+  if (rc) return rc;
   HM_TRY(hipStreamSynchronize(s));
-  HM_TRY(hipFree(minA));
-  HM_TRY(hipFree(minB));
-  HM_TRY(hipFree(nchild));
-  HM_TRY(hipFree(scan_tmp));
-  t->height = h;
-  t->n_inner = n_inner;
-  for (int i = 0; i <= MAX_LEVELS; ++i) t->level_start[i] = ls[i];
-  t->lds_levels = choose_lds_levels(t, t->p.lds_levels);
-  t->layout_version++;  // captured pipelines hold the old pointers and level table
+  set_levels(t, lv);
   return HM_OK;
 }
 
@@ -330,6 +371,10 @@ void free_tree(hm_tree* t) {
   if (t->pend_v) (void)hipFree(t->pend_v);
   if (t->upd_buf) (void)hipFree(t->upd_buf);
   if (t->upd_host) (void)hipHostFree(t->upd_host);
+  if (t->sb_leaves) (void)hipFree(t->sb_leaves);
+  if (t->sb_inner) (void)hipFree(t->sb_inner);
+  if (t->sb_prefix) (void)hipFree(t->sb_prefix);
+  if (t->rl_buf) (void)hipFree(t->rl_buf);
   if (t->publish_ev) (void)hipEventDestroy(t->publish_ev);
   if (t->own) (void)hipStreamDestroy(t->own);
   delete t;
@@ -362,7 +407,7 @@ int hm_params_init(hm_params* p) {
   p->lds_levels = -1;
   p->group_size = 0;
   p->psa_bits = 16;
-  p->sort_threshold = 1ull << 16;
+  p->sort_threshold = 1ull << 20;  // below this the sort passes cost more launches than they save
   p->mem_budget = 0;
   p->flags = 0;
   return HM_OK;
@@ -716,18 +761,19 @@ static int ensure_upd(hm_tree* t, size_t n, hipStream_t s) {
   return HM_OK;
 }
 
-int hm_update(hm_tree* t, const uint64_t* keys, const uint64_t* vals, uint64_t n, void* stream) {
-  if (!t) return set_err(HM_EINVAL, "tree is NULL");
+}  // extern "C"
+
+namespace hm {
+// Stage an upsert batch; the caller holds t->pub_mu.
+int stage_updates(hm_tree* t, const uint64_t* keys, const uint64_t* vals, uint64_t n, hipStream_t s) {
   if (n == 0) return HM_OK;
   if (!keys || !vals) return set_err(HM_EINVAL, "NULL buffer");
   if (n > kMaxBatch) return set_err(HM_EINVAL, "update batch too large");
-  hipStream_t s = (hipStream_t)stream;
   const bool dk = is_device_ptr(keys), dv = is_device_ptr(vals);
   if (!dk) {  // host keys: check for the reserved key before anything touches the device
     for (uint64_t i = 0; i < n; ++i)
       if (keys[i] == HM_KEY_MAX) return set_err(HM_EINVAL, "key 0xFFFFFFFFFFFFFFFF is reserved for padding");
   }
-  std::lock_guard<std::mutex> g(t->mu);
   if (t->pend_n + n > kMaxBatch) return set_err(HM_EINVAL, "too many staged updates; publish first");
   if (t->pend_n + n > t->pend_cap) {
     size_t cap = std::max<size_t>(t->pend_n + n, 2 * t->pend_cap);
@@ -763,6 +809,15 @@ int hm_update(hm_tree* t, const uint64_t* keys, const uint64_t* vals, uint64_t n
   t->pend_n += n;
   return HM_OK;
 }
+}  // namespace hm
+
+extern "C" {
+
+int hm_update(hm_tree* t, const uint64_t* keys, const uint64_t* vals, uint64_t n, void* stream) {
+  if (!t) return set_err(HM_EINVAL, "tree is NULL");
+  std::lock_guard<std::mutex> g(t->pub_mu);
+  return stage_updates(t, keys, vals, n, (hipStream_t)stream);
+}
 
 // Make s wait for every search / range issued so far (before leaves change in place).
 static int wait_for_readers(hm_tree* t, hipStream_t s) {
@@ -771,13 +826,23 @@ static int wait_for_readers(hm_tree* t, hipStream_t s) {
   return HM_OK;
 }
 
-// Apply the staged batch on s.  Small batches (the serving case) cost a handful of launches and one
-// host round trip for the plan; only a batch that overflows some leaf re-lays out the leaves.
-static int apply_pending(hm_tree* t, hipStream_t s, int* rebuilt) {
+}  // extern "C"
+
+namespace hm {
+
+// Prepare the staged batch on s without changing what searches read: sort and de-duplicate it,
+// locate its keys and plan the epoch (one host round trip).  If some leaf overflows, build the
+// whole new layout (leaves with the batch merged, inner levels, prefix array) into the standby
+// buffers.  Searches on other streams keep running on the active layout meanwhile.  The caller
+// holds t->pub_mu until publish_commit.
+int publish_prepare(hm_tree* t, hipStream_t s, PublishPlan* plan) {
+  *plan = PublishPlan{};
   const size_t n = t->pend_n;
+  plan->n = n;
   if (!n) return HM_OK;
   int rc = ensure_upd(t, n, s);
   if (rc) return rc;
+  if (t->publish_valid) HM_TRY(hipStreamWaitEvent(s, t->publish_ev, 0));  // the previous epoch's writes
   const UpdBufs b = carve_upd(t->upd_buf, n);
   uint32_t* dm = b.summary + 2;
   HM_TRY(hipMemsetAsync(b.summary, 0, 16, s));
@@ -799,115 +864,176 @@ static int apply_pending(hm_tree* t, hipStream_t s, int* rebuilt) {
   HM_TRY(plan_runs(t->d_leaves, b.leaf, b.found, n, dm, b.summary, s));
   HM_TRY(hipMemcpyAsync(t->upd_host, b.summary, 16, hipMemcpyDeviceToHost, s));
   HM_TRY(hipStreamSynchronize(s));
-  const bool overflow = t->upd_host[0] & 1u;
-  const uint64_t n_new = t->upd_host[1];
-  const size_t m = t->upd_host[2];
-  rc = wait_for_readers(t, s);  // every write below waits for the searches issued before
-  if (rc) return rc;
-  if (!overflow) {
-    HM_TRY(apply_runs(t->d_leaves, b.leaf, b.slot, b.found, b.uk, b.uv, n, dm, s));
-    t->n_keys += n_new;
-    if (rebuilt) *rebuilt = 0;
-    return HM_OK;
-  }
+  plan->n_new = t->upd_host[1];
+  plan->m = t->upd_host[2];
+  plan->relayout = (t->upd_host[0] & 1u) != 0;
+  if (!plan->relayout) return HM_OK;
 
-  // overflow: re-lay out all leaves, then rebuild the inner levels and prefix array
-  std::vector<void*> tmps;
-  auto cleanup = [&]() {
-    (void)hipStreamSynchronize(s);
-    for (void* p : tmps) (void)hipFree(p);
-  };
-  auto alloc = [&](size_t bytes) -> void* {
-    void* p = nullptr;
-    if (hipMalloc(&p, std::max<size_t>(bytes, 256)) != hipSuccess) return nullptr;
-    tmps.push_back(p);
-    return p;
-  };
-#define UPD_TRY(expr)                                                                                            \
-  do {                                                                                                           \
-    hipError_t e_ = (expr);                                                                                      \
-    if (e_ != hipSuccess) {                                                                                      \
-      cleanup();                                                                                                 \
-      return set_err(e_ == hipErrorOutOfMemory ? HM_ENOMEM : HM_EDEVICE, "%s failed: %s", #expr,                  \
-                     hipGetErrorString(e_));                                                                     \
-    }                                                                                                            \
-  } while (0)
-#define UPD_ALLOC(var, type, bytes)                                  \
-  type var = (type)alloc(bytes);                                     \
-  if (!var) {                                                        \
-    cleanup();                                                       \
-    return set_err(HM_ENOMEM, "update scratch allocation failed");   \
-  }
-  UPD_ALLOC(ins, uint32_t*, t->n_leaves * 4);
-  UPD_ALLOC(istart, uint32_t*, t->n_leaves * 4);
-  UPD_TRY(hipMemsetAsync(ins, 0, t->n_leaves * 4, s));
-  UPD_TRY(hipMemsetAsync(istart, 0xFF, t->n_leaves * 4, s));
-  UPD_TRY(count_inserts(b.leaf, b.found, m, ins, s));
-  UPD_TRY(insert_starts(b.leaf, b.found, m, istart, s));
-  UPD_ALLOC(cnt, uint32_t*, t->n_leaves * 4);
-  UPD_ALLOC(nstart, uint32_t*, (t->n_leaves + 1) * 4);
-  UPD_ALLOC(ntmp, void*, scan_tmp_bytes(t->n_leaves + 1));
-  UPD_TRY(new_leaf_counts(t->d_leaves, ins, t->n_leaves, t->fill, cnt, s));
-  UPD_TRY(exclusive_scan_u32(cnt, nstart, t->n_leaves, 0u, ntmp, s));
-  uint32_t ls = 0, lc = 0;
-  UPD_TRY(hipMemcpyAsync(&ls, nstart + t->n_leaves - 1, 4, hipMemcpyDeviceToHost, s));
-  UPD_TRY(hipMemcpyAsync(&lc, cnt + t->n_leaves - 1, 4, hipMemcpyDeviceToHost, s));
-  UPD_TRY(hipStreamSynchronize(s));
-  const uint64_t new_leaves = (uint64_t)ls + lc;
-  if (new_leaves >= (1ull << 32)) {
-    cleanup();
-    return set_err(HM_EINVAL, "leaf count would exceed 2^32");
-  }
-  uint64_t new_cap = new_leaves + new_leaves / 16 + 64;
-  {
-    uint32_t lsx[MAX_LEVELS + 1], ni = 0;
-    if (plan_levels(new_leaves, t->F, lsx, &ni) < 0) {
-      cleanup();
-      return set_err(HM_EINVAL, "tree would exceed %d levels", MAX_LEVELS);
-    }
-    const uint64_t inner_more = ni > t->inner_cap ? (uint64_t)(ni + ni / 8 + 8 - t->inner_cap) * (t->NS * 8 + 4) : 0;
-    // old and new leaves coexist while relaying out
-    if (t->p.mem_budget && t->device_bytes() + new_cap * sizeof(Leaf) + inner_more > t->p.mem_budget) {
-      cleanup();
-      return set_err(HM_EBUDGET, "re-layout to %llu leaves exceeds the device-memory budget",
-                     (unsigned long long)new_leaves);
-    }
-  }
-  Leaf* nl = nullptr;
-  UPD_TRY(hipMalloc(&nl, new_cap * sizeof(Leaf)));
-  UPD_TRY(apply_value_updates(t->d_leaves, b.leaf, b.slot, b.found, b.uv, m, s));
-  UPD_TRY(relayout_leaves(t->d_leaves, t->n_leaves, ins, istart, b.uk, b.uv, b.found, nstart, t->fill, nl, s));
-  UPD_TRY(hipStreamSynchronize(s));
-  UPD_TRY(hipFree(t->d_leaves));
-  t->d_leaves = nl;
-  t->n_leaves = new_leaves;
-  t->leaf_cap = new_cap;
-  t->n_keys += n_new;
-  cleanup();
-  rc = rebuild_inner(t, s);
+  // ---- re-layout into the standby buffers ----
+  ProfScope prof("relayout", s);
+  const uint64_t nl = t->n_leaves;
+  const size_t a4 = align256((nl + 1) * 4);
+  rc = ensure_rl(t, 4 * a4 + align256(scan_tmp_bytes(nl + 1)), s);
   if (rc) return rc;
-  t->relayouts++;
-  if (rebuilt) *rebuilt = 1;
+  char* cur = (char*)t->rl_buf;
+  uint32_t* ins = (uint32_t*)cur;
+  uint32_t* first = (uint32_t*)(cur + a4);
+  uint32_t* cnt = (uint32_t*)(cur + 2 * a4);
+  uint32_t* nstart = (uint32_t*)(cur + 3 * a4);
+  void* ntmp = cur + 4 * a4;
+  HM_TRY(hipMemsetAsync(ins, 0, nl * 4, s));
+  HM_TRY(hipMemsetAsync(first, 0xFF, nl * 4, s));
+  HM_TRY(count_inserts(b.leaf, b.found, plan->m, ins, s));
+  HM_TRY(leaf_first_keys(b.leaf, plan->m, first, s));
+  HM_TRY(new_leaf_counts(t->d_leaves, ins, nl, t->fill, cnt, s));
+  HM_TRY(exclusive_scan_u32(cnt, nstart, nl, 0u, ntmp, s));
+  HM_TRY(hipMemcpyAsync(t->upd_host + 4, nstart + nl - 1, 4, hipMemcpyDeviceToHost, s));
+  HM_TRY(hipMemcpyAsync(t->upd_host + 5, cnt + nl - 1, 4, hipMemcpyDeviceToHost, s));
+  HM_TRY(hipStreamSynchronize(s));
+  const uint64_t new_leaves = (uint64_t)t->upd_host[4] + t->upd_host[5];
+  if (new_leaves >= (1ull << 32)) return set_err(HM_EINVAL, "leaf count would exceed 2^32");
+  Levels lv;
+  lv.height = plan_levels(new_leaves, t->F, lv.level_start, &lv.n_inner);
+  if (lv.height < 0) return set_err(HM_EINVAL, "tree would exceed %d levels", MAX_LEVELS);
+  // standby capacity (grow-only, with slack so the next re-layouts fit)
+  const uint64_t leaf_cap = t->sb_leaf_cap >= new_leaves ? t->sb_leaf_cap : new_leaves + new_leaves / 16 + 64;
+  const uint32_t inner_cap = t->sb_inner_cap >= lv.n_inner ? t->sb_inner_cap : lv.n_inner + lv.n_inner / 8 + 8;
+  const uint64_t grow = (leaf_cap - t->sb_leaf_cap) * sizeof(Leaf) + (uint64_t)(inner_cap - t->sb_inner_cap) * (t->NS * 8 + 4);
+  const size_t ib_off = align256(4 * a4 + align256(scan_tmp_bytes(nl + 1)));
+  const size_t ib_need = ib_off + inner_scratch_bytes(new_leaves, t->F, lv.n_inner);
+  if (t->p.mem_budget &&
+      t->device_bytes() + grow + (ib_need > t->rl_bytes ? ib_need - t->rl_bytes : 0) > t->p.mem_budget)
+    return set_err(HM_EBUDGET, "re-layout to %llu leaves exceeds the device-memory budget",
+                   (unsigned long long)new_leaves);
+  if (ib_need > t->rl_bytes) {  // keep the counts: grow into a new buffer and copy them over
+    void* nb = nullptr;
+    HM_TRY(hipMalloc(&nb, ib_need));
+    HM_TRY(hipMemcpyAsync(nb, t->rl_buf, ib_off, hipMemcpyDeviceToDevice, s));
+    HM_TRY(hipStreamSynchronize(s));
+    HM_TRY(hipFree(t->rl_buf));
+    t->rl_buf = nb;
+    t->rl_bytes = ib_need;
+    cur = (char*)t->rl_buf;
+    ins = (uint32_t*)cur;
+    first = (uint32_t*)(cur + a4);
+    nstart = (uint32_t*)(cur + 3 * a4);
+  }
+  if (leaf_cap != t->sb_leaf_cap) {
+    HM_TRY(hipStreamSynchronize(s));
+    if (t->sb_leaves) HM_TRY(hipFree(t->sb_leaves));
+    t->sb_leaves = nullptr;
+    t->sb_leaf_cap = 0;
+    HM_TRY(hipMalloc(&t->sb_leaves, leaf_cap * sizeof(Leaf)));
+    t->sb_leaf_cap = leaf_cap;
+  }
+  if (inner_cap != t->sb_inner_cap) {
+    HM_TRY(hipStreamSynchronize(s));
+    if (t->sb_inner) HM_TRY(hipFree(t->sb_inner));
+    if (t->sb_prefix) HM_TRY(hipFree(t->sb_prefix));
+    t->sb_inner = nullptr;
+    t->sb_prefix = nullptr;
+    t->sb_inner_cap = 0;
+    HM_TRY(hipMalloc(&t->sb_inner, (size_t)inner_cap * t->NS * 8));
+    HM_TRY(hipMalloc(&t->sb_prefix, (size_t)inner_cap * 4));
+    t->sb_inner_cap = inner_cap;
+  }
+  // the standby arrays were the active ones before the last swap: searches issued before it may
+  // still read them
+  {
+    std::lock_guard<std::mutex> g(t->mu);  // the workspace map changes under mu
+    rc = wait_for_readers(t, s);
+  }
+  if (rc) return rc;
+  HM_TRY(relayout_leaves(t->d_leaves, nl, ins, first, b.leaf, b.slot, b.uk, b.uv, b.found, plan->m, nstart, t->fill,
+                         t->sb_leaves, s));
+  rc = build_inner_into(t, t->sb_leaves, new_leaves, t->sb_inner, t->sb_prefix, lv, ib_off, s);
+  if (rc) return rc;
+  HM_TRY(hipStreamSynchronize(s));
+  plan->new_leaves = new_leaves;
+  plan->lv = lv;
   return HM_OK;
-#undef UPD_TRY
-#undef UPD_ALLOC
 }
 
-int hm_epoch_publish(hm_tree* t, void* stream, int* out_rebuilt) {
-  if (!t) return set_err(HM_EINVAL, "tree is NULL");
-  if (out_rebuilt) *out_rebuilt = 0;
-  hipStream_t s = (hipStream_t)stream;
+// Allocate the standby layout and re-layout scratch ahead of time (a serving loop does this at
+// start, so the first overflow does not pay for multi-GB allocations).  Best effort under the
+// memory budget: returns HM_EBUDGET without allocating if it would not fit.
+int reserve_relayout(hm_tree* t, hipStream_t s) {
+  std::lock_guard<std::mutex> pub(t->pub_mu);
+  const uint64_t nl = t->n_leaves;
+  const uint64_t leaf_cap = nl + nl / 16 + 64;
+  uint32_t ls[MAX_LEVELS + 1], ni = 0;
+  if (plan_levels(leaf_cap, t->F, ls, &ni) < 0) return set_err(HM_EINVAL, "too many levels");
+  const uint32_t inner_cap = ni + ni / 8 + 8;
+  const size_t a4 = align256((nl + 1) * 4);
+  const size_t rl = align256(4 * a4 + align256(scan_tmp_bytes(nl + 1))) + inner_scratch_bytes(leaf_cap, t->F, ni);
+  uint64_t grow = 0;
+  if (t->sb_leaf_cap < leaf_cap) grow += (leaf_cap - t->sb_leaf_cap) * sizeof(Leaf);
+  if (t->sb_inner_cap < inner_cap) grow += (uint64_t)(inner_cap - t->sb_inner_cap) * (t->NS * 8 + 4);
+  if (t->rl_bytes < rl) grow += rl - t->rl_bytes;
+  if (t->p.mem_budget && t->device_bytes() + grow > t->p.mem_budget)
+    return set_err(HM_EBUDGET, "no room for a standby layout under the memory budget");
+  HM_TRY(hipStreamSynchronize(s));
+  if (t->sb_leaf_cap < leaf_cap) {
+    if (t->sb_leaves) HM_TRY(hipFree(t->sb_leaves));
+    t->sb_leaves = nullptr;
+    t->sb_leaf_cap = 0;
+    HM_TRY(hipMalloc(&t->sb_leaves, leaf_cap * sizeof(Leaf)));
+    t->sb_leaf_cap = leaf_cap;
+  }
+  if (t->sb_inner_cap < inner_cap) {
+    if (t->sb_inner) HM_TRY(hipFree(t->sb_inner));
+    if (t->sb_prefix) HM_TRY(hipFree(t->sb_prefix));
+    t->sb_inner = nullptr;
+    t->sb_prefix = nullptr;
+    t->sb_inner_cap = 0;
+    HM_TRY(hipMalloc(&t->sb_inner, (size_t)inner_cap * t->NS * 8));
+    HM_TRY(hipMalloc(&t->sb_prefix, (size_t)inner_cap * 4));
+    t->sb_inner_cap = inner_cap;
+  }
+  return ensure_rl(t, rl, s);
+}
+
+// Make the prepared epoch visible: merge in place (after every earlier search) or swap in the
+// standby layout; later searches wait for it on the device.  The caller holds t->pub_mu.
+int publish_commit(hm_tree* t, hipStream_t s, const PublishPlan& plan, int* rebuilt) {
   std::lock_guard<std::mutex> g(t->mu);
-  // the previous publish's leaf writes come first; searches still running only read the tree, so
-  // the sort / locate / plan steps run beside them and only the writes wait (inside apply_pending)
-  if (t->publish_valid) HM_TRY(hipStreamWaitEvent(s, t->publish_ev, 0));
-  int rc = apply_pending(t, s, out_rebuilt);
-  if (rc) return rc;
+  if (rebuilt) *rebuilt = plan.relayout ? 1 : 0;
+  if (plan.n && !plan.relayout) {
+    const UpdBufs b = carve_upd(t->upd_buf, plan.n);
+    int rc = wait_for_readers(t, s);  // leaves change in place
+    if (rc) return rc;
+    HM_TRY(apply_runs(t->d_leaves, b.leaf, b.slot, b.found, b.uk, b.uv, plan.n, b.summary + 2, s));
+  } else if (plan.relayout) {
+    std::swap(t->d_leaves, t->sb_leaves);
+    std::swap(t->leaf_cap, t->sb_leaf_cap);
+    std::swap(t->d_inner, t->sb_inner);
+    std::swap(t->d_prefix, t->sb_prefix);
+    std::swap(t->inner_cap, t->sb_inner_cap);
+    t->n_leaves = plan.new_leaves;
+    set_levels(t, plan.lv);
+    t->relayouts++;
+  }
+  t->n_keys += plan.n_new;
   t->pend_n = 0;
   HM_TRY(hipEventRecord(t->publish_ev, s));
   t->publish_valid = true;
   t->epoch++;
   return HM_OK;
+}
+
+}  // namespace hm
+
+extern "C" {
+
+int hm_epoch_publish(hm_tree* t, void* stream, int* out_rebuilt) {
+  if (!t) return set_err(HM_EINVAL, "tree is NULL");
+  if (out_rebuilt) *out_rebuilt = 0;
+  hipStream_t s = (hipStream_t)stream;
+  std::lock_guard<std::mutex> g(t->pub_mu);
+  PublishPlan plan;
+  int rc = publish_prepare(t, s, &plan);
+  if (rc) return rc;
+  return publish_commit(t, s, plan, out_rebuilt);
 }
 
 // ---- binary snapshot ------------------------------------------------------------------------

@@ -1,5 +1,5 @@
 // Batched pipeline: many uneven query batches packed into one buffer, served by one hipGraph
-// replay of sort -> persistent search -> un-permute with the query count read from device memory.
+// replay of sort -> persistent search -> un-sort with the query count read from device memory.
 //
 // The reference drives fixed-size buckets from a host thread over two streams, one
 // H2D -> SortPairs -> kernel -> D2H sequence per bucket (H/regular_bpt/prefix-sort.cu:261-309).

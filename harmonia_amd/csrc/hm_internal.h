@@ -141,13 +141,9 @@ hipError_t launch_range(const TreeView& tv, const uint64_t* start, const uint64_
 hipError_t mark_last_of_runs(const uint64_t* sorted_keys, size_t n, uint32_t* keep, hipStream_t s);
 hipError_t compact_pairs(const uint64_t* k, const uint64_t* v, const uint32_t* keep, const uint32_t* keep_scan,
                          size_t n, uint64_t* ko, uint64_t* vo, hipStream_t s);
-hipError_t apply_value_updates(Leaf* leaves, const uint32_t* leaf, const uint32_t* slot, const uint8_t* found,
-                               const uint64_t* vals, size_t n, hipStream_t s);
 // Per-leaf insert counts for the not-found keys (keys sorted => leaf ids nondecreasing).
 hipError_t count_inserts(const uint32_t* leaf, const uint8_t* found, size_t n, uint32_t* ins_per_leaf,
                          hipStream_t s);
-// ins_start[leaf] = first index (into the sorted batch) of the leaf's inserts (valid where count > 0).
-hipError_t insert_starts(const uint32_t* leaf, const uint8_t* found, size_t n, uint32_t* ins_start, hipStream_t s);
 // Small batches (n <= kSmallUpdate): one block sorts the staged (key, value) pairs by key in LDS,
 // keeps the last write of each key and writes the m unique pairs to (uk, uv), m to *dm.
 constexpr uint32_t kSmallUpdate = 4096;
@@ -165,9 +161,13 @@ hipError_t apply_runs(Leaf* leaves, const uint32_t* leaf, const uint32_t* slot, 
 // Out of place: new leaf count per old leaf (1 if it fits, else ceil(total / fill)).
 hipError_t new_leaf_counts(const Leaf* leaves, const uint32_t* ins_per_leaf, size_t n_leaves, int fill,
                            uint32_t* cnt, hipStream_t s);
-hipError_t relayout_leaves(const Leaf* old_leaves, size_t n_old, const uint32_t* ins_per_leaf,
-                           const uint32_t* ins_start, const uint64_t* bkeys, const uint64_t* bvals,
-                           const uint8_t* found, const uint32_t* new_start, int fill, Leaf* new_leaves,
+// first[leaf[j]] = j for the first batch index j of each leaf's run (first pre-filled with ~0).
+hipError_t leaf_first_keys(const uint32_t* leaf, size_t n, uint32_t* first, hipStream_t s);
+// Copy every old leaf into the new layout at new_start[i]: batch values of existing keys applied,
+// new keys merged, overflowing leaves split evenly (old leaves are only read).
+hipError_t relayout_leaves(const Leaf* old_leaves, size_t n_old, const uint32_t* ins_per_leaf, const uint32_t* first,
+                           const uint32_t* bleaf, const uint32_t* bslot, const uint64_t* bkeys, const uint64_t* bvals,
+                           const uint8_t* found, size_t m, const uint32_t* new_start, int fill, Leaf* new_leaves,
                            hipStream_t s);
 
 }  // namespace hm

@@ -10,7 +10,7 @@
 // number of ascending keys with matching values.  Prints one JSON object per run.
 //
 // Usage: hm_serve_bench --keys-log2 27 --requests 20000 --rate 5000 [--plain] [--closed]
-//                       [--inflight 4] [--budget-mb 4096] [--seed 1] [--leaf-fill 6]
+//                       [--inflight 4] [--budget-mb 4096] [--seed 1] [--leaf-fill 5]
 //   --plain   one request per launch, one worker (the baseline path)
 //   --closed  submit back to back (throughput probe) instead of Poisson arrivals
 #include <algorithm>
@@ -111,7 +111,9 @@ extern "C" int hm_serve_bench_main(int argc, char** argv, char* out_json, uint64
   }
   hm_params p;
   CHECK(hm_params_init(&p));
-  p.leaf_fill = (uint32_t)arg_d(argc, argv, "--leaf-fill", 6);  // slack for the insert stream
+  // 5 of 8 slots per leaf: slack for the insert stream, so few inserts overflow a leaf (a re-layout of
+  // the 2^27-key tree takes ~4 ms on the update path; reads keep going meanwhile)
+  p.leaf_fill = (uint32_t)arg_d(argc, argv, "--leaf-fill", 5);
   hm_tree* t = nullptr;
   auto tb = std::chrono::steady_clock::now();
   CHECK(hm_tree_build(keys.data(), vals.data(), n, &p, &t));
@@ -164,7 +166,7 @@ extern "C" int hm_serve_bench_main(int argc, char** argv, char* out_json, uint64
     }
   }
 
-  hm_server_params sp{budget, 1u << 22, plain ? 1u : inflight, plain ? 0u : 1u, 0};
+  hm_server_params sp{budget, 1u << 20, plain ? 1u : inflight, plain ? 0u : 1u, 0};
   hm_server* sv = nullptr;
   CHECK(hm_server_create(t, &sp, &sv));
   // warm-up outside the measurement

@@ -9,11 +9,12 @@
 // Updates: when the oldest pending request is an update, a worker takes every pending update and
 // applies them as one epoch (in submission order, so the last write of a key wins).  Reads and
 // updates that are pending at the same time are concurrent operations and may be ordered either
-// way; a request submitted after another one completed always sees its effect.  A read batch holds
-// the epoch lock shared until its device work is enqueued; the update worker takes it exclusively
-// (while holding the queue lock, so no new read batch can start) and releases it once the publish
-// is enqueued: hm_epoch_publish makes the device wait for earlier searches, and later searches wait
-// for the publish.  Every read therefore sees one whole epoch, reported with its result.
+// way; a request submitted after another one completed always sees its effect.  The update worker
+// stages the batch and prepares the epoch while read batches keep running on the current one (a
+// re-layout is built in standby buffers); only the commit -- an enqueue of the in-place merge after
+// every earlier search, or a pointer swap -- holds the epoch lock exclusively.  A read batch holds
+// that lock shared until its device work is enqueued, so every read sees one whole epoch, which is
+// reported with its result.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -34,6 +35,12 @@ using namespace hm;
 using Clock = std::chrono::steady_clock;
 
 namespace {
+
+#define HM_TRY_RC(expr)       \
+  do {                        \
+    const int rc_ = (expr);   \
+    if (rc_) return rc_;      \
+  } while (0)
 
 enum ReqType { REQ_LOOKUP = 0, REQ_RANGE = 1, REQ_UPDATE = 2 };
 
@@ -223,24 +230,44 @@ int run_reads(hm_server* sv, Worker& w, std::vector<ReqPtr>& lookups, std::vecto
   return HM_OK;
 }
 
-// Apply a coalesced update batch as one epoch.  Called with the epoch lock held exclusively; it is
-// released once the publish is enqueued.
-// A request that cannot be staged (e.g. it holds the reserved key) fails alone; status[i] is the
-// outcome of ups[i].
-void run_updates(hm_server* sv, Worker& w, std::vector<ReqPtr>& ups, std::unique_lock<std::shared_mutex>& ex,
-                 std::vector<int>& status, uint64_t* epoch) {
+// Apply a coalesced update batch as one epoch.  Staging and the prepare phase (sort, locate, plan,
+// and a full re-layout into the standby buffers when some leaf overflows) run while read batches
+// keep going on the current epoch; only the commit (an enqueue or a pointer swap) holds the epoch
+// lock exclusively.  A request that cannot be staged (e.g. it holds the reserved key) fails alone;
+// status[i] is the outcome of ups[i].
+void run_updates(hm_server* sv, Worker& w, std::vector<ReqPtr>& ups, std::vector<int>& status, uint64_t* epoch) {
   hm_tree* t = sv->t;
   status.assign(ups.size(), HM_OK);
-  for (size_t i = 0; i < ups.size(); ++i)
-    status[i] = hm_update(t, ups[i]->keys, ups[i]->vals, ups[i]->n, w.s);
-  int rebuilt = 0;
-  int rc = hm_epoch_publish(t, w.s, &rebuilt);
-  *epoch = tree_epoch(t);
-  ex.unlock();
+  std::lock_guard<std::mutex> pub(t->pub_mu);
+  for (size_t i = 0; i < ups.size(); ++i) status[i] = stage_updates(t, ups[i]->keys, ups[i]->vals, ups[i]->n, w.s);
+  PublishPlan plan;
+  int rc = publish_prepare(t, w.s, &plan);
+  if (!rc) {
+    // no new read batch may start while the commit waits for the ones still enqueueing
+    std::unique_lock<std::mutex> lk(sv->mu);
+    std::unique_lock<std::shared_mutex> ex(sv->epoch_mu);
+    lk.unlock();
+    int rebuilt = 0;
+    rc = publish_commit(t, w.s, plan, &rebuilt);
+    *epoch = tree_epoch(t);
+  }
   if (!rc && hipStreamSynchronize(w.s) != hipSuccess) rc = set_err(HM_EDEVICE, "update stream failed");
   if (rc)
     for (int& st : status)
       if (!st) st = rc;
+}
+
+int reserve_worker(hm_server* sv, Worker& w) {
+  const uint64_t b = sv->p.max_batch;
+  for (Buf* x : {&w.hq, &w.hv, &w.dq, &w.dv}) HM_TRY_RC(x->ensure(b * 8));
+  for (Buf* x : {&w.hf, &w.df}) HM_TRY_RC(x->ensure(b));
+  for (Buf* x : {&w.hl, &w.dl}) HM_TRY_RC(x->ensure(b * 12));
+  for (Buf* x : {&w.hc, &w.dc}) HM_TRY_RC(x->ensure(b * 4));
+  for (Buf* x : {&w.hk, &w.dk}) HM_TRY_RC(x->ensure(b * 16));
+  HM_TRY_RC(w.doff.ensure((b + 1) * 8));
+  // the tree's workspace for this stream: the largest search pipeline or range scan of a batch
+  const size_t ws = std::max(pipe_bytes(b, 32, HM_OUT_UNPERMUTE), align256(b * 8) + align256(scan_tmp_bytes(b + 1)));
+  return reserve_workspace(sv->t, w.s, ws);
 }
 
 // The first pending read, or nullptr.
@@ -256,7 +283,6 @@ void worker_loop(hm_server* sv, Worker* w) {
     std::vector<ReqPtr> lookups, ranges, ups;
     uint64_t bytes = 0;
     std::shared_lock<std::shared_mutex> epoch_lock(sv->epoch_mu, std::defer_lock);
-    std::unique_lock<std::shared_mutex> ex_lock(sv->epoch_mu, std::defer_lock);
     {
       std::unique_lock<std::mutex> lk(sv->mu);
       ++sv->idle;
@@ -275,7 +301,6 @@ void worker_loop(hm_server* sv, Worker* w) {
         sv->pending.pop_front();
         if (h->type == REQ_UPDATE) {
           ups.push_back(h);
-          ex_lock.lock();
         } else {
           (h->type == REQ_LOOKUP ? lookups : ranges).push_back(h);
           bytes = request_bytes(*h);
@@ -287,7 +312,6 @@ void worker_loop(hm_server* sv, Worker* w) {
         std::deque<ReqPtr> rest;
         for (auto& r : sv->pending) (r->type == REQ_UPDATE ? ups.push_back(r) : rest.push_back(r));
         sv->pending.swap(rest);
-        ex_lock.lock();  // waits only for read batches still enqueueing; none can start meanwhile
       } else {
         // every pending read that fits the batch and memory limits
         uint64_t nq = 0;
@@ -320,7 +344,7 @@ void worker_loop(hm_server* sv, Worker* w) {
     uint64_t epoch = 0;
     if (!ups.empty()) {
       std::vector<int> status;
-      run_updates(sv, *w, ups, ex_lock, status, &epoch);
+      run_updates(sv, *w, ups, status, &epoch);
       {
         std::lock_guard<std::mutex> g(sv->mu);
         ++sv->n_epochs;
@@ -374,7 +398,7 @@ int hm_server_create(hm_tree* t, const hm_server_params* p, hm_server** out) {
   sv->t = t;
   if (p) sv->p = *p;
   if (!sv->p.mem_budget) sv->p.mem_budget = 1ull << 30;
-  if (!sv->p.max_batch) sv->p.max_batch = 1u << 22;
+  if (!sv->p.max_batch) sv->p.max_batch = 1u << 20;
   if (!sv->p.max_inflight) sv->p.max_inflight = 4;
   if (sv->p.max_inflight > 16) sv->p.max_inflight = 16;
   for (uint32_t i = 0; i < sv->p.max_inflight; ++i) {
@@ -385,6 +409,17 @@ int hm_server_create(hm_tree* t, const hm_server_params* p, hm_server** out) {
       return set_err(HM_EDEVICE, "hipStreamCreate failed");
     }
     sv->workers.push_back(std::move(w));
+  }
+  // a re-layout while serving builds into standby buffers: allocate them now if the budget allows
+  (void)reserve_relayout(t, sv->workers[0]->s);
+  // size every worker's buffers for a full batch now: a grow while serving would allocate pinned
+  // memory (slow) and free device memory (synchronizes the device), stalling every request
+  for (auto& w : sv->workers) {
+    const int rc = reserve_worker(sv, *w);
+    if (rc) {
+      hm_server_destroy(sv);
+      return rc;
+    }
   }
   for (auto& w : sv->workers) w->th = std::thread(worker_loop, sv, w.get());
   *out = sv;

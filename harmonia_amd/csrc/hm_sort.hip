@@ -59,11 +59,13 @@ __global__ __launch_bounds__(kThreads) void radix_hist_kernel(const K* __restric
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   if (tile >= nb) return;  // block-uniform
   if (dn) n = *dn;  // device-side count (graph replays): tiles past n count nothing
-  __shared__ uint32_t hist[256];
-  hist[threadIdx.x] = 0;
+  // one histogram per wave (LDS atomics; lanes of a wave that hit one bin are serialized by the LDS,
+  // which is cheaper than ranking them when only counts are needed)
+  __shared__ uint32_t hist[kWaves][256];
+#pragma unroll
+  for (int ww = 0; ww < kWaves; ++ww) hist[ww][threadIdx.x] = 0;
   __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const uint64_t lt = (1ull << lane) - 1ull;
   const size_t base = (size_t)tile * kTile + (size_t)w * 64 * kItems;
   const size_t last = n ? n - 1 : 0;
   // issue every load of the tile before the first use, so the loads overlap
@@ -76,13 +78,13 @@ __global__ __launch_bounds__(kThreads) void radix_hist_kernel(const K* __restric
 #pragma unroll
   for (int r = 0; r < kItems; ++r) {
     const size_t i = base + (size_t)r * 64 + lane;
-    const bool valid = i < n;
-    const uint32_t d = valid ? (uint32_t)(kk[r] >> shift) & mask : 0u;
-    const uint64_t peers = match_digit(d, __ballot(valid));
-    if (valid && (peers & lt) == 0) atomicAdd(&hist[d], (uint32_t)__popcll(peers));
+    if (i < n) atomicAdd(&hist[w][(uint32_t)(kk[r] >> shift) & mask], 1u);
   }
   __syncthreads();
-  counts[(size_t)threadIdx.x * nb + tile] = hist[threadIdx.x];
+  uint32_t c = 0;
+#pragma unroll
+  for (int ww = 0; ww < kWaves; ++ww) c += hist[ww][threadIdx.x];
+  counts[(size_t)threadIdx.x * nb + tile] = c;
 }
 
 // Write (key, value) to (kout, vout); without HAS_VAL keys only.  With SAVE_LP, lp_out[i] = the

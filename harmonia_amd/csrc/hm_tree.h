@@ -57,7 +57,7 @@ struct hm_tree {
   int lds_levels = 0;
   uint64_t layout_version = 0;  // bumped whenever a device array of the layout is reallocated
   hm::SearchCfg cfg_direct{4, 4, 4, HM_OUT_UNPERMUTE};  // unsorted batches: memory-transaction bound
-  hm::SearchCfg cfg_psa{1, 2, 4, HM_OUT_UNPERMUTE};     // PSA-ordered batches: neighbours share nodes
+  hm::SearchCfg cfg_psa{2, 4, 8, HM_OUT_UNPERMUTE};     // PSA-ordered batches: neighbours share nodes
   std::mutex mu;
   std::unordered_map<hipStream_t, hm::Workspace> ws;
   hipStream_t own = nullptr;
@@ -71,6 +71,16 @@ struct hm_tree {
   void* upd_buf = nullptr;       // grow-only device scratch of hm_update / hm_epoch_publish
   size_t upd_bytes = 0;
   uint32_t* upd_host = nullptr;  // pinned: the publish plan summary read back by the host
+  // Standby layout: a re-layout is built here while searches keep reading the active arrays,
+  // then the two are swapped.  Kept (grow-only) for the next re-layout.
+  hm::Leaf* sb_leaves = nullptr;
+  uint64_t sb_leaf_cap = 0;
+  uint64_t* sb_inner = nullptr;
+  uint32_t* sb_prefix = nullptr;
+  uint32_t sb_inner_cap = 0;
+  void* rl_buf = nullptr;  // grow-only scratch of re-layouts and inner rebuilds
+  size_t rl_bytes = 0;
+  std::mutex pub_mu;  // serializes update staging and publishes; taken before mu
 
   hm::TreeView view() const {
     hm::TreeView tv{};
@@ -85,20 +95,46 @@ struct hm_tree {
     return tv;
   }
   uint64_t device_bytes() const {
-    return (uint64_t)inner_cap * NS * 8 + (uint64_t)inner_cap * 4 + leaf_cap * sizeof(hm::Leaf) + pend_cap * 16 +
-           upd_bytes;
+    return (uint64_t)(inner_cap + sb_inner_cap) * (NS * 8 + 4) + (leaf_cap + sb_leaf_cap) * sizeof(hm::Leaf) +
+           pend_cap * 16 + upd_bytes + rl_bytes;
   }
 };
 
 namespace hm {
+// Levels of a layout.
+struct Levels {
+  int height = 0;
+  uint32_t n_inner = 0;
+  uint32_t level_start[MAX_LEVELS + 1] = {0};
+};
+
+// One epoch being published: what publish_prepare found and built, for publish_commit.
+struct PublishPlan {
+  size_t n = 0;          // staged updates
+  size_t m = 0;          // unique keys among them
+  uint64_t n_new = 0;    // keys not in the tree before
+  bool relayout = false; // the standby layout holds the new epoch
+  uint64_t new_leaves = 0;
+  Levels lv;
+};
+// Staging and the two publish phases; the caller holds t->pub_mu across them.  Searches may run
+// during staging and publish_prepare; publish_commit is short (an enqueue or a pointer swap).
+int stage_updates(hm_tree* t, const uint64_t* keys, const uint64_t* vals, uint64_t n, hipStream_t s);
+int publish_prepare(hm_tree* t, hipStream_t s, PublishPlan* plan);
+int publish_commit(hm_tree* t, hipStream_t s, const PublishPlan& plan, int* rebuilt);
+// Allocate the standby layout and re-layout scratch now (best effort under the memory budget).
+int reserve_relayout(hm_tree* t, hipStream_t s);
+
 int ensure_ws(hm_tree* t, hipStream_t s, size_t bytes, Workspace** out);
+// Grow stream s's workspace to at least `bytes` now (takes t->mu).
+int reserve_workspace(hm_tree* t, hipStream_t s, size_t bytes);
 int choose_lds_levels(const hm_tree* t, int req);
 // Resolve the launch shape of one search call (tree defaults overridden by opts); psa_bits = 0
 // means no PSA (then out_mode = 0, results in input order).
 int resolve_search(const hm_tree* t, uint64_t n, const hm_search_opts* o, SearchLaunch* sl, int* psa_bits,
                    int* out_mode);
 
-// Device scratch of one sort -> search -> (un-permute) pipeline over up to n queries.
+// Device scratch of one sort -> search -> (un-sort) pipeline over up to n queries.
 struct PipeBufs {
   uint64_t *k0, *k1;  // PSA-sorted keys (ping-pong)
   uint32_t *p0, *p1;  // positions (ping-pong; scatter and PSA-order outputs)
@@ -109,7 +145,7 @@ struct PipeBufs {
 };
 size_t pipe_bytes(size_t n, int psa_bits, int out_mode);
 PipeBufs carve_pipe(char*& cur, size_t n, int psa_bits, int out_mode);
-// Enqueue sort -> search -> un-permute on s.  n sizes every launch; dn (device) may hold a smaller
+// Enqueue sort -> search -> un-sort on s.  n sizes every launch; dn (device) may hold a smaller
 // actual count, so the same enqueue can be captured once into a graph and replayed.  The PSA-order
 // mode copies positions to dpos with a device-to-device copy of n entries.
 int enqueue_pipeline(const TreeView& tv, const SearchLaunch& sl, int psa_bits, int out_mode, const uint64_t* dq,

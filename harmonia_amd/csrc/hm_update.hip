@@ -49,28 +49,11 @@ __global__ void compact_kernel(const uint64_t* __restrict__ k, const uint64_t* _
     }
 }
 
-__global__ void apply_values_kernel(Leaf* __restrict__ leaves, const uint32_t* __restrict__ leaf,
-                                    const uint32_t* __restrict__ slot, const uint8_t* __restrict__ found,
-                                    const uint64_t* __restrict__ vals, size_t n) {
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    if (found[i]) leaves[leaf[i]].val[slot[i]] = vals[i];
-}
-
 __global__ void count_inserts_kernel(const uint32_t* __restrict__ leaf, const uint8_t* __restrict__ found, size_t n,
                                      uint32_t* __restrict__ ins) {
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
     if (!found[i]) atomicAdd(&ins[leaf[i]], 1u);
-}
-
-// The batch is sorted by key, so leaf ids are nondecreasing: the first not-found index of a leaf is
-// the smallest i with !found[i] whose leaf has no earlier not-found key.
-__global__ void insert_starts_kernel(const uint32_t* __restrict__ leaf, const uint8_t* __restrict__ found, size_t n,
-                                     uint32_t* __restrict__ start) {
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    if (!found[i]) atomicMin(&start[leaf[i]], (uint32_t)i);
 }
 
 // Stream the merge of a leaf's keys with its inserts (the batch keys from `start` on that are not
@@ -108,14 +91,27 @@ __global__ void new_counts_kernel(const Leaf* __restrict__ leaves, const uint32_
   }
 }
 
+__global__ void leaf_first_kernel(const uint32_t* __restrict__ leaf, size_t n, uint32_t* __restrict__ first) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    if (i == 0 || leaf[i] != leaf[i - 1]) first[leaf[i]] = (uint32_t)i;  // run heads: one writer per leaf
+}
+
+// Old leaf i -> new leaves [new_start[i], new_start[i] + parts): existing keys of the batch get
+// their new value in the copy, new keys are merged in; the old leaves are only read.
 __global__ void relayout_kernel(const Leaf* __restrict__ old_leaves, size_t n_old, const uint32_t* __restrict__ ins,
-                                const uint32_t* __restrict__ start, const uint64_t* __restrict__ bk,
-                                const uint64_t* __restrict__ bv, const uint8_t* __restrict__ found,
+                                const uint32_t* __restrict__ first, const uint32_t* __restrict__ bleaf,
+                                const uint32_t* __restrict__ bslot, const uint64_t* __restrict__ bk,
+                                const uint64_t* __restrict__ bv, const uint8_t* __restrict__ found, size_t m,
                                 const uint32_t* __restrict__ new_start, int fill, Leaf* __restrict__ new_leaves) {
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_old; i += stride) {
     const uint32_t a = ins[i];
-    const Leaf l = old_leaves[i];
+    const uint32_t f0 = first[i];
+    Leaf l = old_leaves[i];
+    if (f0 != 0xFFFFFFFFu)
+      for (size_t j = f0; j < m && bleaf[j] == (uint32_t)i; ++j)
+        if (found[j]) l.val[bslot[j]] = bv[j];
     Leaf* dst = new_leaves + new_start[i];
     if (!a) {
       *dst = l;
@@ -132,7 +128,7 @@ __global__ void relayout_kernel(const Leaf* __restrict__ old_leaves, size_t n_ol
         dst[p].val[s] = 0ull;
       }
     int part = 0, lo = 0, hi = c / parts;
-    merge_leaf(l, a, start[i], bk, bv, found, [&](int idx, uint64_t k, uint64_t v) {
+    merge_leaf(l, a, f0, bk, bv, found, [&](int idx, uint64_t k, uint64_t v) {
       while (idx >= hi && part + 1 < parts) {
         ++part;
         lo = hi;
@@ -323,26 +319,11 @@ hipError_t compact_pairs(const uint64_t* k, const uint64_t* v, const uint32_t* k
   return hipGetLastError();
 }
 
-hipError_t apply_value_updates(Leaf* leaves, const uint32_t* leaf, const uint32_t* slot, const uint8_t* found,
-                               const uint64_t* vals, size_t n, hipStream_t s) {
-  if (!n) return hipSuccess;
-  ProfScope prof("update", s);
-  hipLaunchKernelGGL(apply_values_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, leaves, leaf, slot, found, vals, n);
-  return hipGetLastError();
-}
-
 hipError_t count_inserts(const uint32_t* leaf, const uint8_t* found, size_t n, uint32_t* ins_per_leaf,
                          hipStream_t s) {
   if (!n) return hipSuccess;
   ProfScope prof("update", s);
   hipLaunchKernelGGL(count_inserts_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, leaf, found, n, ins_per_leaf);
-  return hipGetLastError();
-}
-
-hipError_t insert_starts(const uint32_t* leaf, const uint8_t* found, size_t n, uint32_t* ins_start, hipStream_t s) {
-  if (!n) return hipSuccess;
-  ProfScope prof("update", s);
-  hipLaunchKernelGGL(insert_starts_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, leaf, found, n, ins_start);
   return hipGetLastError();
 }
 
@@ -354,13 +335,20 @@ hipError_t new_leaf_counts(const Leaf* leaves, const uint32_t* ins_per_leaf, siz
   return hipGetLastError();
 }
 
-hipError_t relayout_leaves(const Leaf* old_leaves, size_t n_old, const uint32_t* ins_per_leaf,
-                           const uint32_t* ins_start, const uint64_t* bkeys, const uint64_t* bvals,
-                           const uint8_t* found, const uint32_t* new_start, int fill, Leaf* new_leaves,
+hipError_t leaf_first_keys(const uint32_t* leaf, size_t n, uint32_t* first, hipStream_t s) {
+  if (!n) return hipSuccess;
+  ProfScope prof("update", s);
+  hipLaunchKernelGGL(leaf_first_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, leaf, n, first);
+  return hipGetLastError();
+}
+
+hipError_t relayout_leaves(const Leaf* old_leaves, size_t n_old, const uint32_t* ins_per_leaf, const uint32_t* first,
+                           const uint32_t* bleaf, const uint32_t* bslot, const uint64_t* bkeys, const uint64_t* bvals,
+                           const uint8_t* found, size_t m, const uint32_t* new_start, int fill, Leaf* new_leaves,
                            hipStream_t s) {
   ProfScope prof("update", s);
   hipLaunchKernelGGL(relayout_kernel, dim3(grid_for(n_old, 256)), dim3(256), 0, s, old_leaves, n_old, ins_per_leaf,
-                     ins_start, bkeys, bvals, found, new_start, fill, new_leaves);
+                     first, bleaf, bslot, bkeys, bvals, found, m, new_start, fill, new_leaves);
   return hipGetLastError();
 }
 

@@ -57,7 +57,7 @@ class BPlusTree:
     """A device-resident Harmonia B+tree (fanout F in {32, 64, 128}, 8-key leaves)."""
 
     def __init__(self, fanout=64, leaf_fill=8, lds_levels=-1, group_size=0, psa_bits=16,
-                 sort_threshold=1 << 16, mem_budget=0):
+                 sort_threshold=None, mem_budget=0):
         self._lib = _lib.load()
         self.params = HmParams()
         check(self._lib.hm_params_init(ctypes.byref(self.params)))
@@ -66,7 +66,8 @@ class BPlusTree:
         self.params.lds_levels = lds_levels
         self.params.group_size = group_size
         self.params.psa_bits = psa_bits
-        self.params.sort_threshold = sort_threshold
+        if sort_threshold is not None:  # else the library default
+            self.params.sort_threshold = sort_threshold
         self.params.mem_budget = mem_budget
         self._h = None
         self._pending_keys = []
@@ -143,7 +144,7 @@ class BPlusTree:
                      out_pos=None):
         """Batched search; returns (values, found) in input order (or PSA order with out_pos).
 
-        output_mode: 0 tree default, 1 input order via radix un-permute, 2 input order via direct
+        output_mode: 0 tree default, 1 input order via the replayed sort passes, 2 input order via direct
         scatter, 3 (or sorted_output=True) PSA order with the permutation in out_pos."""
         if sorted_output:
             output_mode = _lib.HM_OUT_PSA_ORDER
@@ -305,7 +306,7 @@ class Batcher:
     """Many uneven query batches served by one hipGraph replay (hm_batcher_*).
 
     add() packs a batch into the batcher's device buffer and returns its offset; flush() launches
-    PSA sort -> one persistent search launch -> un-permute over everything added; copy_out() reads
+    PSA sort -> one persistent search launch -> un-sort over everything added; copy_out() reads
     one batch's results (or results() gives the device pointers)."""
 
     def __init__(self, tree, capacity, stream=None, group_size=0, lds_levels=-1, psa_bits=-1, blocks_per_cu=0,

@@ -71,7 +71,7 @@ typedef struct hm_stats {
 
 /* hm_search_opts.output_mode (with PSA on; without PSA results are always in input order) */
 #define HM_OUT_DEFAULT 0 /* the tree's choice */
-#define HM_OUT_UNPERMUTE 1 /* input order: results radix-sorted back by position (coalesced writes) */
+#define HM_OUT_UNPERMUTE 1 /* input order: the sort passes replayed backwards over the results (coalesced) */
 #define HM_OUT_SCATTER 2 /* input order: each result written straight to its position */
 #define HM_OUT_PSA_ORDER 3 /* PSA order plus out_pos, as the reference returns them (H/regular_bpt/prefix-sort.cu:296-297) */
 
@@ -133,7 +133,7 @@ int hm_epoch_publish(hm_tree* t, void* stream, int* out_rebuilt);
 
 /* Batched pipeline for many uneven query batches.  Batches are packed back to back into the
  * batcher's device buffer (hm_batcher_add); hm_batcher_flush then serves all of them with one
- * hipGraph launch: PSA radix sort -> one persistent search launch -> un-permute, every kernel reading
+ * hipGraph launch: PSA radix sort -> one persistent search launch -> un-sort, every kernel reading
  * the packed query count from device memory, so the graph is captured once and replayed for any
  * count <= capacity.  Batch i's results sit at its offset in the batcher's result buffers.
  * Replaces the reference's fixed-bucket, two-stream host loop (H/regular_bpt/prefix-sort.cu:261-309,
@@ -153,15 +153,17 @@ int hm_batcher_destroy(hm_batcher* b);
 
 /* Serving loop.  The tree stays resident; lookup, range and update requests are admitted as they
  * arrive.  Worker threads (one HIP stream each, max_inflight of them) repeatedly take every pending
- * read request up to the next update (continuous batching: whatever arrived while the other workers'
- * batches were on the GPU), coalesce the lookups into one batched search and the ranges into one
- * batched scan, subject to the device-memory budget for in-flight batches; updates are applied
- * between read epochs (hm_epoch_publish), so a read sees the tree before or after an update, never
- * in between.  The reference has no serving layer (SURVEY: one driver per bucket, no latency). */
+ * read request (continuous batching: whatever arrived while the other workers' batches were on the
+ * GPU), coalesce the lookups into one batched search and the ranges into one batched scan, subject
+ * to the device-memory budget for in-flight batches.  When the oldest pending request is an update,
+ * every pending update is applied as one epoch: staged and prepared while reads continue on the
+ * current epoch, then committed under a short exclusive lock, so a read sees the tree before or
+ * after an epoch, never in between.  The reference has no serving layer: its drivers run one fixed
+ * bucket after another and report no latency (H/regular_bpt/prefix-sort.cu:261-349). */
 typedef struct hm_server hm_server;
 typedef struct hm_server_params {
   uint64_t mem_budget;   /* device bytes for in-flight request batches (0 = 1 GiB) */
-  uint32_t max_batch;    /* queries (or range keys) coalesced into one launch (0 = 1 << 22) */
+  uint32_t max_batch;    /* queries (or range keys) coalesced into one launch (0 = 1 << 20) */
   uint32_t max_inflight; /* batches on the GPU at once, one worker + stream each (0 = 4) */
   uint32_t coalesce;     /* 1 = continuous batching; 0 = one request per launch (the plain path) */
   uint32_t reserved;
