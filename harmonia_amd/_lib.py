@@ -125,6 +125,10 @@ class HmServerStats(ctypes.Structure):
         ("mean_us", ctypes.c_double),
         ("max_us", ctypes.c_double),
         ("span_s", ctypes.c_double),
+        ("max_read_batch_us", ctypes.c_double),
+        ("max_update_us", ctypes.c_double),
+        ("max_prepare_us", ctypes.c_double),
+        ("max_commit_wait_us", ctypes.c_double),
     ]
 
     def as_dict(self):

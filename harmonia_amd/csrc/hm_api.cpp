@@ -230,18 +230,22 @@ int enqueue_pipeline(const TreeView& tv, const SearchLaunch& sl, int psa_bits, i
     // passes run backwards to put the results in input order
     HM_TRY(radix_sort_keys_replay(dq, n, 64 - psa_bits, 64, b.k0, b.k1, b.replay, b.tmp, s, &which, dn));
     const uint64_t* sk = which ? b.k1 : b.k0;
-    HM_TRY(launch_search(tv, sl, sk, nullptr, b.rv, b.rf, n, OUT_DIRECT, s, dn));
+    SearchLaunch ss = sl;
+    ss.sorted = true;
+    HM_TRY(launch_search(tv, ss, sk, nullptr, b.rv, b.rf, n, OUT_DIRECT, s, dn));
     HM_TRY(unsort_results(b.rv, b.rf, n, psa_bits, b.replay, b.vt, b.ft, dvals, dfound, s, dn));
     return HM_OK;
   }
   HM_TRY(radix_sort_pos(dq, n, 64 - psa_bits, 64, b.k0, b.p0, b.k1, b.p1, b.tmp, s, &which, dn));
   const uint64_t* sk = which ? b.k1 : b.k0;
   const uint32_t* sp = which ? b.p1 : b.p0;
+  SearchLaunch ss = sl;
+  ss.sorted = true;
   if (out_mode == HM_OUT_PSA_ORDER) {
-    HM_TRY(launch_search(tv, sl, sk, sp, dvals, dfound, n, OUT_DIRECT, s, dn));
+    HM_TRY(launch_search(tv, ss, sk, sp, dvals, dfound, n, OUT_DIRECT, s, dn));
     HM_TRY(hipMemcpyAsync(dpos, sp, n * 4, hipMemcpyDeviceToDevice, s));
   } else {
-    HM_TRY(launch_search(tv, sl, sk, sp, dvals, dfound, n, OUT_SCATTER, s, dn));
+    HM_TRY(launch_search(tv, ss, sk, sp, dvals, dfound, n, OUT_SCATTER, s, dn));
   }
   return HM_OK;
 }

@@ -5,15 +5,35 @@
 // H2D -> SortPairs -> kernel -> D2H sequence per bucket (H/regular_bpt/prefix-sort.cu:261-309).
 // Here the graph is captured once per (tree layout, path) and replayed for any packed count up to
 // the capacity; small flushes skip PSA (the tree's sort threshold), large ones sort first.
+//
+// Adding a batch costs no device call: a device batch is recorded (pointer, count, slot) and a host
+// batch is copied into pinned staging.  The flush packs everything with one H2D copy of the staged
+// host part and one gather kernel over the recorded device batches, then replays the graph.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
+#include <vector>
 
 #include "hm_tree.h"
 
 using namespace hm;
 
 namespace {
+
+constexpr uint64_t kGatherPiece = 1u << 16;  // queries per gather block
+
+struct GatherItem {
+  const uint64_t* src;
+  uint64_t dst;  // slot in the batcher buffer
+  uint64_t n;    // <= kGatherPiece
+};
+
+__global__ __launch_bounds__(256) void gather_kernel(const GatherItem* __restrict__ items, uint64_t* __restrict__ dst) {
+  const GatherItem it = items[blockIdx.x];
+  uint64_t* d = dst + it.dst;
+  for (uint64_t i = threadIdx.x; i < it.n; i += blockDim.x) d[i] = it.src[i];
+}
 
 struct Graph {
   hipGraph_t graph = nullptr;
@@ -44,6 +64,19 @@ struct hm_batcher {
   void* scratch = nullptr;
   Graph g_psa, g_direct;
   uint64_t fill = 0;
+  // batches added since the last flush
+  std::vector<GatherItem> items;     // device batches, split into pieces of at most kGatherPiece
+  uint64_t* h_stage = nullptr;       // pinned copy of host batches (capacity u64, allocated on first use)
+  uint64_t host_lo = ~0ull, host_hi = 0;
+  hipEvent_t stage_ev = nullptr;     // the last upload of h_stage (host adds wait for it)
+  bool stage_busy = false;
+  // gather lists: two pinned upload buffers used alternately, each reused once its upload is done
+  GatherItem* h_items[2] = {nullptr, nullptr};
+  GatherItem* d_items[2] = {nullptr, nullptr};
+  size_t items_cap[2] = {0, 0};
+  hipEvent_t items_ev[2] = {nullptr, nullptr};
+  bool items_busy[2] = {false, false};
+  int slot = 0;
 };
 
 namespace {
@@ -53,8 +86,13 @@ void free_batcher(hm_batcher* b) {
   if (b->s) (void)hipStreamSynchronize(b->s);
   b->g_psa.reset();
   b->g_direct.reset();
-  for (void* p : {(void*)b->d_q, (void*)b->d_vals, (void*)b->d_found, (void*)b->d_n, b->scratch})
+  for (void* p : {(void*)b->d_q, (void*)b->d_vals, (void*)b->d_found, (void*)b->d_n, b->scratch,
+                  (void*)b->d_items[0], (void*)b->d_items[1]})
     if (p) (void)hipFree(p);
+  for (void* p : {(void*)b->h_stage, (void*)b->h_items[0], (void*)b->h_items[1]})
+    if (p) (void)hipHostFree(p);
+  for (hipEvent_t e : {b->stage_ev, b->items_ev[0], b->items_ev[1]})
+    if (e) (void)hipEventDestroy(e);
   if (b->own_stream && b->s) (void)hipStreamDestroy(b->s);
   delete b;
 }
@@ -142,12 +180,69 @@ int hm_batcher_add(hm_batcher* b, const uint64_t* q, uint64_t n, uint64_t* out_o
   if (b->fill + n > b->cap) return set_err(HM_EBUDGET, "batch of %llu does not fit (%llu of %llu used)",
                                            (unsigned long long)n, (unsigned long long)b->fill,
                                            (unsigned long long)b->cap);
-  const hipMemcpyKind k = is_device_ptr(q) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-  HM_TRY(hipMemcpyAsync(b->d_q + b->fill, q, n * 8, k, b->s));
+  if (is_device_ptr(q)) {
+    for (uint64_t o = 0; o < n; o += kGatherPiece)
+      b->items.push_back(GatherItem{q + o, b->fill + o, std::min<uint64_t>(kGatherPiece, n - o)});
+  } else {
+    if (!b->h_stage) {
+      HM_TRY(hipHostMalloc((void**)&b->h_stage, b->cap * 8, hipHostMallocDefault));
+      HM_TRY(hipEventCreateWithFlags(&b->stage_ev, hipEventDisableTiming));
+    }
+    if (b->stage_busy) {  // the previous flush may still be reading the staging buffer
+      HM_TRY(hipEventSynchronize(b->stage_ev));
+      b->stage_busy = false;
+    }
+    std::memcpy(b->h_stage + b->fill, q, n * 8);
+    b->host_lo = std::min(b->host_lo, b->fill);
+    b->host_hi = b->fill + n;
+  }
   if (out_offset) *out_offset = b->fill;
   b->fill += n;
   return HM_OK;
 }
+
+namespace {
+// Pack the added batches into d_q (stream order on b->s).
+int pack(hm_batcher* b) {
+  if (b->host_hi > b->host_lo) {  // host batches first: device batches inside the range are re-gathered
+    HM_TRY(hipMemcpyAsync(b->d_q + b->host_lo, b->h_stage + b->host_lo, (b->host_hi - b->host_lo) * 8,
+                          hipMemcpyHostToDevice, b->s));
+    HM_TRY(hipEventRecord(b->stage_ev, b->s));
+    b->stage_busy = true;
+  }
+  const size_t m = b->items.size();
+  if (m) {
+    const int k = b->slot;
+    b->slot ^= 1;
+    if (b->items_busy[k]) {  // this upload buffer was used two flushes ago
+      HM_TRY(hipEventSynchronize(b->items_ev[k]));
+      b->items_busy[k] = false;
+    }
+    if (m > b->items_cap[k]) {
+      if (b->h_items[k]) HM_TRY(hipHostFree(b->h_items[k]));
+      if (b->d_items[k]) HM_TRY(hipFree(b->d_items[k]));  // its last use has completed (event above)
+      b->h_items[k] = nullptr;
+      b->d_items[k] = nullptr;
+      b->items_cap[k] = 0;
+      const size_t c = std::max<size_t>(2 * m, 1024);
+      HM_TRY(hipHostMalloc((void**)&b->h_items[k], c * sizeof(GatherItem), hipHostMallocDefault));
+      HM_TRY(hipMalloc(&b->d_items[k], c * sizeof(GatherItem)));
+      if (!b->items_ev[k]) HM_TRY(hipEventCreateWithFlags(&b->items_ev[k], hipEventDisableTiming));
+      b->items_cap[k] = c;
+    }
+    std::memcpy(b->h_items[k], b->items.data(), m * sizeof(GatherItem));
+    HM_TRY(hipMemcpyAsync(b->d_items[k], b->h_items[k], m * sizeof(GatherItem), hipMemcpyHostToDevice, b->s));
+    hipLaunchKernelGGL(gather_kernel, dim3((unsigned)m), dim3(256), 0, b->s, b->d_items[k], b->d_q);
+    HM_TRY(hipGetLastError());
+    HM_TRY(hipEventRecord(b->items_ev[k], b->s));
+    b->items_busy[k] = true;
+  }
+  b->items.clear();
+  b->host_lo = ~0ull;
+  b->host_hi = 0;
+  return HM_OK;
+}
+}  // namespace
 
 int hm_batcher_flush(hm_batcher* b) {
   if (!b) return set_err(HM_EINVAL, "batcher is NULL");
@@ -165,6 +260,8 @@ int hm_batcher_flush(hm_batcher* b) {
   int rc = ensure_ws(t, b->s, 0, &w);  // registers the stream so epoch publishes wait for it
   if (rc) return rc;
   if (t->publish_valid) HM_TRY(hipStreamWaitEvent(b->s, t->publish_ev, 0));
+  rc = pack(b);
+  if (rc) return rc;
   HM_TRY(launch_set_u32(b->d_n, (uint32_t)b->fill, b->s));
   {
     ProfScope prof("batcher_graph", b->s);

@@ -166,6 +166,92 @@ __device__ __forceinline__ void descend_q(const TreeView& tv, int lds_levels, co
   for (int q = 0; q < Q; ++q) node[q] -= tv.n_inner;
 }
 
+// Make this wave's LDS writes visible to its own later LDS reads (and keep the compiler from moving
+// LDS accesses across): the LDS executes one wave's operations in order, so no block barrier is
+// needed for a wave-private buffer.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// upper_bound rank of t in a node held in LDS (one lane per query).
+template <int F>
+__device__ __forceinline__ uint32_t node_rank_lds(const uint64_t* nd, uint64_t t) {
+  constexpr int IS = idx_slots(F);
+  int j = 0;
+#pragma unroll
+  for (int i = 0; i < IS; ++i) j += nd[i] <= t ? 1 : 0;
+  const uint64_t* g = nd + IS + 8 * j;
+  int c = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) c += g[i] <= t ? 1 : 0;
+  return (uint32_t)(8 * j + c);
+}
+
+// One global level of the descent for PSA-sorted queries, one lane per query, Q queries per lane.
+// Neighbouring sorted queries mostly share their node, so instead of every lane reading two 64-B
+// granules of it, the wave loads each distinct node once (coalesced 16-B pieces) into its private
+// LDS slot and the lanes that need it rank from LDS.  Up to K distinct nodes per query slot are
+// served that way; lanes left over (unsorted input) read their node directly.
+template <int F, int Q, int K>
+__device__ __forceinline__ void sorted_level(const uint64_t* __restrict__ inner, const uint32_t* __restrict__ prefix,
+                                             uint64_t* wslot, const uint64_t (&t)[Q], uint32_t (&node)[Q], int lane) {
+  constexpr int NS = node_slots(F);
+  constexpr int CH = NS / 2;          // 16-byte pieces per node
+  constexpr int PL = (CH + 63) / 64;  // pieces per lane
+  uint32_t todo = (1u << Q) - 1u;     // this lane's queries without a result yet (bit q)
+  uint32_t res[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) res[q] = 0;
+#pragma unroll 1
+  for (int it = 0; it < K; ++it) {
+    uint32_t n0[Q];
+    uint32_t live = 0;  // wave-uniform: query slots some lane still needs
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const uint64_t act = __ballot((todo >> q) & 1u);
+      if (act) live |= 1u << q;
+      n0[q] = act ? (uint32_t)__builtin_amdgcn_readlane((int)node[q], __ffsll((long long)act) - 1) : 0u;
+    }
+    if (!live) break;
+    // every slot is loaded (a finished slot reloads node n0 = 0, harmless) so all loads of the
+    // step are issued before the first is waited for and stay in registers
+    ulonglong2 piece[Q][PL];
+    uint32_t base[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const ulonglong2* src = reinterpret_cast<const ulonglong2*>(inner + (size_t)n0[q] * NS);
+#pragma unroll
+      for (int k = 0; k < PL; ++k) {
+        const int idx = lane + 64 * k;
+        piece[q][k] = src[idx < CH ? idx : CH - 1];
+      }
+      base[q] = prefix[n0[q]];
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+#pragma unroll
+      for (int k = 0; k < PL; ++k)
+        if (lane + 64 * k < CH) reinterpret_cast<ulonglong2*>(wslot + q * NS)[lane + 64 * k] = piece[q][k];
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      if (((todo >> q) & 1u) && node[q] == n0[q]) {
+        res[q] = base[q] + node_rank_lds<F>(wslot + q * NS, t[q]);
+        todo &= ~(1u << q);
+      }
+    }
+    wave_lds_sync();  // before the next iteration overwrites the slots
+  }
+#pragma unroll
+  for (int q = 0; q < Q; ++q)
+    if ((todo >> q) & 1u) res[q] = prefix[node[q]] + node_rank<F, 1>(inner + (size_t)node[q] * NS, t[q], 0, lane);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) node[q] = res[q];
+}
+
 // Stage the top `lds_levels` levels (nodes, then their prefix entries) into dynamic LDS.
 template <int F>
 __device__ __forceinline__ void stage_top_levels(const TreeView& tv, int lds_levels, uint64_t* smem,

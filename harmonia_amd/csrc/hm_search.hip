@@ -14,6 +14,12 @@ namespace {
 
 constexpr int kThreads = 256;
 
+size_t sorted_lds_bytes(const TreeView& tv, int levels, int Q) {
+  const size_t nl = levels > 0 ? tv.level_start[levels] : 0;
+  const size_t top_words = (nl * node_slots(tv.fanout) + (nl + 1) / 2 + 1) & ~(size_t)1;
+  return (top_words + (size_t)(kThreads / 64) * Q * node_slots(tv.fanout)) * sizeof(uint64_t);
+}
+
 template <int F, int G, int Q, int OUT>
 __global__ __launch_bounds__(kThreads) void search_kernel(TreeView tv, int lds_levels,
                                                           const uint64_t* __restrict__ q,
@@ -89,6 +95,73 @@ __global__ __launch_bounds__(kThreads) void search_kernel(TreeView tv, int lds_l
   }
 }
 
+// Search over PSA-sorted queries, one lane per query, Q per lane: the global levels go through
+// sorted_level (each distinct node of a wave read once into the wave's LDS slot), the leaf is read
+// per lane (its 8 keys, then only the matching value).  Wave slots follow the staged top levels.
+template <int F, int Q, int OUT>
+__global__ __launch_bounds__(kThreads) void search_sorted_kernel(TreeView tv, int lds_levels,
+                                                                 const uint64_t* __restrict__ q,
+                                                                 const uint32_t* __restrict__ pos,
+                                                                 uint64_t* __restrict__ out_vals,
+                                                                 uint8_t* __restrict__ out_found, uint32_t n,
+                                                                 const uint32_t* __restrict__ dn) {
+  extern __shared__ __align__(16) uint64_t smem[];
+  uint32_t* spre;
+  stage_top_levels<F>(tv, lds_levels, smem, spre);
+  constexpr int NS = node_slots(F);
+  const uint32_t nl = lds_levels > 0 ? tv.level_start[lds_levels] : 0u;
+  const uint32_t top_words = (nl * NS + (nl + 1) / 2 + 1) & ~1u;  // 16-byte aligned
+  uint64_t* wslot = smem + top_words + (threadIdx.x >> 6) * (Q * NS);
+  const int lane = threadIdx.x & 63;
+  constexpr uint32_t tile = kThreads * Q;
+  if (dn) n = *dn;
+  uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  chunk = (chunk + tile - 1) / tile * tile;
+  const uint32_t vb = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t begin = vb * chunk;
+  const uint32_t end = begin + chunk < n ? begin + chunk : n;
+  const uint64_t* __restrict__ inner = tv.inner;
+  const uint32_t* __restrict__ prefix = tv.prefix;
+
+  for (uint32_t qb = begin; qb < end; qb += tile) {  // uniform across the block
+    uint64_t t[Q], td[Q];
+    uint32_t node[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const uint32_t qi = qb + i * kThreads + threadIdx.x;
+      t[i] = q[qi < end ? qi : end - 1];  // lanes past the end repeat the last query (all lanes take part)
+      td[i] = t[i] == KEY_MAX ? KEY_MAX - 1 : t[i];
+      node[i] = 0;
+    }
+    for (int l = 0; l < lds_levels; ++l) {
+#pragma unroll
+      for (int i = 0; i < Q; ++i) node[i] = spre[node[i]] + node_rank_lds<F>(smem + (size_t)node[i] * NS, td[i]);
+    }
+    for (int l = lds_levels; l < tv.height; ++l) sorted_level<F, Q, 4>(inner, prefix, wslot, td, node, lane);
+    uint64_t k[Q][8];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) load_keys<8>(tv.leaves[node[i] - tv.n_inner].key, k[i]);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const uint32_t qi = qb + i * kThreads + threadIdx.x;
+      const uint32_t leaf = node[i] - tv.n_inner;
+      const int lt = group_count_lt<1, 8>(k[i], t[i], 0);
+      const bool eq = group_any_eq<1, 8>(k[i], t[i], 0) && t[i] != KEY_MAX;
+      if (qi < end) {
+        const uint64_t val = eq ? tv.leaves[leaf].val[lt] : 0ull;
+        if constexpr (OUT == OUT_DIRECT) {
+          out_vals[qi] = val;
+          out_found[qi] = eq ? 1 : 0;
+        } else {
+          const uint32_t o = pos[qi];
+          out_vals[o] = val;
+          out_found[o] = eq ? 1 : 0;
+        }
+      }
+    }
+  }
+}
+
 // Locate for the update path: leaf id, lower_bound slot and found flag per key (input order).
 template <int F, int G>
 __global__ __launch_bounds__(kThreads) void locate_kernel(TreeView tv, const uint64_t* __restrict__ keys, uint32_t n,
@@ -145,6 +218,28 @@ hipError_t launch_fgq(const TreeView& tv, const SearchLaunch& sl, const uint64_t
   return hipGetLastError();
 }
 
+template <int F, int Q>
+hipError_t launch_sorted_q(const TreeView& tv, const SearchLaunch& sl, const uint64_t* q, const uint32_t* pos,
+                           uint64_t* out_vals, uint8_t* out_found, size_t n, SearchOut mode, hipStream_t s,
+                           const uint32_t* dn) {
+  const LaunchCfg& lc = launch_cfg();
+  constexpr uint32_t per_block = kThreads * Q;
+  size_t max_blocks = (size_t)lc.num_cus * (size_t)(sl.blocks_per_cu > 0 ? sl.blocks_per_cu : 8);
+  size_t need = (n + per_block - 1) / per_block;
+  size_t nb = need < max_blocks ? need : max_blocks;
+  nb = (nb + 7) / 8 * 8;
+  int levels = sl.lds_levels;  // the wave slots come on top of the staged levels: stay within 64 KiB
+  while (levels > 0 && sorted_lds_bytes(tv, levels, Q) > 65536) --levels;
+  const size_t lds = sorted_lds_bytes(tv, levels, Q);
+  if (mode == OUT_DIRECT)
+    hipLaunchKernelGGL((search_sorted_kernel<F, Q, OUT_DIRECT>), dim3((unsigned)nb), dim3(kThreads), lds, s, tv,
+                       levels, q, pos, out_vals, out_found, (uint32_t)n, dn);
+  else
+    hipLaunchKernelGGL((search_sorted_kernel<F, Q, OUT_SCATTER>), dim3((unsigned)nb), dim3(kThreads), lds, s, tv,
+                       levels, q, pos, out_vals, out_found, (uint32_t)n, dn);
+  return hipGetLastError();
+}
+
 template <int F, int G>
 hipError_t launch_fg(const TreeView& tv, const SearchLaunch& sl, const uint64_t* q, const uint32_t* pos,
                      uint64_t* out_vals, uint8_t* out_found, size_t n, SearchOut mode, hipStream_t s,
@@ -161,6 +256,14 @@ template <int F>
 hipError_t launch_f(const TreeView& tv, const SearchLaunch& sl, const uint64_t* q, const uint32_t* pos,
                     uint64_t* out_vals, uint8_t* out_found, size_t n, SearchOut mode, hipStream_t s,
                      const uint32_t* dn) {
+  if (sl.sorted && sl.group == 1) {
+    switch (sl.queries_per_group) {
+      case 1: return launch_sorted_q<F, 1>(tv, sl, q, pos, out_vals, out_found, n, mode, s, dn);
+      case 2: return launch_sorted_q<F, 2>(tv, sl, q, pos, out_vals, out_found, n, mode, s, dn);
+      case 4: return launch_sorted_q<F, 4>(tv, sl, q, pos, out_vals, out_found, n, mode, s, dn);
+      default: return hipErrorInvalidValue;
+    }
+  }
   switch (sl.group) {
     case 1: return launch_fg<F, 1>(tv, sl, q, pos, out_vals, out_found, n, mode, s, dn);
     case 2: return launch_fg<F, 2>(tv, sl, q, pos, out_vals, out_found, n, mode, s, dn);

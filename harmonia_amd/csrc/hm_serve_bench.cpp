@@ -231,13 +231,16 @@ extern "C" int hm_serve_bench_main(int argc, char** argv, char* out_json, uint64
       "\"span_s\": %.4f, \"submit_s\": %.4f, \"queries_per_s\": %.1f, \"requests_per_s\": %.1f, \"p50_us\": %.1f, "
       "\"p99_us\": %.1f, \"mean_us\": %.1f, \"max_us\": %.1f, \"launches\": %llu, \"epochs\": %llu, "
       "\"lookups\": %llu, \"ranges\": %llu, \"updates\": %llu, \"bad_results\": %llu, \"keys\": %llu, "
-      "\"inflight\": %u, \"build_s\": %.2f, \"leaf_fill\": %u, \"relayouts\": %llu}\n",
+      "\"inflight\": %u, \"build_s\": %.2f, \"leaf_fill\": %u, \"relayouts\": %llu, "
+      "\"max_read_batch_us\": %.1f, \"max_update_us\": %.1f, \"max_prepare_us\": %.1f, "
+      "\"max_commit_wait_us\": %.1f}\n",
       plain ? "plain" : "served", closed ? "closed" : "poisson", rate, (unsigned long long)n_req,
       (unsigned long long)st.queries, st.span_s, submit_s, st.queries / st.span_s, st.completed / st.span_s,
       st.p50_us, st.p99_us, st.mean_us, st.max_us, (unsigned long long)st.launches, (unsigned long long)st.epochs,
       (unsigned long long)st.lookups, (unsigned long long)st.ranges, (unsigned long long)st.updates,
       (unsigned long long)bad, (unsigned long long)n, sp.max_inflight, build_s, p.leaf_fill,
-      (unsigned long long)ts.relayouts);
+      (unsigned long long)ts.relayouts, st.max_read_batch_us, st.max_update_us, st.max_prepare_us,
+      st.max_commit_wait_us);
   return bad ? 3 : 0;
 }
 

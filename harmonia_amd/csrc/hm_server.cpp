@@ -120,6 +120,7 @@ struct hm_server {
   uint64_t inflight_bytes = 0;
   uint64_t outstanding = 0;  // submitted, not yet completed
   uint32_t idle = 0;         // workers waiting for work
+  bool updating = false;     // a worker is applying an update batch
   std::shared_mutex epoch_mu;
   std::vector<std::unique_ptr<Worker>> workers;
   // statistics
@@ -128,6 +129,7 @@ struct hm_server {
            n_epochs = 0;
   bool have_first = false;
   Clock::time_point first_submit, last_complete;
+  double max_read_batch_us = 0, max_update_us = 0, max_prepare_us = 0, max_commit_wait_us = 0;
 };
 
 namespace {
@@ -241,11 +243,17 @@ void run_updates(hm_server* sv, Worker& w, std::vector<ReqPtr>& ups, std::vector
   std::lock_guard<std::mutex> pub(t->pub_mu);
   for (size_t i = 0; i < ups.size(); ++i) status[i] = stage_updates(t, ups[i]->keys, ups[i]->vals, ups[i]->n, w.s);
   PublishPlan plan;
+  const auto t0 = Clock::now();
   int rc = publish_prepare(t, w.s, &plan);
+  const auto t1 = Clock::now();
+  double wait_us = 0;
   if (!rc) {
     // no new read batch may start while the commit waits for the ones still enqueueing
     std::unique_lock<std::mutex> lk(sv->mu);
     std::unique_lock<std::shared_mutex> ex(sv->epoch_mu);
+    wait_us = std::chrono::duration<double, std::micro>(Clock::now() - t1).count();
+    sv->max_prepare_us = std::max(sv->max_prepare_us, std::chrono::duration<double, std::micro>(t1 - t0).count());
+    sv->max_commit_wait_us = std::max(sv->max_commit_wait_us, wait_us);
     lk.unlock();
     int rebuilt = 0;
     rc = publish_commit(t, w.s, plan, &rebuilt);
@@ -286,32 +294,36 @@ void worker_loop(hm_server* sv, Worker* w) {
     {
       std::unique_lock<std::mutex> lk(sv->mu);
       ++sv->idle;
+      // one update batch at a time: while it is being applied, the other workers serve reads
+      // (pending updates wait for the next epoch instead of parking workers on the publish lock)
+      auto update_ready = [&] { return !sv->updating && sv->pending.front()->type == REQ_UPDATE; };
       sv->cv_work.wait(lk, [&] {
-        if (sv->stop) return true;
-        if (sv->pending.empty()) return false;
-        if (sv->pending.front()->type == REQ_UPDATE) return true;
-        if (!sv->p.coalesce) return sv->inflight_bytes == 0;
+        if (sv->pending.empty()) return sv->stop;  // on stop, every pending request is served first
+        if (update_ready()) return true;
+        if (!sv->p.coalesce) return sv->inflight_bytes == 0 && sv->pending.front()->type != REQ_UPDATE;
         const Request* h = first_read(sv);
         return h && (sv->inflight_bytes == 0 || sv->inflight_bytes + request_bytes(*h) <= sv->p.mem_budget);
       });
       --sv->idle;
-      if (sv->stop && sv->pending.empty()) return;
+      if (sv->pending.empty()) return;  // stopping
       if (!sv->p.coalesce) {  // one request per launch, in arrival order
         ReqPtr h = sv->pending.front();
         sv->pending.pop_front();
         if (h->type == REQ_UPDATE) {
           ups.push_back(h);
+          sv->updating = true;
         } else {
           (h->type == REQ_LOOKUP ? lookups : ranges).push_back(h);
           bytes = request_bytes(*h);
           sv->inflight_bytes += bytes;
           epoch_lock.lock();
         }
-      } else if (sv->pending.front()->type == REQ_UPDATE) {
+      } else if (update_ready()) {
         // every pending update, in submission order, as one epoch
         std::deque<ReqPtr> rest;
         for (auto& r : sv->pending) (r->type == REQ_UPDATE ? ups.push_back(r) : rest.push_back(r));
         sv->pending.swap(rest);
+        sv->updating = true;
       } else {
         // every pending read that fits the batch and memory limits
         uint64_t nq = 0;
@@ -342,12 +354,16 @@ void worker_loop(hm_server* sv, Worker* w) {
       if (!sv->pending.empty()) sv->cv_work.notify_one();
     }
     uint64_t epoch = 0;
+    const auto tb = Clock::now();
     if (!ups.empty()) {
       std::vector<int> status;
       run_updates(sv, *w, ups, status, &epoch);
       {
         std::lock_guard<std::mutex> g(sv->mu);
         ++sv->n_epochs;
+        sv->updating = false;
+        sv->max_update_us =
+            std::max(sv->max_update_us, std::chrono::duration<double, std::micro>(Clock::now() - tb).count());
       }
       complete_all(sv, ups, status.data(), HM_OK, epoch);
       sv->cv_work.notify_all();
@@ -357,6 +373,8 @@ void worker_loop(hm_server* sv, Worker* w) {
       {
         std::lock_guard<std::mutex> g(sv->mu);
         sv->inflight_bytes -= bytes;
+        sv->max_read_batch_us =
+            std::max(sv->max_read_batch_us, std::chrono::duration<double, std::micro>(Clock::now() - tb).count());
       }
       sv->cv_work.notify_all();
       complete_all(sv, lookups, nullptr, rc, epoch);
@@ -510,6 +528,10 @@ int hm_server_get_stats(hm_server* s, hm_server_stats* st) {
     st->max_us = v.back();
     st->span_s = std::chrono::duration<double>(s->last_complete - s->first_submit).count();
   }
+  st->max_read_batch_us = s->max_read_batch_us;
+  st->max_update_us = s->max_update_us;
+  st->max_prepare_us = s->max_prepare_us;
+  st->max_commit_wait_us = s->max_commit_wait_us;
   return HM_OK;
 }
 
@@ -518,6 +540,7 @@ int hm_server_reset_stats(hm_server* s) {
   std::lock_guard<std::mutex> g(s->mu);
   s->lat_us.clear();
   s->n_lookups = s->n_ranges = s->n_updates = s->n_queries = s->n_rejected = s->n_launches = s->n_epochs = 0;
+  s->max_read_batch_us = s->max_update_us = s->max_prepare_us = s->max_commit_wait_us = 0;
   s->have_first = false;
   return HM_OK;
 }

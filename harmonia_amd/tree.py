@@ -319,11 +319,14 @@ class Batcher:
         check(self._lib.hm_batcher_create(tree._handle(), self.capacity, ctypes.byref(o), _stream(stream),
                                           ctypes.byref(h)))
         self._h = h
+        self._keep = []  # device batches are read by the flush: keep them alive until sync()
 
     def add(self, q):
         q = _u64(q)
         off = ctypes.c_uint64(0)
         check(self._lib.hm_batcher_add(self._h, _ptr(q), _len(q), ctypes.byref(off)))
+        if hasattr(q, "data_ptr"):
+            self._keep.append(q)
         return int(off.value)
 
     def flush(self):
@@ -344,11 +347,13 @@ class Batcher:
 
     def sync(self):
         check(self._lib.hm_batcher_sync(self._h))
+        self._keep.clear()
 
     def close(self):
         if self._h is not None:
-            self._lib.hm_batcher_destroy(self._h)
+            self._lib.hm_batcher_destroy(self._h)  # waits for the batcher's stream
             self._h = None
+            self._keep.clear()
 
     def __del__(self):
         try:

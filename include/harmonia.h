@@ -141,7 +141,9 @@ int hm_epoch_publish(hm_tree* t, void* stream, int* out_rebuilt);
  * output_mode may be HM_OUT_UNPERMUTE or HM_OUT_SCATTER.  A batcher is used from one thread. */
 typedef struct hm_batcher hm_batcher;
 int hm_batcher_create(hm_tree* t, uint64_t capacity, const hm_search_opts* opts, void* stream, hm_batcher** out);
-/* Copy one batch in (host or device pointer); *out_offset = its first slot. HM_EBUDGET if full. */
+/* Add one batch; *out_offset = its first slot.  HM_EBUDGET if full.  A host batch is copied before
+ * the call returns; a device batch is only recorded and read by the next flush's work on the
+ * batcher's stream, so keep it valid and unchanged until then (hm_batcher_sync). */
 int hm_batcher_add(hm_batcher* b, const uint64_t* q, uint64_t n, uint64_t* out_offset);
 /* Launch the pipeline over everything added since the last flush (asynchronous). */
 int hm_batcher_flush(hm_batcher* b);
@@ -172,6 +174,9 @@ typedef struct hm_server_stats {
   uint64_t completed, lookups, ranges, updates, queries, rejected, launches, epochs;
   double p50_us, p99_us, mean_us, max_us; /* request latency, submit -> results written */
   double span_s;                          /* first submit -> last completion */
+  /* worker-side durations (longest since the last reset): one read batch from taking it to its
+   * results written; one update epoch, its prepare phase, and the wait for the exclusive commit */
+  double max_read_batch_us, max_update_us, max_prepare_us, max_commit_wait_us;
 } hm_server_stats;
 int hm_server_create(hm_tree* t, const hm_server_params* p, hm_server** out);
 /* Requests: every buffer (inputs and outputs, host memory) must stay valid and unchanged until the

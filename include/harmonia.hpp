@@ -147,6 +147,12 @@ class BPlusTree {
     if (!t_) throw Error(HM_EINVAL, "harmonia: tree is not built");
     return t_;
   }
+  // Binary snapshot of the published tree / reload (hm_tree_save, hm_tree_load).
+  void save(const std::string& path) const { check(hm_tree_save(handle(), path.c_str())); }
+  void load(const std::string& path) {
+    reset();
+    check(hm_tree_load(path.c_str(), &params_, &t_));
+  }
 
  private:
   void reset() {
@@ -157,6 +163,16 @@ class BPlusTree {
   hm_tree* t_ = nullptr;
   std::vector<uint64_t> keys_, vals_;
 };
+
+// Keys of a text dataset in the reference's format (one decimal key per line, H/src/main.cpp:50-68).
+inline std::vector<uint64_t> read_keys_text(const std::string& path) {
+  uint64_t n = 0;
+  check(hm_read_keys_text(path.c_str(), nullptr, 0, &n));
+  std::vector<uint64_t> keys(n);
+  check(hm_read_keys_text(path.c_str(), keys.data(), n, &n));
+  keys.resize(n);
+  return keys;
+}
 
 }  // namespace harmonia
 

@@ -1,5 +1,7 @@
 """Text datasets in the reference's format (one decimal key per line; range files as start/end line
 pairs), parsed by the library's host code -- no GPU needed."""
+import os
+
 import numpy as np
 import pytest
 
@@ -45,3 +47,32 @@ def test_ranges(tmp_path):
     assert np.array_equal(a, s) and np.array_equal(b, e)
     with pytest.raises(HarmoniaError, match="odd"):
         read_ranges_text(write(tmp_path, "odd.txt", "1\n2\n3\n"))
+
+
+def test_cpp_mirror_reads_dataset(tmp_path):
+    """The C++ mirror (include/harmonia.hpp) compiles against the C ABI, links the library and reads a
+    dataset through it (host code only)."""
+    import shutil
+    import subprocess
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    if not shutil.which("g++"):
+        pytest.skip("no g++")
+    src = tmp_path / "m.cpp"
+    src.write_text("\n".join([
+        '#include "harmonia.hpp"',
+        "#include <cstdio>",
+        "int main(int c, char** v) {",
+        "  auto k = harmonia::read_keys_text(v[1]);",
+        "  unsigned long long s = 0;",
+        "  for (auto x : k) s += x;",
+        '  std::printf("%zu %llu\\n", k.size(), s);',
+        "  return 0;",
+        "}", ""]))
+    exe = tmp_path / "m"
+    lib = os.path.join(root, "harmonia_amd")
+    subprocess.check_call(["g++", "-std=c++17", "-I", os.path.join(root, "include"), str(src), "-L", lib,
+                           "-lharmonia_amd", f"-Wl,-rpath,{lib}", "-o", str(exe)])
+    data = write(tmp_path, "k.txt", "5\n7\n11\n")
+    out = subprocess.check_output([str(exe), data], text=True).split()
+    assert out == ["3", "23"]

@@ -102,3 +102,33 @@ def test_autotune_keeps_results_exact(gpu):
     assert r2["fanout"] == r["fanout"]
     v, f = t.search_batch(q)
     assert np.array_equal(f, ef) and np.array_equal(v, ev)
+
+
+def test_batcher_device_and_host_batches(gpu):
+    """Device batches (gathered by the flush, pieces of 2^16), host batches (staged), mixed in one
+    flush, several flushes back to back before one sync."""
+    import torch
+
+    t, sk, sv = make(300000, 12)
+    b = Batcher(t, 1 << 19)
+    rounds = []
+    for r in range(3):
+        sizes = [70000, 5, 131072 + 17, 3000, 1, 40000]
+        offs = []
+        for i, m in enumerate(sizes):
+            q = W.point_queries(sk, m, 1000 * r + i)
+            src = torch.from_numpy(q.view(np.int64)).to(gpu) if i % 2 == 0 else q
+            offs.append((b.add(src), q))
+        b.flush()
+        out = [(off, q, torch.empty(len(q), dtype=torch.int64, device=gpu),
+                torch.empty(len(q), dtype=torch.uint8, device=gpu)) for off, q in offs]
+        rounds.append(out)
+        # device-to-device copies in stream order, before the next round's flush reuses the buffers
+        for off, q, v, f in out:
+            b.copy_out(off, len(q), v, f)
+    b.sync()
+    for out in rounds:
+        for off, q, v, f in out:
+            ev, ef = oracle.search_sorted(sk, sv, q)
+            assert np.array_equal(f.cpu().numpy(), ef) and np.array_equal(v.cpu().numpy().view(np.uint64), ev), off
+    b.close()
