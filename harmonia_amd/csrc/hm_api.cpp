@@ -40,6 +40,20 @@ int set_err(int code, const char* fmt, ...) {
 
 size_t align256(size_t x) { return (x + 255) / 256 * 256; }
 
+int host_alloc_mapped(void** p, size_t bytes) {
+  *p = nullptr;
+  HM_TRY(hipHostMalloc(p, bytes, hipHostMallocMapped | hipHostMallocCoherent));
+  void* dp = nullptr;
+  hipError_t e = hipHostGetDevicePointer(&dp, *p, 0);
+  if (e != hipSuccess || dp != *p) {
+    (void)hipGetLastError();
+    (void)hipHostFree(*p);
+    *p = nullptr;
+    return set_err(HM_EDEVICE, "pinned host memory is not mapped at its host address");
+  }
+  return HM_OK;
+}
+
 bool is_device_ptr(const void* p) {
   if (!p) return false;
   hipPointerAttribute_t a;
@@ -375,6 +389,8 @@ void free_tree(hm_tree* t) {
   if (t->pend_v) (void)hipFree(t->pend_v);
   if (t->upd_buf) (void)hipFree(t->upd_buf);
   if (t->upd_host) (void)hipHostFree(t->upd_host);
+  if (t->pin_stage) (void)hipHostFree(t->pin_stage);
+  if (t->stage_ev) (void)hipEventDestroy(t->stage_ev);
   if (t->sb_leaves) (void)hipFree(t->sb_leaves);
   if (t->sb_inner) (void)hipFree(t->sb_inner);
   if (t->sb_prefix) (void)hipFree(t->sb_prefix);
@@ -563,13 +579,15 @@ int hm_tree_destroy(hm_tree* t) {
   return HM_OK;
 }
 
-int hm_search_ex(hm_tree* t, const uint64_t* q, uint64_t n, uint64_t* out_vals, uint8_t* out_found,
-                 const hm_search_opts* o, void* stream) {
+}  // extern "C"
+
+namespace hm {
+int search_impl(hm_tree* t, const uint64_t* q, uint64_t n, uint64_t* out_vals, uint8_t* out_found,
+                const hm_search_opts* o, hipStream_t s, bool mapped) {
   if (!t) return set_err(HM_EINVAL, "tree is NULL");
   if (n == 0) return HM_OK;
   if (!q || !out_vals || !out_found) return set_err(HM_EINVAL, "NULL buffer");
   if (n > kMaxBatch) return set_err(HM_EINVAL, "batch of %llu queries exceeds 2^31-1", (unsigned long long)n);
-  hipStream_t s = (hipStream_t)stream;
   std::lock_guard<std::mutex> g(t->mu);
 
   SearchLaunch sl;
@@ -577,8 +595,9 @@ int hm_search_ex(hm_tree* t, const uint64_t* q, uint64_t n, uint64_t* out_vals, 
   int rc = resolve_search(t, n, o, &sl, &bits, &out_mode);
   if (rc) return rc;
   const bool psa_order = out_mode == HM_OUT_PSA_ORDER;
-  const bool hq = !is_device_ptr(q), hv = !is_device_ptr(out_vals), hf = !is_device_ptr(out_found);
-  const bool hp = psa_order && !is_device_ptr(o->out_pos);
+  const bool hq = !mapped && !is_device_ptr(q), hv = !mapped && !is_device_ptr(out_vals),
+             hf = !mapped && !is_device_ptr(out_found);
+  const bool hp = psa_order && !mapped && !is_device_ptr(o->out_pos);
   const size_t kq = align256(n * 8), kp = align256(n * 4);
   size_t need = pipe_bytes(n, bits, out_mode);
   if (hq) need += kq;
@@ -613,19 +632,29 @@ int hm_search_ex(hm_tree* t, const uint64_t* q, uint64_t n, uint64_t* out_vals, 
   if (hq || hv || hf || hp) HM_TRY(hipStreamSynchronize(s));
   return HM_OK;
 }
+}  // namespace hm
+
+extern "C" {
+
+int hm_search_ex(hm_tree* t, const uint64_t* q, uint64_t n, uint64_t* out_vals, uint8_t* out_found,
+                 const hm_search_opts* o, void* stream) {
+  return search_impl(t, q, n, out_vals, out_found, o, (hipStream_t)stream, false);
+}
 
 int hm_search(hm_tree* t, const uint64_t* q, uint64_t n, uint64_t* out_vals, uint8_t* out_found, void* stream) {
   return hm_search_ex(t, q, n, out_vals, out_found, nullptr, stream);
 }
 
+}  // extern "C"
+
+namespace hm {
 static int range_common(hm_tree* t, const uint64_t* start, const uint64_t* hi, const uint32_t* len, uint32_t cap_const,
                         uint64_t n, uint64_t* out_offsets, uint64_t* out_keys, uint64_t* out_vals,
-                        uint32_t* out_counts, uint64_t capacity, void* stream) {
-  hipStream_t s = (hipStream_t)stream;
+                        uint32_t* out_counts, uint64_t capacity, hipStream_t s, bool mapped) {
   std::lock_guard<std::mutex> g(t->mu);
-  const bool hs = !is_device_ptr(start), hh = hi && !is_device_ptr(hi), hl = len && !is_device_ptr(len);
-  const bool ho = out_offsets && !is_device_ptr(out_offsets), hk = !is_device_ptr(out_keys),
-             hv = !is_device_ptr(out_vals), hc = !is_device_ptr(out_counts);
+  auto host = [&](const void* p) { return p && !mapped && !is_device_ptr(p); };
+  const bool hs = host(start), hh = host(hi), hl = host(len);
+  const bool ho = host(out_offsets), hk = host(out_keys), hv = host(out_vals), hc = host(out_counts);
   const size_t k8 = align256(n * 8), k8p = align256((n + 1) * 8), k4 = align256(n * 4);
   const size_t cap_bytes = align256(capacity * 8);
   size_t need = (hs ? k8 : 0) + (hh ? k8 : 0) + (hl ? k4 : 0) + (out_offsets ? k8p : 0) + (hk ? cap_bytes : 0) +
@@ -678,13 +707,14 @@ static int range_common(hm_tree* t, const uint64_t* start, const uint64_t* hi, c
   return HM_OK;
 }
 
-int hm_range(hm_tree* t, const uint64_t* start, const uint32_t* len, uint64_t n, uint64_t* out_offsets,
-             uint64_t* out_keys, uint64_t* out_vals, uint32_t* out_counts, uint64_t capacity, void* stream) {
+int range_impl(hm_tree* t, const uint64_t* start, const uint32_t* len, uint64_t n, uint64_t* out_offsets,
+               uint64_t* out_keys, uint64_t* out_vals, uint32_t* out_counts, uint64_t capacity, hipStream_t s,
+               bool mapped) {
   if (!t) return set_err(HM_EINVAL, "tree is NULL");
   if (n == 0) {
     if (out_offsets) {
       uint64_t z = 0;
-      if (is_device_ptr(out_offsets))
+      if (!mapped && is_device_ptr(out_offsets))
         HM_TRY(hipMemcpy(out_offsets, &z, 8, hipMemcpyHostToDevice));
       else
         out_offsets[0] = 0;
@@ -694,7 +724,16 @@ int hm_range(hm_tree* t, const uint64_t* start, const uint32_t* len, uint64_t n,
   if (!start || !len || !out_offsets || !out_keys || !out_vals || !out_counts)
     return set_err(HM_EINVAL, "NULL buffer");
   if (n > kMaxBatch) return set_err(HM_EINVAL, "too many ranges");
-  return range_common(t, start, nullptr, len, 0, n, out_offsets, out_keys, out_vals, out_counts, capacity, stream);
+  return range_common(t, start, nullptr, len, 0, n, out_offsets, out_keys, out_vals, out_counts, capacity, s, mapped);
+}
+}  // namespace hm
+
+extern "C" {
+
+int hm_range(hm_tree* t, const uint64_t* start, const uint32_t* len, uint64_t n, uint64_t* out_offsets,
+             uint64_t* out_keys, uint64_t* out_vals, uint32_t* out_counts, uint64_t capacity, void* stream) {
+  return range_impl(t, start, len, n, out_offsets, out_keys, out_vals, out_counts, capacity, (hipStream_t)stream,
+                    false);
 }
 
 int hm_range_between(hm_tree* t, const uint64_t* lo, const uint64_t* hi, uint64_t n, uint32_t cap, uint64_t* out_keys,
@@ -704,7 +743,8 @@ int hm_range_between(hm_tree* t, const uint64_t* lo, const uint64_t* hi, uint64_
   if (!lo || !hi || !out_keys || !out_vals || !out_counts) return set_err(HM_EINVAL, "NULL buffer");
   if (cap == 0) return set_err(HM_EINVAL, "cap must be > 0");
   if (n > kMaxBatch) return set_err(HM_EINVAL, "too many ranges");
-  return range_common(t, lo, hi, nullptr, cap, n, nullptr, out_keys, out_vals, out_counts, n * (uint64_t)cap, stream);
+  return range_common(t, lo, hi, nullptr, cap, n, nullptr, out_keys, out_vals, out_counts, n * (uint64_t)cap,
+                      (hipStream_t)stream, false);
 }
 
 // Device scratch of one publish over n staged updates (carved from t->upd_buf).
@@ -748,7 +788,10 @@ static UpdBufs carve_upd(void* base, size_t n) {
 
 // Grow-only: a publish in steady state allocates nothing (hipFree would synchronize the device).
 static int ensure_upd(hm_tree* t, size_t n, hipStream_t s) {
-  if (!t->upd_host) HM_TRY(hipHostMalloc((void**)&t->upd_host, 256, hipHostMallocDefault));
+  if (!t->upd_host) {
+    int rc = host_alloc_mapped((void**)&t->upd_host, 256);
+    if (rc) return rc;
+  }
   const size_t need = upd_need(std::max<size_t>(n, 1));
   if (t->upd_bytes >= need) return HM_OK;
   const size_t want = std::max(need, std::min<size_t>(2 * t->upd_bytes, upd_need(kMaxBatch)));
@@ -768,6 +811,21 @@ static int ensure_upd(hm_tree* t, size_t n, hipStream_t s) {
 }  // extern "C"
 
 namespace hm {
+// Pinned bounce buffer for host update batches (keys and values of up to cap staged updates).
+static int ensure_pin(hm_tree* t, size_t cap) {
+  if (t->pin_cap >= cap) return HM_OK;
+  if (t->pin_stage) {
+    if (t->stage_valid) HM_TRY(hipEventSynchronize(t->stage_ev));  // its last copies are done
+    HM_TRY(hipHostFree(t->pin_stage));
+    t->pin_stage = nullptr;
+    t->pin_cap = 0;
+  }
+  int rc = host_alloc_mapped((void**)&t->pin_stage, cap * 16);
+  if (rc) return rc;
+  t->pin_cap = cap;
+  return HM_OK;
+}
+
 // Stage an upsert batch; the caller holds t->pub_mu.
 int stage_updates(hm_tree* t, const uint64_t* keys, const uint64_t* vals, uint64_t n, hipStream_t s) {
   if (n == 0) return HM_OK;
@@ -788,8 +846,9 @@ int stage_updates(hm_tree* t, const uint64_t* keys, const uint64_t* vals, uint64
     HM_TRY(hipMalloc(&nk, cap * 8));
     HM_TRY(hipMalloc(&nv, cap * 8));
     if (t->pend_n) {
-      HM_TRY(hipMemcpyAsync(nk, t->pend_k, t->pend_n * 8, hipMemcpyDeviceToDevice, s));
-      HM_TRY(hipMemcpyAsync(nv, t->pend_v, t->pend_n * 8, hipMemcpyDeviceToDevice, s));
+      if (t->stage_valid) HM_TRY(hipStreamWaitEvent(s, t->stage_ev, 0));  // staged on other streams
+      HM_TRY(copy_words(nk, t->pend_k, t->pend_n, s));
+      HM_TRY(copy_words(nv, t->pend_v, t->pend_n, s));
     }
     HM_TRY(hipStreamSynchronize(s));
     if (t->pend_k) HM_TRY(hipFree(t->pend_k));
@@ -798,15 +857,36 @@ int stage_updates(hm_tree* t, const uint64_t* keys, const uint64_t* vals, uint64
     t->pend_v = nv;
     t->pend_cap = cap;
   }
-  HM_TRY(hipMemcpyAsync(t->pend_k + t->pend_n, keys, n * 8, dk ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
-  HM_TRY(hipMemcpyAsync(t->pend_v + t->pend_n, vals, n * 8, dv ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+  if (!t->stage_ev) HM_TRY(hipEventCreateWithFlags(&t->stage_ev, hipEventDisableTiming));
+  if (!dk || !dv) {
+    // host sources go through mapped pinned memory that a copy kernel reads: no DMA-engine copy
+    // (those serialize across every stream of the process) and no pageable copy that blocks.
+    // Slots [pend_n, pend_n + n) are free: a publish finishes every staging copy before it resets
+    // pend_n.
+    int rc = ensure_pin(t, t->pend_cap);
+    if (rc) return rc;
+    if (!dk) {
+      std::memcpy(t->pin_stage + t->pend_n, keys, n * 8);
+      keys = t->pin_stage + t->pend_n;
+    }
+    if (!dv) {
+      std::memcpy(t->pin_stage + t->pin_cap + t->pend_n, vals, n * 8);
+      vals = t->pin_stage + t->pin_cap + t->pend_n;
+    }
+  }
+  // chain the staging streams, so stage_ev covers every batch staged so far
+  if (t->stage_valid) HM_TRY(hipStreamWaitEvent(s, t->stage_ev, 0));
+  HM_TRY(copy_words(t->pend_k + t->pend_n, keys, n, s));
+  HM_TRY(copy_words(t->pend_v + t->pend_n, vals, n, s));
+  HM_TRY(hipEventRecord(t->stage_ev, s));
+  t->stage_valid = true;
   if (dk) {  // device keys: check on the device (validate_sorted sets bit 0 for KEY_MAX; bit 1 is ignored)
     int rc = ensure_upd(t, t->pend_n + n, s);
     if (rc) return rc;
     UpdBufs b = carve_upd(t->upd_buf, t->pend_n + n);
     HM_TRY(hipMemsetAsync(b.summary + 3, 0, 4, s));
     HM_TRY(validate_sorted(t->pend_k + t->pend_n, n, b.summary + 3, s));
-    HM_TRY(hipMemcpyAsync(t->upd_host + 3, b.summary + 3, 4, hipMemcpyDeviceToHost, s));
+    HM_TRY(copy_u32s(t->upd_host + 3, b.summary + 3, 1, s));
     HM_TRY(hipStreamSynchronize(s));
     if (t->upd_host[3] & 1u) return set_err(HM_EINVAL, "key 0xFFFFFFFFFFFFFFFF is reserved for padding");
   }
@@ -847,6 +927,7 @@ int publish_prepare(hm_tree* t, hipStream_t s, PublishPlan* plan) {
   int rc = ensure_upd(t, n, s);
   if (rc) return rc;
   if (t->publish_valid) HM_TRY(hipStreamWaitEvent(s, t->publish_ev, 0));  // the previous epoch's writes
+  if (t->stage_valid) HM_TRY(hipStreamWaitEvent(s, t->stage_ev, 0));       // staging on other streams
   const UpdBufs b = carve_upd(t->upd_buf, n);
   uint32_t* dm = b.summary + 2;
   HM_TRY(hipMemsetAsync(b.summary, 0, 16, s));
@@ -866,7 +947,7 @@ int publish_prepare(hm_tree* t, hipStream_t s, PublishPlan* plan) {
   const TreeView tv = t->view();
   HM_TRY(launch_locate(tv, b.uk, n, b.leaf, b.slot, b.found, s, dm));
   HM_TRY(plan_runs(t->d_leaves, b.leaf, b.found, n, dm, b.summary, s));
-  HM_TRY(hipMemcpyAsync(t->upd_host, b.summary, 16, hipMemcpyDeviceToHost, s));
+  HM_TRY(copy_u32s(t->upd_host, b.summary, 4, s));
   HM_TRY(hipStreamSynchronize(s));
   plan->n_new = t->upd_host[1];
   plan->m = t->upd_host[2];
@@ -891,8 +972,8 @@ int publish_prepare(hm_tree* t, hipStream_t s, PublishPlan* plan) {
   HM_TRY(leaf_first_keys(b.leaf, plan->m, first, s));
   HM_TRY(new_leaf_counts(t->d_leaves, ins, nl, t->fill, cnt, s));
   HM_TRY(exclusive_scan_u32(cnt, nstart, nl, 0u, ntmp, s));
-  HM_TRY(hipMemcpyAsync(t->upd_host + 4, nstart + nl - 1, 4, hipMemcpyDeviceToHost, s));
-  HM_TRY(hipMemcpyAsync(t->upd_host + 5, cnt + nl - 1, 4, hipMemcpyDeviceToHost, s));
+  HM_TRY(copy_u32s(t->upd_host + 4, nstart + nl - 1, 1, s));
+  HM_TRY(copy_u32s(t->upd_host + 5, cnt + nl - 1, 1, s));
   HM_TRY(hipStreamSynchronize(s));
   const uint64_t new_leaves = (uint64_t)t->upd_host[4] + t->upd_host[5];
   if (new_leaves >= (1ull << 32)) return set_err(HM_EINVAL, "leaf count would exceed 2^32");
@@ -912,7 +993,7 @@ int publish_prepare(hm_tree* t, hipStream_t s, PublishPlan* plan) {
   if (ib_need > t->rl_bytes) {  // keep the counts: grow into a new buffer and copy them over
     void* nb = nullptr;
     HM_TRY(hipMalloc(&nb, ib_need));
-    HM_TRY(hipMemcpyAsync(nb, t->rl_buf, ib_off, hipMemcpyDeviceToDevice, s));
+    HM_TRY(copy_words((uint64_t*)nb, (const uint64_t*)t->rl_buf, ib_off / 8, s));
     HM_TRY(hipStreamSynchronize(s));
     HM_TRY(hipFree(t->rl_buf));
     t->rl_buf = nb;
@@ -956,6 +1037,31 @@ int publish_prepare(hm_tree* t, hipStream_t s, PublishPlan* plan) {
   plan->new_leaves = new_leaves;
   plan->lv = lv;
   return HM_OK;
+}
+
+// Size the update staging and publish scratch for batches of up to n keys now (grow-only).
+int reserve_updates(hm_tree* t, size_t n, hipStream_t s) {
+  std::lock_guard<std::mutex> pub(t->pub_mu);
+  int rc = ensure_upd(t, n, s);
+  if (rc) return rc;
+  if (t->pend_cap < n) {
+    if (t->p.mem_budget && t->device_bytes() + (n - t->pend_cap) * 16 > t->p.mem_budget)
+      return set_err(HM_EBUDGET, "update staging exceeds the device-memory budget");
+    uint64_t *nk = nullptr, *nv = nullptr;
+    HM_TRY(hipMalloc(&nk, n * 8));
+    HM_TRY(hipMalloc(&nv, n * 8));
+    if (t->pend_n) {
+      HM_TRY(hipMemcpyAsync(nk, t->pend_k, t->pend_n * 8, hipMemcpyDeviceToDevice, s));
+      HM_TRY(hipMemcpyAsync(nv, t->pend_v, t->pend_n * 8, hipMemcpyDeviceToDevice, s));
+    }
+    HM_TRY(hipStreamSynchronize(s));
+    if (t->pend_k) HM_TRY(hipFree(t->pend_k));
+    if (t->pend_v) HM_TRY(hipFree(t->pend_v));
+    t->pend_k = nk;
+    t->pend_v = nv;
+    t->pend_cap = n;
+  }
+  return ensure_pin(t, t->pend_cap);
 }
 
 // Allocate the standby layout and re-layout scratch ahead of time (a serving loop does this at

@@ -66,13 +66,13 @@ struct hm_batcher {
   uint64_t fill = 0;
   // batches added since the last flush
   std::vector<GatherItem> items;     // device batches, split into pieces of at most kGatherPiece
-  uint64_t* h_stage = nullptr;       // pinned copy of host batches (capacity u64, allocated on first use)
+  uint64_t* h_stage = nullptr;       // mapped pinned copy of host batches (capacity u64, on first use)
   uint64_t host_lo = ~0ull, host_hi = 0;
   hipEvent_t stage_ev = nullptr;     // the last upload of h_stage (host adds wait for it)
   bool stage_busy = false;
-  // gather lists: two pinned upload buffers used alternately, each reused once its upload is done
+  // gather lists: two mapped pinned buffers that the gather kernel reads in place, used alternately,
+  // each reused once its gather is done
   GatherItem* h_items[2] = {nullptr, nullptr};
-  GatherItem* d_items[2] = {nullptr, nullptr};
   size_t items_cap[2] = {0, 0};
   hipEvent_t items_ev[2] = {nullptr, nullptr};
   bool items_busy[2] = {false, false};
@@ -86,8 +86,7 @@ void free_batcher(hm_batcher* b) {
   if (b->s) (void)hipStreamSynchronize(b->s);
   b->g_psa.reset();
   b->g_direct.reset();
-  for (void* p : {(void*)b->d_q, (void*)b->d_vals, (void*)b->d_found, (void*)b->d_n, b->scratch,
-                  (void*)b->d_items[0], (void*)b->d_items[1]})
+  for (void* p : {(void*)b->d_q, (void*)b->d_vals, (void*)b->d_found, (void*)b->d_n, b->scratch})
     if (p) (void)hipFree(p);
   for (void* p : {(void*)b->h_stage, (void*)b->h_items[0], (void*)b->h_items[1]})
     if (p) (void)hipHostFree(p);
@@ -185,7 +184,8 @@ int hm_batcher_add(hm_batcher* b, const uint64_t* q, uint64_t n, uint64_t* out_o
       b->items.push_back(GatherItem{q + o, b->fill + o, std::min<uint64_t>(kGatherPiece, n - o)});
   } else {
     if (!b->h_stage) {
-      HM_TRY(hipHostMalloc((void**)&b->h_stage, b->cap * 8, hipHostMallocDefault));
+      int rc = host_alloc_mapped((void**)&b->h_stage, b->cap * 8);
+      if (rc) return rc;
       HM_TRY(hipEventCreateWithFlags(&b->stage_ev, hipEventDisableTiming));
     }
     if (b->stage_busy) {  // the previous flush may still be reading the staging buffer
@@ -205,8 +205,8 @@ namespace {
 // Pack the added batches into d_q (stream order on b->s).
 int pack(hm_batcher* b) {
   if (b->host_hi > b->host_lo) {  // host batches first: device batches inside the range are re-gathered
-    HM_TRY(hipMemcpyAsync(b->d_q + b->host_lo, b->h_stage + b->host_lo, (b->host_hi - b->host_lo) * 8,
-                          hipMemcpyHostToDevice, b->s));
+    // a copy kernel reads the mapped staging buffer: no DMA-engine copy in the flush
+    HM_TRY(copy_words(b->d_q + b->host_lo, b->h_stage + b->host_lo, b->host_hi - b->host_lo, b->s));
     HM_TRY(hipEventRecord(b->stage_ev, b->s));
     b->stage_busy = true;
   }
@@ -214,25 +214,22 @@ int pack(hm_batcher* b) {
   if (m) {
     const int k = b->slot;
     b->slot ^= 1;
-    if (b->items_busy[k]) {  // this upload buffer was used two flushes ago
+    if (b->items_busy[k]) {  // this list was used two flushes ago
       HM_TRY(hipEventSynchronize(b->items_ev[k]));
       b->items_busy[k] = false;
     }
     if (m > b->items_cap[k]) {
-      if (b->h_items[k]) HM_TRY(hipHostFree(b->h_items[k]));
-      if (b->d_items[k]) HM_TRY(hipFree(b->d_items[k]));  // its last use has completed (event above)
+      if (b->h_items[k]) HM_TRY(hipHostFree(b->h_items[k]));  // its last use has completed (event above)
       b->h_items[k] = nullptr;
-      b->d_items[k] = nullptr;
       b->items_cap[k] = 0;
       const size_t c = std::max<size_t>(2 * m, 1024);
-      HM_TRY(hipHostMalloc((void**)&b->h_items[k], c * sizeof(GatherItem), hipHostMallocDefault));
-      HM_TRY(hipMalloc(&b->d_items[k], c * sizeof(GatherItem)));
+      int rc = host_alloc_mapped((void**)&b->h_items[k], c * sizeof(GatherItem));
+      if (rc) return rc;
       if (!b->items_ev[k]) HM_TRY(hipEventCreateWithFlags(&b->items_ev[k], hipEventDisableTiming));
       b->items_cap[k] = c;
     }
     std::memcpy(b->h_items[k], b->items.data(), m * sizeof(GatherItem));
-    HM_TRY(hipMemcpyAsync(b->d_items[k], b->h_items[k], m * sizeof(GatherItem), hipMemcpyHostToDevice, b->s));
-    hipLaunchKernelGGL(gather_kernel, dim3((unsigned)m), dim3(256), 0, b->s, b->d_items[k], b->d_q);
+    hipLaunchKernelGGL(gather_kernel, dim3((unsigned)m), dim3(256), 0, b->s, (const GatherItem*)b->h_items[k], b->d_q);
     HM_TRY(hipGetLastError());
     HM_TRY(hipEventRecord(b->items_ev[k], b->s));
     b->items_busy[k] = true;

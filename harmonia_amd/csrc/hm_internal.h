@@ -127,6 +127,10 @@ hipError_t launch_search(const TreeView& tv, const SearchLaunch& sl, const uint6
 size_t lds_bytes_for(const TreeView& tv, int levels);
 // *p = v as a one-thread kernel (a graph-safe way to hand a count to the next replay).
 hipError_t launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s);
+// dst[i] = src[i] for n u64 words as a kernel: either side may be device memory or mapped pinned
+// host memory, so moving small buffers between host and device takes no DMA-engine copy.
+hipError_t copy_words(uint64_t* dst, const uint64_t* src, size_t n, hipStream_t s);
+hipError_t copy_u32s(uint32_t* dst, const uint32_t* src, size_t n, hipStream_t s);
 // Locate (leaf, slot, found) per key (update path); n sizes the grid, dn (device) may hold the
 // actual count.
 hipError_t launch_locate(const TreeView& tv, const uint64_t* keys, size_t n, uint32_t* leaf_out, uint32_t* slot_out,

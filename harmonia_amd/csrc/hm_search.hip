@@ -334,4 +334,28 @@ hipError_t launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s) {
   hipLaunchKernelGGL(set_u32_kernel, dim3(1), dim3(64), 0, s, p, v);
   return hipGetLastError();
 }
+
+namespace {
+template <typename T>
+__global__ void copy_elems_kernel(T* __restrict__ dst, const T* __restrict__ src, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+template <typename T>
+hipError_t copy_elems(T* dst, const T* src, size_t n, hipStream_t s) {
+  if (!n) return hipSuccess;
+  size_t nb = (n + 255) / 256;
+  if (nb > 1024) nb = 1024;
+  hipLaunchKernelGGL(copy_elems_kernel<T>, dim3((unsigned)nb), dim3(256), 0, s, dst, src, n);
+  return hipGetLastError();
+}
+}  // namespace
+
+hipError_t copy_words(uint64_t* dst, const uint64_t* src, size_t n, hipStream_t s) {
+  return copy_elems(dst, src, n, s);
+}
+hipError_t copy_u32s(uint32_t* dst, const uint32_t* src, size_t n, hipStream_t s) {
+  return copy_elems(dst, src, n, s);
+}
 }  // namespace hm

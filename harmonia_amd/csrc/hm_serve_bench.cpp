@@ -169,12 +169,18 @@ extern "C" int hm_serve_bench_main(int argc, char** argv, char* out_json, uint64
   hm_server_params sp{budget, 1u << 20, plain ? 1u : inflight, plain ? 0u : 1u, 0};
   hm_server* sv = nullptr;
   CHECK(hm_server_create(t, &sp, &sv));
-  // warm-up outside the measurement
+  // warm-up outside the measurement: every request kind once (first launches load the kernels)
   {
-    std::vector<uint64_t> wv(4096);
+    std::vector<uint64_t> wv(4096), wk(64), wv2(64);
     std::vector<uint8_t> wf(4096);
+    uint32_t wlen = 64, wc = 0;
     uint64_t tk;
     CHECK(hm_server_lookup(sv, keys.data(), 4096, wv.data(), wf.data(), &tk));
+    CHECK(hm_server_wait(sv, tk, nullptr, nullptr));
+    CHECK(hm_server_range(sv, keys.data(), &wlen, 1, wk.data(), wv2.data(), &wc, 64, &tk));
+    CHECK(hm_server_wait(sv, tk, nullptr, nullptr));
+    const uint64_t uk = perm63(base + n + M + (1ull << 41) + 1, salt), uv = value_of(uk);
+    CHECK(hm_server_update(sv, &uk, &uv, 1, &tk));
     CHECK(hm_server_wait(sv, tk, nullptr, nullptr));
     CHECK(hm_server_reset_stats(sv));
   }

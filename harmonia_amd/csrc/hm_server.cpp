@@ -12,14 +12,16 @@
 // way; a request submitted after another one completed always sees its effect.  The update worker
 // stages the batch and prepares the epoch while read batches keep running on the current one (a
 // re-layout is built in standby buffers); only the commit -- an enqueue of the in-place merge after
-// every earlier search, or a pointer swap -- holds the epoch lock exclusively.  A read batch holds
-// that lock shared until its device work is enqueued, so every read sees one whole epoch, which is
-// reported with its result.
+// every earlier search, or a pointer swap -- holds the epoch lock exclusively.  A read batch packs
+// its inputs first and then holds that lock shared only while its device work is enqueued, so every
+// read sees one whole epoch, which is reported with its result.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <memory>
@@ -76,7 +78,8 @@ struct Buf {
     release();  // grows geometrically: a hipFree synchronizes the whole device
     size_t want = std::max(std::max(b, 2 * old), (size_t)1 << 20);
     if (pinned) {
-      HM_TRY(hipHostMalloc(&p, want, hipHostMallocDefault));
+      const int rc = host_alloc_mapped(&p, want);
+      if (rc) return rc;
     } else {
       HM_TRY(hipMalloc(&p, want));
     }
@@ -92,9 +95,11 @@ struct Buf {
 
 struct Worker {
   hipStream_t s = nullptr;
+  // mapped pinned buffers: the search and range kernels read the packed inputs and write the
+  // results here directly, so a batch issues no copy commands at all
   Buf hq{nullptr, 0, true}, hv{nullptr, 0, true}, hf{nullptr, 0, true}, hl{nullptr, 0, true},
       hc{nullptr, 0, true}, hk{nullptr, 0, true};
-  Buf dq, dv, df, dl, dc, dk, doff;
+  Buf doff;  // range offsets stay on the device
   std::thread th;
 };
 
@@ -164,33 +169,40 @@ uint64_t tree_epoch(hm_tree* t) {
 
 // Run one coalesced read batch on worker w.  Called with the epoch lock held shared until the
 // device work is enqueued.
+// HM_SERVER_TRACE=1 in the environment: report every worker step slower than 2 ms on stderr.
+bool trace_on() {
+  static const bool on = [] {
+    const char* e = std::getenv("HM_SERVER_TRACE");
+    return e && *e && *e != '0';
+  }();
+  return on;
+}
+void trace_step(const char* what, Clock::time_point a, Clock::time_point b, uint64_t size) {
+  const double us = std::chrono::duration<double, std::micro>(b - a).count();
+  if (us > 2000.0)
+    std::fprintf(stderr, "[hm_server] %s took %.0f us (size %llu)\n", what, us, (unsigned long long)size);
+}
+
 int run_reads(hm_server* sv, Worker& w, std::vector<ReqPtr>& lookups, std::vector<ReqPtr>& ranges,
               std::shared_lock<std::shared_mutex>& epoch_lock, uint64_t* epoch) {
   hm_tree* t = sv->t;
-  *epoch = tree_epoch(t);
   uint64_t nq = 0;
   for (auto& r : lookups) nq += r->n;
   uint64_t nr = 0, rtot = 0;
   for (auto& r : ranges) nr += r->n, rtot += r->range_total;
   int rc = HM_OK;
+  // pack the inputs into pinned staging first: host copies do not hold the epoch lock
   if (nq) {
-    if ((rc = w.hq.ensure(nq * 8)) || (rc = w.hv.ensure(nq * 8)) || (rc = w.hf.ensure(nq)) ||
-        (rc = w.dq.ensure(nq * 8)) || (rc = w.dv.ensure(nq * 8)) || (rc = w.df.ensure(nq)))
-      return rc;
+    if ((rc = w.hq.ensure(nq * 8)) || (rc = w.hv.ensure(nq * 8)) || (rc = w.hf.ensure(nq))) return rc;
     uint64_t o = 0;
     for (auto& r : lookups) {
       std::memcpy((uint64_t*)w.hq.p + o, r->keys, (size_t)r->n * 8);
       o += r->n;
     }
-    HM_TRY(hipMemcpyAsync(w.dq.p, w.hq.p, nq * 8, hipMemcpyHostToDevice, w.s));
-    if ((rc = hm_search(t, (const uint64_t*)w.dq.p, nq, (uint64_t*)w.dv.p, (uint8_t*)w.df.p, w.s))) return rc;
-    HM_TRY(hipMemcpyAsync(w.hv.p, w.dv.p, nq * 8, hipMemcpyDeviceToHost, w.s));
-    HM_TRY(hipMemcpyAsync(w.hf.p, w.df.p, nq, hipMemcpyDeviceToHost, w.s));
   }
   if (nr) {
     if ((rc = w.hl.ensure(nr * 12)) || (rc = w.hc.ensure(nr * 4)) || (rc = w.hk.ensure(std::max<uint64_t>(rtot, 1) * 16)) ||
-        (rc = w.dl.ensure(nr * 12)) || (rc = w.dc.ensure(nr * 4)) || (rc = w.doff.ensure((nr + 1) * 8)) ||
-        (rc = w.dk.ensure(std::max<uint64_t>(rtot, 1) * 16)))
+        (rc = w.doff.ensure((nr + 1) * 8)))
       return rc;
     uint64_t* hs = (uint64_t*)w.hl.p;
     uint32_t* hlen = (uint32_t*)((char*)w.hl.p + nr * 8);
@@ -200,18 +212,33 @@ int run_reads(hm_server* sv, Worker& w, std::vector<ReqPtr>& lookups, std::vecto
       std::memcpy(hlen + o, r->lens, (size_t)r->n * 4);
       o += r->n;
     }
-    HM_TRY(hipMemcpyAsync(w.dl.p, w.hl.p, nr * 12, hipMemcpyHostToDevice, w.s));
-    uint64_t* dkeys = (uint64_t*)w.dk.p;
-    uint64_t* dvals = dkeys + std::max<uint64_t>(rtot, 1);
-    if ((rc = hm_range(t, (const uint64_t*)w.dl.p, (const uint32_t*)((char*)w.dl.p + nr * 8), nr,
-                       (uint64_t*)w.doff.p, dkeys, dvals, (uint32_t*)w.dc.p, rtot, w.s)))
+  }
+  // then enqueue the device work under the shared epoch lock: the batch sees one whole epoch
+  epoch_lock.lock();
+  const auto t_enq = Clock::now();
+  *epoch = tree_epoch(t);
+  if (nq) {
+    const auto a0 = Clock::now();
+    if ((rc = search_impl(t, (const uint64_t*)w.hq.p, nq, (uint64_t*)w.hv.p, (uint8_t*)w.hf.p, nullptr, w.s, true)))
       return rc;
-    HM_TRY(hipMemcpyAsync(w.hk.p, dkeys, rtot * 8, hipMemcpyDeviceToHost, w.s));
-    HM_TRY(hipMemcpyAsync((uint64_t*)w.hk.p + rtot, dvals, rtot * 8, hipMemcpyDeviceToHost, w.s));
-    HM_TRY(hipMemcpyAsync(w.hc.p, w.dc.p, nr * 4, hipMemcpyDeviceToHost, w.s));
+    if (trace_on()) trace_step("  search enqueue", a0, Clock::now(), nq);
+  }
+  if (nr) {
+    const auto a0 = Clock::now();
+    uint64_t* hkeys = (uint64_t*)w.hk.p;
+    uint64_t* hvals = hkeys + rtot;
+    if ((rc = range_impl(t, (const uint64_t*)w.hl.p, (const uint32_t*)((char*)w.hl.p + nr * 8), nr, (uint64_t*)w.doff.p,
+                         hkeys, hvals, (uint32_t*)w.hc.p, rtot, w.s, true)))
+      return rc;
+    if (trace_on()) trace_step("  range enqueue", a0, Clock::now(), nr);
   }
   epoch_lock.unlock();  // the device work is ordered against later publishes from here on
+  const auto t_sync = Clock::now();
   HM_TRY(hipStreamSynchronize(w.s));
+  if (trace_on()) {
+    trace_step("read enqueue (epoch lock held)", t_enq, t_sync, nq + rtot);
+    trace_step("read device work", t_sync, Clock::now(), nq + rtot);
+  }
   // hand results back
   uint64_t o = 0;
   for (auto& r : lookups) {
@@ -241,17 +268,27 @@ void run_updates(hm_server* sv, Worker& w, std::vector<ReqPtr>& ups, std::vector
   hm_tree* t = sv->t;
   status.assign(ups.size(), HM_OK);
   std::lock_guard<std::mutex> pub(t->pub_mu);
-  for (size_t i = 0; i < ups.size(); ++i) status[i] = stage_updates(t, ups[i]->keys, ups[i]->vals, ups[i]->n, w.s);
+  const auto ts = Clock::now();
+  uint64_t staged = 0;
+  for (size_t i = 0; i < ups.size(); ++i) {
+    status[i] = stage_updates(t, ups[i]->keys, ups[i]->vals, ups[i]->n, w.s);
+    staged += ups[i]->n;
+  }
   PublishPlan plan;
   const auto t0 = Clock::now();
   int rc = publish_prepare(t, w.s, &plan);
   const auto t1 = Clock::now();
+  if (trace_on()) {
+    trace_step("update staging", ts, t0, staged);
+    trace_step(plan.relayout ? "publish prepare (re-layout)" : "publish prepare", t0, t1, staged);
+  }
   double wait_us = 0;
   if (!rc) {
     // no new read batch may start while the commit waits for the ones still enqueueing
     std::unique_lock<std::mutex> lk(sv->mu);
     std::unique_lock<std::shared_mutex> ex(sv->epoch_mu);
     wait_us = std::chrono::duration<double, std::micro>(Clock::now() - t1).count();
+    if (trace_on()) trace_step("commit wait for the epoch lock", t1, Clock::now(), staged);
     sv->max_prepare_us = std::max(sv->max_prepare_us, std::chrono::duration<double, std::micro>(t1 - t0).count());
     sv->max_commit_wait_us = std::max(sv->max_commit_wait_us, wait_us);
     lk.unlock();
@@ -267,11 +304,11 @@ void run_updates(hm_server* sv, Worker& w, std::vector<ReqPtr>& ups, std::vector
 
 int reserve_worker(hm_server* sv, Worker& w) {
   const uint64_t b = sv->p.max_batch;
-  for (Buf* x : {&w.hq, &w.hv, &w.dq, &w.dv}) HM_TRY_RC(x->ensure(b * 8));
-  for (Buf* x : {&w.hf, &w.df}) HM_TRY_RC(x->ensure(b));
-  for (Buf* x : {&w.hl, &w.dl}) HM_TRY_RC(x->ensure(b * 12));
-  for (Buf* x : {&w.hc, &w.dc}) HM_TRY_RC(x->ensure(b * 4));
-  for (Buf* x : {&w.hk, &w.dk}) HM_TRY_RC(x->ensure(b * 16));
+  for (Buf* x : {&w.hq, &w.hv}) HM_TRY_RC(x->ensure(b * 8));
+  HM_TRY_RC(w.hf.ensure(b));
+  HM_TRY_RC(w.hl.ensure(b * 12));
+  HM_TRY_RC(w.hc.ensure(b * 4));
+  HM_TRY_RC(w.hk.ensure(b * 16));
   HM_TRY_RC(w.doff.ensure((b + 1) * 8));
   // the tree's workspace for this stream: the largest search pipeline or range scan of a batch
   const size_t ws = std::max(pipe_bytes(b, 32, HM_OUT_UNPERMUTE), align256(b * 8) + align256(scan_tmp_bytes(b + 1)));
@@ -316,7 +353,6 @@ void worker_loop(hm_server* sv, Worker* w) {
           (h->type == REQ_LOOKUP ? lookups : ranges).push_back(h);
           bytes = request_bytes(*h);
           sv->inflight_bytes += bytes;
-          epoch_lock.lock();
         }
       } else if (update_ready()) {
         // every pending update, in submission order, as one epoch
@@ -348,7 +384,6 @@ void worker_loop(hm_server* sv, Worker* w) {
         }
         sv->pending.swap(rest);
         sv->inflight_bytes += bytes;
-        epoch_lock.lock();  // before mu is released: an update taken later waits for these reads
       }
       ++sv->n_launches;
       if (!sv->pending.empty()) sv->cv_work.notify_one();
@@ -430,6 +465,8 @@ int hm_server_create(hm_tree* t, const hm_server_params* p, hm_server** out) {
   }
   // a re-layout while serving builds into standby buffers: allocate them now if the budget allows
   (void)reserve_relayout(t, sv->workers[0]->s);
+  // and size the update path for coalesced epochs (a grow frees device memory: a device-wide sync)
+  (void)reserve_updates(t, std::max<size_t>(sv->p.max_batch, 1u << 17), sv->workers[0]->s);
   // size every worker's buffers for a full batch now: a grow while serving would allocate pinned
   // memory (slow) and free device memory (synchronizes the device), stalling every request
   for (auto& w : sv->workers) {
@@ -556,8 +593,7 @@ int hm_server_destroy(hm_server* s) {
     if (w->th.joinable()) w->th.join();
   for (auto& w : s->workers) {
     (void)hipStreamSynchronize(w->s);
-    for (Buf* b : {&w->hq, &w->hv, &w->hf, &w->hl, &w->hc, &w->hk, &w->dq, &w->dv, &w->df, &w->dl, &w->dc, &w->dk,
-                   &w->doff})
+    for (Buf* b : {&w->hq, &w->hv, &w->hf, &w->hl, &w->hc, &w->hk, &w->doff})
       b->release();
     (void)hipStreamDestroy(w->s);
   }

@@ -8,13 +8,10 @@
 // order, so the result equals a stable sort of (key, pos) by key >> begin_bit.  The same kernels
 // sort the bulk-build input (u64 value payload over every varying bit).
 //
-// One pass = histogram kernel (per-tile digit counts, stored tile-major: one 1-KiB row per tile)
-// -> exclusive scan in (digit, tile) order over that layout (column sums per chunk of tiles, one
-// block scanning the chunk sums and digit totals, then the per-tile rows) -> scatter kernel
-// (wave-level digit match via 8 ballots gives each item its stable rank inside its wave; per-wave
-// counters are scanned across the block; the tile is then staged in LDS in digit order so the
-// global writes of each digit run are coalesced).  Every kernel reads or writes its tile's 256
-// counters as one contiguous row instead of 256 strided words.  Tiles are
+// One pass = histogram kernel (per-tile digit counts, digit-major) -> exclusive scan over
+// (digit, tile) -> scatter kernel (wave-level digit match via 8 ballots gives each item its
+// stable rank inside its wave; per-wave counters are scanned across the block; the tile is then
+// staged in LDS in digit order so the global writes of each digit run are coalesced).  Tiles are
 // mapped to blocks so that each XCD processes a contiguous range of tiles: the digit runs of
 // neighbouring tiles share cache lines at their ends, and those partial lines then meet in one L2.
 //
@@ -87,48 +84,7 @@ __global__ __launch_bounds__(kThreads) void radix_hist_kernel(const K* __restric
   uint32_t c = 0;
 #pragma unroll
   for (int ww = 0; ww < kWaves; ++ww) c += hist[ww][threadIdx.x];
-  counts[(size_t)tile * 256 + threadIdx.x] = c;
-}
-
-// Exclusive scan of tile-major counts in (digit, tile) order: offset(t, d) = sum of counts of
-// digits < d over all tiles + counts of digit d in tiles < t.
-constexpr uint32_t kScanChunk = 64;  // tiles per chunk
-uint32_t scan_chunks(size_t nb) { return (uint32_t)((nb + kScanChunk - 1) / kScanChunk); }
-
-__global__ __launch_bounds__(256) void tile_col_sums_kernel(const uint32_t* __restrict__ cnt, uint32_t nb,
-                                                            uint32_t* __restrict__ part) {
-  const uint32_t t0 = blockIdx.x * kScanChunk;
-  const uint32_t t1 = t0 + kScanChunk < nb ? t0 + kScanChunk : nb;
-  uint32_t sum = 0;
-  for (uint32_t t = t0; t < t1; ++t) sum += cnt[(size_t)t * 256 + threadIdx.x];
-  part[(size_t)blockIdx.x * 256 + threadIdx.x] = sum;
-}
-
-__global__ __launch_bounds__(256) void tile_chunk_scan_kernel(uint32_t* __restrict__ part, uint32_t nchunks) {
-  __shared__ uint32_t scan_sh[256 / 64 + 1];
-  const uint32_t d = threadIdx.x;
-  uint32_t run = 0;
-  for (uint32_t c = 0; c < nchunks; ++c) {
-    const uint32_t v = part[(size_t)c * 256 + d];
-    part[(size_t)c * 256 + d] = run;
-    run += v;
-  }
-  uint32_t total;
-  const uint32_t start = block_exclusive_scan<uint32_t, 256>(run, scan_sh, total);
-  for (uint32_t c = 0; c < nchunks; ++c) part[(size_t)c * 256 + d] += start;
-}
-
-__global__ __launch_bounds__(256) void tile_offsets_kernel(const uint32_t* __restrict__ cnt, uint32_t nb,
-                                                           const uint32_t* __restrict__ part,
-                                                           uint32_t* __restrict__ off) {
-  const uint32_t t0 = blockIdx.x * kScanChunk;
-  const uint32_t t1 = t0 + kScanChunk < nb ? t0 + kScanChunk : nb;
-  uint32_t run = part[(size_t)blockIdx.x * 256 + threadIdx.x];
-  for (uint32_t t = t0; t < t1; ++t) {
-    const size_t i = (size_t)t * 256 + threadIdx.x;
-    off[i] = run;
-    run += cnt[i];
-  }
+  counts[(size_t)threadIdx.x * nb + tile] = c;
 }
 
 // Write (key, value) to (kout, vout); without HAS_VAL keys only.  With SAVE_LP, lp_out[i] = the
@@ -150,7 +106,7 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(const K* __rest
   __shared__ K skeys[kTile];
   __shared__ V svals[HAS_VAL ? kTile : 1];
   for (int i = threadIdx.x; i < kWaves * 256; i += kThreads) (&wcnt[0][0])[i] = 0;
-  goff[threadIdx.x] = offsets[(size_t)tile * 256 + threadIdx.x];
+  goff[threadIdx.x] = offsets[(size_t)threadIdx.x * nb + tile];
   __syncthreads();
 
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -226,8 +182,7 @@ __global__ __launch_bounds__(kThreads) void radix_unscatter_kernel(const uint64_
                                                                    uint64_t* __restrict__ vout,
                                                                    uint8_t* __restrict__ fout,
                                                                    const uint16_t* __restrict__ lp, size_t n,
-                                                                   const uint32_t* __restrict__ offsets,
-                                                                   const uint32_t* __restrict__ counts, uint32_t nb,
+                                                                   const uint32_t* __restrict__ offsets, uint32_t nb,
                                                                    const uint32_t* __restrict__ dn) {
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   if (dn) n = *dn;
@@ -238,11 +193,15 @@ __global__ __launch_bounds__(kThreads) void radix_unscatter_kernel(const uint64_
   __shared__ uint64_t sv[kTile];
   __shared__ uint8_t sf[kTile];
   {
-    const size_t row = (size_t)tile * 256 + threadIdx.x;
-    goff[threadIdx.x] = offsets[row];
+    // this tile's count of digit d = next scanned offset - this one (digit-major, exclusive scan)
+    const uint32_t d = threadIdx.x;
+    const size_t idx = (size_t)d * nb + tile;
+    const uint32_t g = offsets[idx];
+    const uint32_t next = idx + 1 < (size_t)256 * nb ? offsets[idx + 1] : (uint32_t)n;
+    goff[d] = g;
     uint32_t total;
-    boff[threadIdx.x] = block_exclusive_scan<uint32_t, kThreads>(counts[row], scan_sh, total);
-    if (threadIdx.x == 0) boff[256] = total;
+    boff[d] = block_exclusive_scan<uint32_t, kThreads>(next - g, scan_sh, total);
+    if (d == 0) boff[256] = total;
   }
   __syncthreads();
   const size_t tile0 = (size_t)tile * kTile;
@@ -300,31 +259,24 @@ __global__ void copy_with_pos_kernel(const uint64_t* __restrict__ kin, const V* 
 }
 
 
-// One digit pass from (sk, sv) into (dk, dv).  cnt and off (256 * nb u32 each, tile-major) receive
-// the tile histograms and their scanned offsets; scan_tmp holds the chunk sums.
+// One digit pass from (sk, sv) into (dk, dv).  counts (256 * nb u32) receive the scanned
+// (digit, tile) offsets; scan_tmp holds scan_tmp_bytes(256 * nb).
 template <typename V, bool HAS_VAL, bool SAVE_LP>
 hipError_t radix_pass(const uint64_t* sk, const V* sv, uint64_t* dk, V* dv, uint16_t* lp, size_t n, const uint32_t* dn,
-                      int shift, int bits, bool gen, uint32_t* cnt, uint32_t* off, void* scan_tmp, hipStream_t s) {
+                      int shift, int bits, bool gen, uint32_t* counts, void* scan_tmp, hipStream_t s) {
   const size_t nb = (n + kTile - 1) / kTile;
   const uint32_t mask = (1u << bits) - 1u;
-  const uint32_t nc = scan_chunks(nb);
-  uint32_t* part = reinterpret_cast<uint32_t*>(scan_tmp);
   {
     ProfScope prof("psa_hist", s);
     hipLaunchKernelGGL((radix_hist_kernel<uint64_t>), dim3(grid_for(nb)), dim3(kThreads), 0, s, sk, n, shift, mask,
-                       cnt, (uint32_t)nb, dn);
+                       counts, (uint32_t)nb, dn);
   }
-  {
-    ProfScope prof("scan", s);
-    hipLaunchKernelGGL(tile_col_sums_kernel, dim3(nc), dim3(256), 0, s, (const uint32_t*)cnt, (uint32_t)nb, part);
-    hipLaunchKernelGGL(tile_chunk_scan_kernel, dim3(1), dim3(256), 0, s, part, nc);
-    hipLaunchKernelGGL(tile_offsets_kernel, dim3(nc), dim3(256), 0, s, (const uint32_t*)cnt, (uint32_t)nb,
-                       (const uint32_t*)part, off);
-  }
+  hipError_t e = exclusive_scan_u32(counts, counts, 256 * nb, 0u, scan_tmp, s);
+  if (e != hipSuccess) return e;
   {
     ProfScope prof("psa_scatter", s);
     hipLaunchKernelGGL((radix_scatter_kernel<uint64_t, V, HAS_VAL, SAVE_LP>), dim3(grid_for(nb)), dim3(kThreads), 0, s,
-                       sk, sv, dk, dv, lp, n, shift, mask, (const uint32_t*)off, (uint32_t)nb, gen ? 1 : 0, dn);
+                       sk, sv, dk, dv, lp, n, shift, mask, (const uint32_t*)counts, (uint32_t)nb, gen ? 1 : 0, dn);
   }
   return hipGetLastError();
 }
@@ -343,9 +295,8 @@ hipError_t radix_sort_impl(const uint64_t* keys_in, const V* vals_in, size_t n, 
                        gen_pos ? 1 : 0);
     return hipGetLastError();
   }
-  uint32_t* cnt = reinterpret_cast<uint32_t*>(tmp);
-  uint32_t* off = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(tmp) + counts_bytes(n));
-  void* scan_tmp = reinterpret_cast<char*>(tmp) + 2 * counts_bytes(n);
+  uint32_t* counts = reinterpret_cast<uint32_t*>(tmp);
+  void* scan_tmp = reinterpret_cast<char*>(tmp) + counts_bytes(n);
   const uint64_t* sk = keys_in;
   const V* sv = vals_in;
   int out = 0;
@@ -354,7 +305,7 @@ hipError_t radix_sort_impl(const uint64_t* keys_in, const V* vals_in, size_t n, 
     const int bits = end_bit - shift < 8 ? end_bit - shift : 8;
     uint64_t* dk = out == 0 ? k0 : k1;
     V* dv = out == 0 ? v0 : v1;
-    hipError_t e = radix_pass<V, true, false>(sk, sv, dk, dv, nullptr, n, dn, shift, bits, gen, cnt, off, scan_tmp, s);
+    hipError_t e = radix_pass<V, true, false>(sk, sv, dk, dv, nullptr, n, dn, shift, bits, gen, counts, scan_tmp, s);
     if (e != hipSuccess) return e;
     sk = dk;
     sv = dv;
@@ -369,9 +320,7 @@ int n_passes(int bits) { return (bits + 7) / 8; }
 
 }  // namespace
 
-size_t radix_tmp_bytes(size_t n) {
-  return 2 * counts_bytes(n) + align_up((size_t)scan_chunks((n + kTile - 1) / kTile + 1) * 256 * sizeof(uint32_t), 256);
-}
+size_t radix_tmp_bytes(size_t n) { return counts_bytes(n) + scan_tmp_bytes(256 * ((n + kTile - 1) / kTile + 1)) + 256; }
 
 hipError_t radix_sort_pos(const uint64_t* keys_in, size_t n, int begin_bit, int end_bit, uint64_t* k0, uint32_t* v0,
                           uint64_t* k1, uint32_t* v1, void* tmp, hipStream_t s, int* which, const uint32_t* dn) {
@@ -385,10 +334,10 @@ hipError_t radix_sort_u64pay(const uint64_t* keys_in, const uint64_t* vals_in, s
                                    nullptr);
 }
 
-// Replay record of pass p at replay + p * stride: tile counts, their scanned offsets, then the
-// staging indices.
-size_t replay_stride(size_t n) { return 2 * counts_bytes(n) + align_up(n * sizeof(uint16_t), 256); }
-size_t replay_bytes(size_t n, int bits) { return (size_t)n_passes(bits) * replay_stride(n); }
+// Replay record of pass p: counts at replay + p * stride, staging indices right after them.
+size_t replay_bytes(size_t n, int bits) {
+  return (size_t)n_passes(bits) * (counts_bytes(n) + align_up(n * sizeof(uint16_t), 256));
+}
 
 hipError_t radix_sort_keys_replay(const uint64_t* keys_in, size_t n, int begin_bit, int end_bit, uint64_t* k0,
                                   uint64_t* k1, void* replay, void* tmp, hipStream_t s, int* which,
@@ -396,16 +345,16 @@ hipError_t radix_sort_keys_replay(const uint64_t* keys_in, size_t n, int begin_b
   *which = 0;
   if (n == 0) return hipSuccess;
   if (begin_bit < 0 || end_bit > 64 || begin_bit >= end_bit) return hipErrorInvalidValue;
-  const size_t stride = replay_stride(n);
+  const size_t stride = counts_bytes(n) + align_up(n * sizeof(uint16_t), 256);
   const uint64_t* sk = keys_in;
   int out = 0, p = 0;
   for (int shift = begin_bit; shift < end_bit; shift += 8, ++p) {
     const int bits = end_bit - shift < 8 ? end_bit - shift : 8;
     char* rec = reinterpret_cast<char*>(replay) + (size_t)p * stride;
     uint64_t* dk = out == 0 ? k0 : k1;
-    hipError_t e = radix_pass<uint32_t, false, true>(
-        sk, nullptr, dk, nullptr, reinterpret_cast<uint16_t*>(rec + 2 * counts_bytes(n)), n, dn, shift, bits, false,
-        reinterpret_cast<uint32_t*>(rec), reinterpret_cast<uint32_t*>(rec + counts_bytes(n)), tmp, s);
+    hipError_t e = radix_pass<uint32_t, false, true>(sk, nullptr, dk, nullptr,
+                                                     reinterpret_cast<uint16_t*>(rec + counts_bytes(n)), n, dn, shift,
+                                                     bits, false, reinterpret_cast<uint32_t*>(rec), tmp, s);
     if (e != hipSuccess) return e;
     sk = dk;
     *which = out;
@@ -420,7 +369,7 @@ hipError_t unsort_results(uint64_t* vals, uint8_t* found, size_t n, int bits, co
   if (n == 0) return hipSuccess;
   const int P = n_passes(bits);
   const size_t nb = (n + kTile - 1) / kTile;
-  const size_t stride = replay_stride(n);
+  const size_t stride = counts_bytes(n) + align_up(n * sizeof(uint16_t), 256);
   const uint64_t* sv = vals;
   const uint8_t* sf = found;
   for (int p = P - 1; p >= 0; --p) {
@@ -436,9 +385,8 @@ hipError_t unsort_results(uint64_t* vals, uint8_t* found, size_t n, int bits, co
     }
     ProfScope prof("unsort", s);
     hipLaunchKernelGGL(radix_unscatter_kernel, dim3(grid_for(nb)), dim3(kThreads), 0, s, sv, sf, dv, df,
-                       reinterpret_cast<const uint16_t*>(rec + 2 * counts_bytes(n)), n,
-                       reinterpret_cast<const uint32_t*>(rec + counts_bytes(n)), reinterpret_cast<const uint32_t*>(rec),
-                       (uint32_t)nb, dn);
+                       reinterpret_cast<const uint16_t*>(rec + counts_bytes(n)), n,
+                       reinterpret_cast<const uint32_t*>(rec), (uint32_t)nb, dn);
     sv = dv;
     sf = df;
   }

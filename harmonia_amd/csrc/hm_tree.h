@@ -16,6 +16,8 @@ namespace hm {
 int set_err(int code, const char* fmt, ...);
 size_t align256(size_t x);
 bool is_device_ptr(const void* p);
+// Pinned host memory that kernels read and write at its host address (mapped, coherent).
+int host_alloc_mapped(void** p, size_t bytes);
 
 #define HM_TRY(expr)                                                                                       \
   do {                                                                                                     \
@@ -68,6 +70,10 @@ struct hm_tree {
   uint64_t* pend_k = nullptr;
   uint64_t* pend_v = nullptr;
   size_t pend_n = 0, pend_cap = 0;
+  uint64_t* pin_stage = nullptr;  // pinned bounce for host update batches: keys [0, pin_cap), values after
+  size_t pin_cap = 0;
+  hipEvent_t stage_ev = nullptr;  // the latest staging copies (a publish on another stream waits for it)
+  bool stage_valid = false;
   void* upd_buf = nullptr;       // grow-only device scratch of hm_update / hm_epoch_publish
   size_t upd_bytes = 0;
   uint32_t* upd_host = nullptr;  // pinned: the publish plan summary read back by the host
@@ -120,10 +126,20 @@ struct PublishPlan {
 // Staging and the two publish phases; the caller holds t->pub_mu across them.  Searches may run
 // during staging and publish_prepare; publish_commit is short (an enqueue or a pointer swap).
 int stage_updates(hm_tree* t, const uint64_t* keys, const uint64_t* vals, uint64_t n, hipStream_t s);
+// hm_search_ex / hm_range on stream s.  mapped = every buffer is device memory or mapped pinned host
+// memory (host_alloc_mapped): kernels use the pointers as given, nothing is staged and the call
+// returns without waiting for the stream.
+int search_impl(hm_tree* t, const uint64_t* q, uint64_t n, uint64_t* out_vals, uint8_t* out_found,
+                const hm_search_opts* o, hipStream_t s, bool mapped);
+int range_impl(hm_tree* t, const uint64_t* start, const uint32_t* len, uint64_t n, uint64_t* out_offsets,
+               uint64_t* out_keys, uint64_t* out_vals, uint32_t* out_counts, uint64_t capacity, hipStream_t s,
+               bool mapped);
 int publish_prepare(hm_tree* t, hipStream_t s, PublishPlan* plan);
 int publish_commit(hm_tree* t, hipStream_t s, const PublishPlan& plan, int* rebuilt);
 // Allocate the standby layout and re-layout scratch now (best effort under the memory budget).
 int reserve_relayout(hm_tree* t, hipStream_t s);
+// Size the update staging and publish scratch for batches of up to n keys now.
+int reserve_updates(hm_tree* t, size_t n, hipStream_t s);
 
 int ensure_ws(hm_tree* t, hipStream_t s, size_t bytes, Workspace** out);
 // Grow stream s's workspace to at least `bytes` now (takes t->mu).

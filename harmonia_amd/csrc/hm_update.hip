@@ -86,7 +86,12 @@ __global__ void new_counts_kernel(const Leaf* __restrict__ leaves, const uint32_
                                   int fill, uint32_t* __restrict__ cnt) {
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_leaves; i += stride) {
-    const uint32_t total = (uint32_t)leaf_count(leaves[i]) + ins[i];
+    const uint32_t a = ins[i];
+    if (!a) {  // most leaves: no new key, one leaf; the leaf itself is not read
+      cnt[i] = 1u;
+      continue;
+    }
+    const uint32_t total = (uint32_t)leaf_count(leaves[i]) + a;
     cnt[i] = total <= (uint32_t)LEAF_KEYS ? 1u : (total + fill - 1) / fill;
   }
 }
@@ -97,22 +102,41 @@ __global__ void leaf_first_kernel(const uint32_t* __restrict__ leaf, size_t n, u
     if (i == 0 || leaf[i] != leaf[i - 1]) first[leaf[i]] = (uint32_t)i;  // run heads: one writer per leaf
 }
 
-// Old leaf i -> new leaves [new_start[i], new_start[i] + parts): existing keys of the batch get
-// their new value in the copy, new keys are merged in; the old leaves are only read.
-__global__ void relayout_kernel(const Leaf* __restrict__ old_leaves, size_t n_old, const uint32_t* __restrict__ ins,
-                                const uint32_t* __restrict__ first, const uint32_t* __restrict__ bleaf,
-                                const uint32_t* __restrict__ bslot, const uint64_t* __restrict__ bk,
-                                const uint64_t* __restrict__ bv, const uint8_t* __restrict__ found, size_t m,
-                                const uint32_t* __restrict__ new_start, int fill, Leaf* __restrict__ new_leaves) {
+// Re-layout, part 1: every leaf the batch does not touch moves unchanged to new_start[i].  Runs of
+// untouched leaves shift by a constant, so this is a coalesced streaming copy in 16-byte pieces.
+__global__ __launch_bounds__(256) void relayout_copy_kernel(const Leaf* __restrict__ old_leaves, size_t n_old,
+                                                            const uint32_t* __restrict__ first,
+                                                            const uint32_t* __restrict__ new_start,
+                                                            Leaf* __restrict__ new_leaves) {
+  constexpr int kPieces = sizeof(Leaf) / 16;
+  const ulonglong2* src = reinterpret_cast<const ulonglong2*>(old_leaves);
+  ulonglong2* dst = reinterpret_cast<ulonglong2*>(new_leaves);
+  const size_t total = n_old * kPieces;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_old; i += stride) {
-    const uint32_t a = ins[i];
-    const uint32_t f0 = first[i];
-    Leaf l = old_leaves[i];
-    if (f0 != 0xFFFFFFFFu)
-      for (size_t j = f0; j < m && bleaf[j] == (uint32_t)i; ++j)
-        if (found[j]) l.val[bslot[j]] = bv[j];
-    Leaf* dst = new_leaves + new_start[i];
+  for (size_t u = (size_t)blockIdx.x * blockDim.x + threadIdx.x; u < total; u += stride) {
+    const size_t i = u / kPieces;
+    if (first[i] == 0xFFFFFFFFu) dst[(size_t)new_start[i] * kPieces + u % kPieces] = src[u];
+  }
+}
+
+// Re-layout, part 2: one thread per leaf the batch touches (a run head of the sorted batch): existing
+// keys get their new value in the copy, new keys are merged in, and a leaf that overflows is split
+// evenly into ceil(total / fill) leaves at new_start[L].
+__global__ void relayout_touched_kernel(const Leaf* __restrict__ old_leaves, const uint32_t* __restrict__ ins,
+                                        const uint32_t* __restrict__ bleaf, const uint32_t* __restrict__ bslot,
+                                        const uint64_t* __restrict__ bk, const uint64_t* __restrict__ bv,
+                                        const uint8_t* __restrict__ found, size_t m,
+                                        const uint32_t* __restrict__ new_start, int fill,
+                                        Leaf* __restrict__ new_leaves) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t h = (size_t)blockIdx.x * blockDim.x + threadIdx.x; h < m; h += stride) {
+    if (h > 0 && bleaf[h] == bleaf[h - 1]) continue;  // not a run head
+    const uint32_t L = bleaf[h];
+    const uint32_t a = ins[L];
+    Leaf l = old_leaves[L];
+    for (size_t j = h; j < m && bleaf[j] == L; ++j)
+      if (found[j]) l.val[bslot[j]] = bv[j];
+    Leaf* dst = new_leaves + new_start[L];
     if (!a) {
       *dst = l;
       continue;
@@ -128,7 +152,7 @@ __global__ void relayout_kernel(const Leaf* __restrict__ old_leaves, size_t n_ol
         dst[p].val[s] = 0ull;
       }
     int part = 0, lo = 0, hi = c / parts;
-    merge_leaf(l, a, f0, bk, bv, found, [&](int idx, uint64_t k, uint64_t v) {
+    merge_leaf(l, a, (uint32_t)h, bk, bv, found, [&](int idx, uint64_t k, uint64_t v) {
       while (idx >= hi && part + 1 < parts) {
         ++part;
         lo = hi;
@@ -347,8 +371,17 @@ hipError_t relayout_leaves(const Leaf* old_leaves, size_t n_old, const uint32_t*
                            const uint8_t* found, size_t m, const uint32_t* new_start, int fill, Leaf* new_leaves,
                            hipStream_t s) {
   ProfScope prof("update", s);
-  hipLaunchKernelGGL(relayout_kernel, dim3(grid_for(n_old, 256)), dim3(256), 0, s, old_leaves, n_old, ins_per_leaf,
-                     first, bleaf, bslot, bkeys, bvals, found, m, new_start, fill, new_leaves);
+  // a few blocks per CU: the copy streams at full rate and searches on other streams still get CUs
+  const size_t pieces = n_old * (sizeof(Leaf) / 16);
+  size_t nb = (pieces + 255) / 256;
+  const size_t cap = (size_t)launch_cfg().num_cus * 4;
+  if (nb > cap) nb = cap;
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(relayout_copy_kernel, dim3((unsigned)nb), dim3(256), 0, s, old_leaves, n_old, first, new_start,
+                     new_leaves);
+  if (m)
+    hipLaunchKernelGGL(relayout_touched_kernel, dim3(grid_for(m, 256)), dim3(256), 0, s, old_leaves, ins_per_leaf,
+                       bleaf, bslot, bkeys, bvals, found, m, new_start, fill, new_leaves);
   return hipGetLastError();
 }
 
