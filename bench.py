@@ -165,6 +165,58 @@ def batched_uneven(dev, n_batches=10000, keys_log2=25, seed=4):
     return res
 
 
+def ablation(tree, dq, out_v, out_f, stream, steps, warmup):
+    """The paper's ablation (Harmonia.pdf p.138-141) on this GPU, all on the same keys, leaves and
+    2^24 unsorted queries: pointer-child trees (the reference's comparators), then the prefix-sum
+    layout with the same kernel shape, then + PSA, + tuned narrowed groups, + LDS-resident levels.
+    Every configuration's results are compared with the full pipeline's."""
+    import torch
+
+    n_q = dq.numel()
+    tree.search_batch(dq, out_vals=out_v, out_found=out_f, stream=stream)
+    stream.synchronize()
+    ref_v, ref_f = out_v.cpu().numpy().copy(), out_f.cpu().numpy().copy()
+    steps_cfg = [
+        ("pointer_linear_g1", "regular tree, child pointers, one lane per query, linear node scan",
+         lambda: tree.search_baseline(dq, out_v, out_f, kind="linear", group_size=1, stream=stream)),
+        ("pointer_linear_g8", "noprefix-nosort: child pointers, 8-lane groups, linear node scan "
+                              "(H/regular_bpt/noprefix-nosort.cu:89-237)",
+         lambda: tree.search_baseline(dq, out_v, out_f, kind="linear", group_size=8, stream=stream)),
+        ("pointer_index_g8", "HB+-tree: child pointers, 8-lane groups, index + key granule (H/src/gbpt.cu:34-113)",
+         lambda: tree.search_baseline(dq, out_v, out_f, kind="pointer", group_size=8, stream=stream)),
+        ("prefix_index_g8", "+ prefix-sum layout: the HB+ kernel with child = prefix[node] + slot",
+         lambda: tree.search_baseline(dq, out_v, out_f, kind="prefix", group_size=8, stream=stream)),
+        ("prefix_g8_persistent", "the production kernel at the same shape (8 lanes, 1 query per group, no LDS, "
+                                 "no PSA): persistent XCD-mapped grid",
+         lambda: tree.search_batch(dq, out_vals=out_v, out_found=out_f, stream=stream, group_size=8,
+                                   queries_per_group=1, lds_levels=0, psa_bits=0)),
+        ("prefix_g8_psa16", "+ PSA partial sort (16 bits), results back in input order",
+         lambda: tree.search_batch(dq, out_vals=out_v, out_found=out_f, stream=stream, group_size=8,
+                                   queries_per_group=1, lds_levels=0, psa_bits=16, output_mode=1)),
+        ("prefix_psa16_ntg", "+ the tree's narrowed-group shape for PSA batches (lanes per query, queries in "
+                             "flight per group)",
+         lambda: tree.search_batch(dq, out_vals=out_v, out_found=out_f, stream=stream, lds_levels=0, psa_bits=16,
+                                   output_mode=1)),
+        ("full", "+ top levels staged in LDS (the default pipeline)",
+         lambda: tree.search_batch(dq, out_vals=out_v, out_found=out_f, stream=stream, psa_bits=16, output_mode=1)),
+    ]
+    res = []
+    for key, what, fn in steps_cfg:
+        ms, _ = time_steps(fn, steps, warmup, stream)
+        out_f.fill_(7)
+        fn()
+        stream.synchronize()
+        same = bool(np.array_equal(out_f.cpu().numpy(), ref_f) and np.array_equal(out_v.cpu().numpy(), ref_v))
+        r = {"config": key, "what": what, "ms_per_step": round(ms, 4), "Mq_per_s": round(n_q / ms / 1e3, 1),
+             "results_identical": same}
+        log("ablation", json.dumps(r))
+        res.append(r)
+        if not same:
+            log("ABLATION RESULT MISMATCH", key)
+            sys.exit(1)
+    return res
+
+
 def served(args):
     """Run the served-load driver (hm_serve_bench_main in harmonia_amd/libhm_servebench.so, called
     in-process): plain path closed-loop capacity, then the serving loop open-loop at three Poisson
@@ -217,6 +269,7 @@ def main():
     ap.add_argument("--variants", default="all", help="'all' or 'headline'")
     ap.add_argument("--no-batched", action="store_true", help="skip the uneven-batches comparison")
     ap.add_argument("--no-served", action="store_true", help="skip the served-load runs")
+    ap.add_argument("--no-ablation", action="store_true", help="skip the layout / PSA / NTG ablation")
     ap.add_argument("--served-keys-log2", type=int, default=27)
     ap.add_argument("--served-requests", type=int, default=20000)
     ap.add_argument("--calibrate", action="store_true",
@@ -341,6 +394,11 @@ def main():
                 "bytes_per_query": BYTES_PER_QUERY}
     e2e_achieved = BYTES_PER_QUERY_PSA * n_q / (head["ms_per_step"] * 1e-3) / 1e9
 
+    abl = None
+    if not args.no_ablation:
+        log("ablation ...")
+        abl = ablation(tree, dq["unsorted"], out_v, out_f, stream, max(3, args.steps // 2), 2)
+
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
@@ -375,6 +433,7 @@ def main():
         "variants": results,
         "build_ms": round(build_ms, 1),
         "tree": {"height": st.height, "n_inner": st.n_inner, "n_leaves": st.n_leaves, "lds_levels": st.lds_levels},
+        "ablation": abl,
         "batched_uneven": batched,
         "served_mixed": serve,
     }

@@ -20,6 +20,9 @@ HM_OUT_DEFAULT = 0
 HM_OUT_UNPERMUTE = 1
 HM_OUT_SCATTER = 2
 HM_OUT_PSA_ORDER = 3
+HM_BASELINE_POINTER_CHILD = 1
+HM_BASELINE_POINTER_LINEAR = 2
+HM_BASELINE_PREFIX_PLAIN = 3
 
 u8p = ctypes.POINTER(ctypes.c_uint8)
 u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -141,6 +144,7 @@ SIGNATURES = [
     ("hm_tree_destroy", ctypes.c_int, [vp]),
     ("hm_search", ctypes.c_int, [vp, vp, ctypes.c_uint64, vp, vp, vp]),
     ("hm_search_ex", ctypes.c_int, [vp, vp, ctypes.c_uint64, vp, vp, ctypes.POINTER(HmSearchOpts), vp]),
+    ("hm_baseline_search", ctypes.c_int, [vp, ctypes.c_int, ctypes.c_uint32, vp, ctypes.c_uint64, vp, vp, vp]),
     ("hm_range", ctypes.c_int, [vp, vp, vp, ctypes.c_uint64, vp, vp, vp, vp, ctypes.c_uint64, vp]),
     ("hm_range_between", ctypes.c_int, [vp, vp, vp, ctypes.c_uint64, ctypes.c_uint32, vp, vp, vp, vp]),
     ("hm_update", ctypes.c_int, [vp, vp, vp, ctypes.c_uint64, vp]),

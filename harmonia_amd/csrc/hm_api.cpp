@@ -395,6 +395,7 @@ void free_tree(hm_tree* t) {
   if (t->sb_inner) (void)hipFree(t->sb_inner);
   if (t->sb_prefix) (void)hipFree(t->sb_prefix);
   if (t->rl_buf) (void)hipFree(t->rl_buf);
+  if (t->bl_child) (void)hipFree(t->bl_child);
   if (t->publish_ev) (void)hipEventDestroy(t->publish_ev);
   if (t->own) (void)hipStreamDestroy(t->own);
   delete t;
@@ -643,6 +644,49 @@ int hm_search_ex(hm_tree* t, const uint64_t* q, uint64_t n, uint64_t* out_vals, 
 
 int hm_search(hm_tree* t, const uint64_t* q, uint64_t n, uint64_t* out_vals, uint8_t* out_found, void* stream) {
   return hm_search_ex(t, q, n, out_vals, out_found, nullptr, stream);
+}
+
+int hm_baseline_search(hm_tree* t, int kind, uint32_t group_size, const uint64_t* q, uint64_t n, uint64_t* out_vals,
+                       uint8_t* out_found, void* stream) {
+  if (!t) return set_err(HM_EINVAL, "tree is NULL");
+  if (kind != HM_BASELINE_POINTER_CHILD && kind != HM_BASELINE_POINTER_LINEAR && kind != HM_BASELINE_PREFIX_PLAIN)
+    return set_err(HM_EINVAL, "unknown baseline kind %d", kind);
+  if (group_size != 1 && group_size != 2 && group_size != 4 && group_size != 8)
+    return set_err(HM_EINVAL, "group_size %u not in {1, 2, 4, 8}", group_size);
+  if (n == 0) return HM_OK;
+  if (!q || !out_vals || !out_found) return set_err(HM_EINVAL, "NULL buffer");
+  if (n > kMaxBatch) return set_err(HM_EINVAL, "batch of %llu queries exceeds 2^31-1", (unsigned long long)n);
+  if (!is_device_ptr(q) || !is_device_ptr(out_vals) || !is_device_ptr(out_found))
+    return set_err(HM_EINVAL, "baseline search takes device buffers only");
+  hipStream_t s = (hipStream_t)stream;
+  std::lock_guard<std::mutex> g(t->mu);
+  const size_t need = (size_t)t->n_inner * t->F;
+  if (need > t->bl_cap) {
+    if (t->p.mem_budget && t->device_bytes() + (need - t->bl_cap) * 4 > t->p.mem_budget)
+      return set_err(HM_EBUDGET, "baseline child array exceeds the device-memory budget");
+    if (t->bl_child) {
+      HM_TRY(hipDeviceSynchronize());  // searches on any stream may still read it
+      HM_TRY(hipFree(t->bl_child));
+      t->bl_child = nullptr;
+      t->bl_cap = 0;
+    }
+    HM_TRY(hipMalloc(&t->bl_child, need * 4));
+    t->bl_cap = need;
+    t->bl_version = ~0ull;
+  }
+  Workspace* w = nullptr;
+  int rc = ensure_ws(t, s, 0, &w);  // registers the stream, so a publish waits for these reads
+  if (rc) return rc;
+  if (t->publish_valid) HM_TRY(hipStreamWaitEvent(s, t->publish_ev, 0));
+  if (t->bl_version != t->layout_version || t->bl_epoch != t->epoch) {
+    HM_TRY(build_child_array(t->d_prefix, t->n_inner, t->F, t->bl_child, s));
+    t->bl_version = t->layout_version;
+    t->bl_epoch = t->epoch;
+  }
+  const int k = kind == HM_BASELINE_POINTER_CHILD ? 0 : kind == HM_BASELINE_POINTER_LINEAR ? 1 : 2;
+  HM_TRY(launch_search_baseline(t->view(), t->bl_child, q, out_vals, out_found, (uint32_t)n, (int)group_size, k, s));
+  HM_TRY(hipEventRecord(w->last_use, s));
+  return HM_OK;
 }
 
 }  // extern "C"

@@ -131,6 +131,13 @@ hipError_t launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s);
 // host memory, so moving small buffers between host and device takes no DMA-engine copy.
 hipError_t copy_words(uint64_t* dst, const uint64_t* src, size_t n, hipStream_t s);
 hipError_t copy_u32s(uint32_t* dst, const uint32_t* src, size_t n, hipStream_t s);
+
+// Comparator baselines (pointer-child trees): child[node * F + slot] = prefix[node] + slot, and a
+// plain search kernel (one query per group) that follows child[] (kind 0: index + key granule,
+// kind 1: granule-by-granule node scan) or, as the control, prefix[node] + slot (kind 2).
+hipError_t build_child_array(const uint32_t* prefix, uint32_t n_inner, int F, uint32_t* child, hipStream_t s);
+hipError_t launch_search_baseline(const TreeView& tv, const uint32_t* child, const uint64_t* q, uint64_t* out_vals,
+                                  uint8_t* out_found, uint32_t n, int group, int kind, hipStream_t s);
 // Locate (leaf, slot, found) per key (update path); n sizes the grid, dn (device) may hold the
 // actual count.
 hipError_t launch_locate(const TreeView& tv, const uint64_t* keys, size_t n, uint32_t* leaf_out, uint32_t* slot_out,

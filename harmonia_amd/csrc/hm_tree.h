@@ -87,6 +87,10 @@ struct hm_tree {
   void* rl_buf = nullptr;  // grow-only scratch of re-layouts and inner rebuilds
   size_t rl_bytes = 0;
   std::mutex pub_mu;  // serializes update staging and publishes; taken before mu
+  // comparator baselines: explicit child index per inner slot, rebuilt when the layout or epoch changes
+  uint32_t* bl_child = nullptr;
+  size_t bl_cap = 0;  // u32 entries
+  uint64_t bl_version = ~0ull, bl_epoch = ~0ull;
 
   hm::TreeView view() const {
     hm::TreeView tv{};
@@ -102,7 +106,7 @@ struct hm_tree {
   }
   uint64_t device_bytes() const {
     return (uint64_t)(inner_cap + sb_inner_cap) * (NS * 8 + 4) + (leaf_cap + sb_leaf_cap) * sizeof(hm::Leaf) +
-           pend_cap * 16 + upd_bytes + rl_bytes;
+           pend_cap * 16 + upd_bytes + rl_bytes + bl_cap * 4;
   }
 };
 

@@ -160,6 +160,19 @@ class BPlusTree:
                                      ctypes.byref(o), _stream(stream)))
         return out_vals, out_found
 
+    def search_baseline(self, q, out_vals, out_found, kind="pointer", group_size=1, stream=None):
+        """Comparator search over a pointer-child tree (same keys and leaves; device tensors only).
+
+        kind "pointer": index granule, key granule, then child[] (HB+-tree, H/src/gbpt.cu:34-113);
+        kind "linear": granule-by-granule node scan, then child[] (H/regular_bpt/noprefix-nosort.cu:89-237);
+        kind "prefix": the same plain kernel as "pointer" over the prefix-sum array (the control).
+        Results in input order, identical to search_batch."""
+        k = {"pointer": _lib.HM_BASELINE_POINTER_CHILD, "linear": _lib.HM_BASELINE_POINTER_LINEAR,
+             "prefix": _lib.HM_BASELINE_PREFIX_PLAIN}[kind]
+        check(self._lib.hm_baseline_search(self._handle(), k, group_size, _ptr(q), _len(q), _ptr(out_vals),
+                                           _ptr(out_found), _stream(stream)))
+        return out_vals, out_found
+
     def search(self, key):
         """Single search: the value, or None when absent (the reference returns -1)."""
         v, f = self.search_batch(np.array([key], dtype=np.uint64), psa_bits=0)

@@ -107,6 +107,26 @@ int hm_search(hm_tree* t, const uint64_t* q, uint64_t n, uint64_t* out_vals, uin
 int hm_search_ex(hm_tree* t, const uint64_t* q, uint64_t n, uint64_t* out_vals, uint8_t* out_found,
                  const hm_search_opts* opts, void* stream);
 
+/* Comparator baselines for layout ablations (not a serving path): the reference's pointer-child
+ * trees over the same keys and leaves, so only the inner layout differs.  The next node is a
+ * dependent read of an explicit child index, child[node * F + slot], instead of prefix[node] + slot.
+ *   HM_BASELINE_POINTER_CHILD : index granule, then key granule, then child[] -- the HB+-tree
+ *                               search bpt_search_kernel (H/src/gbpt.cu:34-113)
+ *   HM_BASELINE_POINTER_LINEAR: the node's keys scanned one 64-B granule per step with an early
+ *                               exit, then child[] -- multi_noprefix_nosort_kernel
+ *                               (H/regular_bpt/noprefix-nosort.cu:89-237) over the CFB child
+ *                               arrays (H/regular_bpt/cfb.h:16-24)
+ *   HM_BASELINE_PREFIX_PLAIN  : the control -- the same plain kernel as POINTER_CHILD, next node
+ *                               from the prefix-sum array (isolates the layout)
+ * One query per group of group_size lanes (1, 2, 4, 8), no LDS levels, no PSA; results in input
+ * order, identical to hm_search.  Device buffers only; async on stream.  The child array
+ * (n_inner * F u32) is built on first use and again after a publish. */
+#define HM_BASELINE_POINTER_CHILD 1
+#define HM_BASELINE_POINTER_LINEAR 2
+#define HM_BASELINE_PREFIX_PLAIN 3
+int hm_baseline_search(hm_tree* t, int kind, uint32_t group_size, const uint64_t* q, uint64_t n, uint64_t* out_vals,
+                       uint8_t* out_found, void* stream);
+
 /* Range scan: for each i, the first min(len[i], #keys >= start[i]) keys >= start[i] in ascending
  * order.  out_offsets[0..n] = exclusive scan of len (capacity per range); range i writes its pairs at
  * out_keys/out_vals[out_offsets[i] ..] and its count to out_counts[i].  Writes never pass
