@@ -315,7 +315,7 @@ size_t inner_scratch_bytes(uint64_t n_leaves, int F, uint32_t n_inner) {
 // the n_inner nodes plan_levels gives; scratch (inner_scratch_bytes) comes from rl_buf at offset
 // `scratch_off`.  Enqueued on s, no host synchronization.
 int build_inner_into(hm_tree* t, const Leaf* leaves, uint64_t n_leaves, uint64_t* inner, uint32_t* prefix,
-                     const Levels& lv, size_t scratch_off, hipStream_t s) {
+                     const Levels& lv, size_t scratch_off, hipStream_t s, bool mins_ready) {
   char* cur = (char*)t->rl_buf + scratch_off;
   uint64_t* minA = (uint64_t*)cur;
   cur += align256(std::max<uint64_t>(n_leaves, 1) * 8);
@@ -324,7 +324,7 @@ int build_inner_into(hm_tree* t, const Leaf* leaves, uint64_t n_leaves, uint64_t
   uint32_t* nchild = (uint32_t*)cur;
   cur += align256((size_t)lv.n_inner * 4);
   void* scan_tmp = cur;
-  HM_TRY(leaf_min_keys(leaves, n_leaves, minA, s));
+  if (!mins_ready) HM_TRY(leaf_min_keys(leaves, n_leaves, minA, s));
   uint64_t m_child = n_leaves;
   uint64_t *cmin = minA, *nmin = minB;
   for (int l = lv.height - 1; l >= 0; --l) {
@@ -368,7 +368,7 @@ int rebuild_inner(hm_tree* t, hipStream_t s) {
   }
   int rc = ensure_rl(t, inner_scratch_bytes(t->n_leaves, t->F, lv.n_inner), s);
   if (rc) return rc;
-  rc = build_inner_into(t, t->d_leaves, t->n_leaves, t->d_inner, t->d_prefix, lv, 0, s);
+  rc = build_inner_into(t, t->d_leaves, t->n_leaves, t->d_inner, t->d_prefix, lv, 0, s, false);
   if (rc) return rc;
   HM_TRY(hipStreamSynchronize(s));
   set_levels(t, lv);
@@ -1073,9 +1073,19 @@ int publish_prepare(hm_tree* t, hipStream_t s, PublishPlan* plan) {
     rc = wait_for_readers(t, s);
   }
   if (rc) return rc;
-  HM_TRY(relayout_leaves(t->d_leaves, nl, ins, first, b.leaf, b.slot, b.uk, b.uv, b.found, plan->m, nstart, t->fill,
-                         t->sb_leaves, s));
-  rc = build_inner_into(t, t->sb_leaves, new_leaves, t->sb_inner, t->sb_prefix, lv, ib_off, s);
+  // the leaf minima go straight to the inner rebuild's scratch (its first array)
+  uint64_t* new_mins = (uint64_t*)((char*)t->rl_buf + ib_off);
+  HM_TRY(relayout_touched(t->d_leaves, ins, b.leaf, b.slot, b.uk, b.uv, b.found, plan->m, nstart, t->fill,
+                          t->sb_leaves, new_mins, s));
+  // the streaming copy goes in paced chunks of ~0.5 GiB: a search whose stream shares a hardware
+  // queue with this one then waits behind one chunk, not the whole copy
+  constexpr uint64_t kChunk = 1ull << 22;
+  for (uint64_t lo = 0; lo < nl; lo += kChunk) {
+    const uint64_t hi = std::min<uint64_t>(nl, lo + kChunk);
+    HM_TRY(relayout_copy(t->d_leaves, lo, hi, first, nstart, t->sb_leaves, new_mins, s));
+    if (hi < nl) HM_TRY(hipStreamSynchronize(s));
+  }
+  rc = build_inner_into(t, t->sb_leaves, new_leaves, t->sb_inner, t->sb_prefix, lv, ib_off, s, true);
   if (rc) return rc;
   HM_TRY(hipStreamSynchronize(s));
   plan->new_leaves = new_leaves;

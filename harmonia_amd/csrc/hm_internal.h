@@ -175,11 +175,16 @@ hipError_t new_leaf_counts(const Leaf* leaves, const uint32_t* ins_per_leaf, siz
                            uint32_t* cnt, hipStream_t s);
 // first[leaf[j]] = j for the first batch index j of each leaf's run (first pre-filled with ~0).
 hipError_t leaf_first_keys(const uint32_t* leaf, size_t n, uint32_t* first, hipStream_t s);
-// Copy every old leaf into the new layout at new_start[i]: batch values of existing keys applied,
-// new keys merged, overflowing leaves split evenly (old leaves are only read).
-hipError_t relayout_leaves(const Leaf* old_leaves, size_t n_old, const uint32_t* ins_per_leaf, const uint32_t* first,
-                           const uint32_t* bleaf, const uint32_t* bslot, const uint64_t* bkeys, const uint64_t* bvals,
-                           const uint8_t* found, size_t m, const uint32_t* new_start, int fill, Leaf* new_leaves,
-                           hipStream_t s);
+// Re-layout into new_leaves (old leaves are only read); both parts also write each new leaf's
+// minimum key to new_mins[new leaf].
+//   relayout_copy: the old leaves in [lo, hi) that the batch does not touch, moved to new_start[i];
+//   relayout_touched: the touched leaves -- batch values of existing keys applied, new keys merged,
+//   overflowing leaves split evenly.
+hipError_t relayout_copy(const Leaf* old_leaves, size_t lo, size_t hi, const uint32_t* first, const uint32_t* new_start,
+                         Leaf* new_leaves, uint64_t* new_mins, hipStream_t s);
+hipError_t relayout_touched(const Leaf* old_leaves, const uint32_t* ins_per_leaf, const uint32_t* bleaf,
+                            const uint32_t* bslot, const uint64_t* bkeys, const uint64_t* bvals, const uint8_t* found,
+                            size_t m, const uint32_t* new_start, int fill, Leaf* new_leaves, uint64_t* new_mins,
+                            hipStream_t s);
 
 }  // namespace hm

@@ -102,20 +102,27 @@ __global__ void leaf_first_kernel(const uint32_t* __restrict__ leaf, size_t n, u
     if (i == 0 || leaf[i] != leaf[i - 1]) first[leaf[i]] = (uint32_t)i;  // run heads: one writer per leaf
 }
 
-// Re-layout, part 1: every leaf the batch does not touch moves unchanged to new_start[i].  Runs of
-// untouched leaves shift by a constant, so this is a coalesced streaming copy in 16-byte pieces.
-__global__ __launch_bounds__(256) void relayout_copy_kernel(const Leaf* __restrict__ old_leaves, size_t n_old,
+// Re-layout, part 1: every leaf in [lo, hi) the batch does not touch moves unchanged to
+// new_start[i].  Runs of untouched leaves shift by a constant, so this is a coalesced streaming copy
+// in 16-byte pieces; the piece holding key[0] also records the leaf's minimum key for the inner
+// rebuild (which then does not read a cache line per leaf again).
+__global__ __launch_bounds__(256) void relayout_copy_kernel(const Leaf* __restrict__ old_leaves, size_t lo, size_t hi,
                                                             const uint32_t* __restrict__ first,
                                                             const uint32_t* __restrict__ new_start,
-                                                            Leaf* __restrict__ new_leaves) {
+                                                            Leaf* __restrict__ new_leaves,
+                                                            uint64_t* __restrict__ new_mins) {
   constexpr int kPieces = sizeof(Leaf) / 16;
   const ulonglong2* src = reinterpret_cast<const ulonglong2*>(old_leaves);
   ulonglong2* dst = reinterpret_cast<ulonglong2*>(new_leaves);
-  const size_t total = n_old * kPieces;
+  const size_t end = hi * kPieces;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (size_t u = (size_t)blockIdx.x * blockDim.x + threadIdx.x; u < total; u += stride) {
+  for (size_t u = lo * kPieces + (size_t)blockIdx.x * blockDim.x + threadIdx.x; u < end; u += stride) {
     const size_t i = u / kPieces;
-    if (first[i] == 0xFFFFFFFFu) dst[(size_t)new_start[i] * kPieces + u % kPieces] = src[u];
+    if (first[i] == 0xFFFFFFFFu) {
+      const ulonglong2 v = src[u];
+      dst[(size_t)new_start[i] * kPieces + u % kPieces] = v;
+      if (u % kPieces == 0) new_mins[new_start[i]] = v.x;
+    }
   }
 }
 
@@ -127,7 +134,7 @@ __global__ void relayout_touched_kernel(const Leaf* __restrict__ old_leaves, con
                                         const uint64_t* __restrict__ bk, const uint64_t* __restrict__ bv,
                                         const uint8_t* __restrict__ found, size_t m,
                                         const uint32_t* __restrict__ new_start, int fill,
-                                        Leaf* __restrict__ new_leaves) {
+                                        Leaf* __restrict__ new_leaves, uint64_t* __restrict__ new_mins) {
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t h = (size_t)blockIdx.x * blockDim.x + threadIdx.x; h < m; h += stride) {
     if (h > 0 && bleaf[h] == bleaf[h - 1]) continue;  // not a run head
@@ -137,8 +144,10 @@ __global__ void relayout_touched_kernel(const Leaf* __restrict__ old_leaves, con
     for (size_t j = h; j < m && bleaf[j] == L; ++j)
       if (found[j]) l.val[bslot[j]] = bv[j];
     Leaf* dst = new_leaves + new_start[L];
+    uint64_t* dmin = new_mins + new_start[L];
     if (!a) {
       *dst = l;
+      *dmin = l.key[0];
       continue;
     }
     // split the merged run evenly: part p holds merged keys [p*c/parts, (p+1)*c/parts),
@@ -163,6 +172,7 @@ __global__ void relayout_touched_kernel(const Leaf* __restrict__ old_leaves, con
         dst[part].key[slot] = k;
         dst[part].val[slot] = v;
       }
+      if (slot == 0) dmin[part] = k;  // every part gets at least one key
     });
   }
 }
@@ -366,22 +376,28 @@ hipError_t leaf_first_keys(const uint32_t* leaf, size_t n, uint32_t* first, hipS
   return hipGetLastError();
 }
 
-hipError_t relayout_leaves(const Leaf* old_leaves, size_t n_old, const uint32_t* ins_per_leaf, const uint32_t* first,
-                           const uint32_t* bleaf, const uint32_t* bslot, const uint64_t* bkeys, const uint64_t* bvals,
-                           const uint8_t* found, size_t m, const uint32_t* new_start, int fill, Leaf* new_leaves,
-                           hipStream_t s) {
+hipError_t relayout_copy(const Leaf* old_leaves, size_t lo, size_t hi, const uint32_t* first, const uint32_t* new_start,
+                         Leaf* new_leaves, uint64_t* new_mins, hipStream_t s) {
+  if (hi <= lo) return hipSuccess;
   ProfScope prof("update", s);
   // a few blocks per CU: the copy streams at full rate and searches on other streams still get CUs
-  const size_t pieces = n_old * (sizeof(Leaf) / 16);
+  const size_t pieces = (hi - lo) * (sizeof(Leaf) / 16);
   size_t nb = (pieces + 255) / 256;
   const size_t cap = (size_t)launch_cfg().num_cus * 4;
   if (nb > cap) nb = cap;
-  if (nb < 1) nb = 1;
-  hipLaunchKernelGGL(relayout_copy_kernel, dim3((unsigned)nb), dim3(256), 0, s, old_leaves, n_old, first, new_start,
-                     new_leaves);
-  if (m)
-    hipLaunchKernelGGL(relayout_touched_kernel, dim3(grid_for(m, 256)), dim3(256), 0, s, old_leaves, ins_per_leaf,
-                       bleaf, bslot, bkeys, bvals, found, m, new_start, fill, new_leaves);
+  hipLaunchKernelGGL(relayout_copy_kernel, dim3((unsigned)nb), dim3(256), 0, s, old_leaves, lo, hi, first, new_start,
+                     new_leaves, new_mins);
+  return hipGetLastError();
+}
+
+hipError_t relayout_touched(const Leaf* old_leaves, const uint32_t* ins_per_leaf, const uint32_t* bleaf,
+                            const uint32_t* bslot, const uint64_t* bkeys, const uint64_t* bvals, const uint8_t* found,
+                            size_t m, const uint32_t* new_start, int fill, Leaf* new_leaves, uint64_t* new_mins,
+                            hipStream_t s) {
+  if (!m) return hipSuccess;
+  ProfScope prof("update", s);
+  hipLaunchKernelGGL(relayout_touched_kernel, dim3(grid_for(m, 256)), dim3(256), 0, s, old_leaves, ins_per_leaf,
+                     bleaf, bslot, bkeys, bvals, found, m, new_start, fill, new_leaves, new_mins);
   return hipGetLastError();
 }
 
