@@ -25,6 +25,14 @@ METRIC = "Mqueries/s at 64M keys & % HBM roofline; served p50/p99 latency vs pla
 PEAK_HBM_GBS = 8000.0
 BYTES_PER_QUERY = 144  # 8 B key in + 8 B value out + one 64-B leaf-key granule + one 64-B value granule
 BYTES_PER_QUERY_PSA = 192  # + 2 radix passes over (8 B key + 4 B position), read and written
+BYTES_PER_UPSERT = 272  # 16 B in + read and write of the leaf's key and value granules
+
+
+def range_bytes(lens):
+    """Algorithmic bytes of ranges of these lengths: start + length in, L (key, value) pairs out, and
+    the key and value granules of every leaf walked (ceil(L/8) + 1 leaves)."""
+    L = np.asarray(lens, dtype=np.int64)
+    return int((16 + 16 * L + 128 * ((L + 7) // 8 + 1)).sum())
 
 
 def log(*a):
@@ -217,6 +225,166 @@ def ablation(tree, dq, out_v, out_f, stream, steps, warmup):
     return res
 
 
+def skewed_and_ranges(dev, lib, steps, warmup):
+    """Skewed search plus range scans: fanout-128 tree bulk-built from 2^26 keys; 2^24 point queries
+    Zipf(0.99) over key rank (PSA 16, input order); 2^20 ranges, start uniform in [0, 2^63), length
+    uniform in [16, 256] (range_prefix_sort_1thread_search_1_thread_scan_kernel, H/src/range.cu:101-227).
+    Zipf results are checked in full (all hits, value = splitmix64_mix(key)); range counts in full
+    against the closed form, pairs on a sample."""
+    import torch
+
+    from harmonia_amd import BPlusTree
+    from harmonia_amd import workloads as W
+
+    seed = W.config_seed(2)
+    n_keys = 1 << 26
+    keys, vals = W.gen_keys(n_keys, seed)
+    tree = BPlusTree.from_keys(torch.from_numpy(keys.view(np.int64)).to(dev),
+                               torch.from_numpy(vals.view(np.int64)).to(dev), fanout=128, leaf_fill=8)
+    del vals
+    sk = np.sort(keys)
+    del keys
+    stream = torch.cuda.Stream()
+    q = W.zipf_queries(sk, 1 << 24, seed + 11)
+    dq = torch.from_numpy(q.view(np.int64)).to(dev)
+    ov = torch.empty(len(q), dtype=torch.int64, device=dev)
+    of = torch.empty(len(q), dtype=torch.uint8, device=dev)
+    fn = lambda: tree.search_batch(dq, out_vals=ov, out_found=of, stream=stream, psa_bits=16, output_mode=1)
+    zms, _ = time_steps(fn, steps, warmup, stream)
+    fn()
+    stream.synchronize()
+    ok = bool(of.all()) and np.array_equal(ov.cpu().numpy().view(np.uint64), W.splitmix64_mix(q))
+    res = {"zipf_search": {"workload": "fanout-128 tree of 2^26 keys, 2^24 Zipf(0.99) point queries, PSA 16, "
+                                       "results in input order", "ms_per_step": round(zms, 4),
+                           "Mq_per_s": round(len(q) / zms / 1e3, 1), "results_ok": ok}}
+    log("zipf", json.dumps(res["zipf_search"]))
+    del dq, ov, of, q
+    starts, lens = W.range_queries(1 << 20, seed + 13)
+    n = len(starts)
+    cap = int(lens.astype(np.int64).sum())
+    d_s = torch.from_numpy(starts.view(np.int64)).to(dev)
+    d_l = torch.from_numpy(lens.view(np.int32)).to(dev)
+    d_off = torch.empty(n + 1, dtype=torch.int64, device=dev)
+    d_k = torch.empty(cap, dtype=torch.int64, device=dev)
+    d_v = torch.empty(cap, dtype=torch.int64, device=dev)
+    d_c = torch.empty(n, dtype=torch.int32, device=dev)
+
+    def rfn():
+        rc = lib.hm_range(tree._handle(), d_s.data_ptr(), d_l.data_ptr(), n, d_off.data_ptr(), d_k.data_ptr(),
+                          d_v.data_ptr(), d_c.data_ptr(), cap, stream.cuda_stream)
+        if rc:
+            raise RuntimeError(f"hm_range failed: {rc}")
+
+    rms, _ = time_steps(rfn, steps, warmup, stream)
+    lib.hm_profile_reset()
+    lib.hm_profile_enable(1)
+    for _ in range(3):
+        rfn()
+    torch.cuda.synchronize()
+    kms, _ = stage_avg(lib, "range")
+    lib.hm_profile_enable(0)
+    lib.hm_profile_reset()
+    lb = np.searchsorted(sk, starts, side="left").astype(np.int64)
+    exp_cnt = np.minimum(lens.astype(np.int64), n_keys - lb)
+    cnt = d_c.cpu().numpy().astype(np.int64)
+    off = d_off.cpu().numpy()
+    ok = bool(np.array_equal(cnt, exp_cnt))
+    rk = d_k.cpu().numpy().view(np.uint64)
+    rv = d_v.cpu().numpy().view(np.uint64)
+    for i in W.rng(seed, 21).integers(0, n, 4096):
+        c, o = int(exp_cnt[i]), int(off[i])
+        ok &= bool(np.array_equal(rk[o:o + c], sk[lb[i]:lb[i] + c]))
+        ok &= bool(np.array_equal(rv[o:o + c], W.splitmix64_mix(sk[lb[i]:lb[i] + c])))
+    algo = range_bytes(exp_cnt)
+    achieved = algo / (kms * 1e-3) / 1e9 if kms else None
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "range_kernel_traffic.json")
+    if os.path.exists(tpath):
+        try:
+            traffic = json.load(open(tpath)).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+    res["ranges"] = {"workload": "2^20 ranges, start uniform in [0, 2^63), length uniform in [16, 256], on the "
+                                 "same tree", "ms_per_step": round(rms, 4),
+                     "M_ranges_per_s": round(n / rms / 1e3, 2), "pairs_per_s_G": round(int(exp_cnt.sum()) / rms / 1e6, 2),
+                     "range_kernel_ms": round(kms, 4) if kms else None,
+                     "algorithmic_bytes": algo,
+                     "roofline": {"bound": "hbm", "achieved": round(achieved, 1) if achieved else None,
+                                  "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                                  "frac": round(achieved / PEAK_HBM_GBS, 4) if achieved else None,
+                                  "traffic": traffic, "kernel": "range_kernel<128> (2^20 ranges per launch)"},
+                     "results_ok": ok}
+    log("ranges", json.dumps(res["ranges"]))
+    tree.close()
+    if not (res["zipf_search"]["results_ok"] and ok):
+        log("SKEWED / RANGE RESULT CHECK FAILED")
+        sys.exit(1)
+    return res
+
+
+def batched_updates(dev, lib):
+    """Batched upsert on the 2^26-key fanout-64 tree (bulk fill 6, so two free slots per leaf): the
+    reference's update experiment shape, batches of 2^16 and 2^20 keys with 74.25% of them already in the
+    tree (H/script/generate-data/generate.py:128-129; H/test.sh option 39), each applied as one epoch
+    (hm_update + hm_epoch_publish, device buffers).  The last batch's keys are read back."""
+    import torch
+
+    from harmonia_amd import BPlusTree
+    from harmonia_amd import workloads as W
+
+    seed = W.config_seed(1)
+    n_keys = 1 << 26
+    keys, vals = W.gen_keys(n_keys, seed)
+    tree = BPlusTree.from_keys(torch.from_numpy(keys.view(np.int64)).to(dev),
+                               torch.from_numpy(vals.view(np.int64)).to(dev), fanout=64, leaf_fill=6)
+    del vals
+    cur = np.sort(keys)
+    del keys
+    stream = torch.cuda.Stream()
+    g = W.rng(seed, 31)
+    out = []
+    step = 0
+    for batch in (1 << 16, 1 << 20, 1 << 16, 1 << 20, 1 << 20):
+        n_old = int(round(batch * 0.7425))
+        old = cur[g.choice(len(cur), n_old, replace=False)]
+        new = W.gen_misses(len(cur), batch - n_old, seed + 100 + step) ^ np.uint64(step << 56)
+        new = np.unique(new)
+        new = new[cur[np.minimum(np.searchsorted(cur, new), len(cur) - 1)] != new]
+        bk = np.concatenate([old, new])
+        g.shuffle(bk)
+        bv = g.integers(0, 1 << 64, len(bk), dtype=np.uint64)
+        dk = torch.from_numpy(bk.view(np.int64)).to(dev)
+        dv = torch.from_numpy(bv.view(np.int64)).to(dev)
+        torch.cuda.synchronize()
+        lib.hm_profile_reset()
+        lib.hm_profile_enable(1)
+        t0 = time.perf_counter()
+        tree.update(dk, dv, stream=stream)
+        rebuilt = tree.publish(stream=stream)
+        stream.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3
+        rl_ms, _ = stage_avg(lib, "relayout")
+        lib.hm_profile_enable(0)
+        lib.hm_profile_reset()
+        cur = np.insert(cur, np.searchsorted(cur, new), new)  # new is sorted, unique and not in cur
+        r = {"batch": len(bk), "existing": n_old, "new": len(new), "ms": round(ms, 3),
+             "M_upserts_per_s": round(len(bk) / ms / 1e3, 2), "relaid_out": bool(rebuilt),
+             "relayout_ms": round(rl_ms, 3) if rl_ms else None,
+             "achieved_GBs_272B": round(BYTES_PER_UPSERT * len(bk) / (ms * 1e-3) / 1e9, 1)}
+        log("update", json.dumps(r))
+        out.append(r)
+        step += 1
+    v, f = tree.search_batch(bk, psa_bits=0)
+    ok = bool(f.all()) and np.array_equal(v, bv) and tree.stats().n_keys == len(cur)
+    tree.close()
+    if not ok:
+        log("UPDATE RESULT CHECK FAILED")
+        sys.exit(1)
+    return {"workload": "2^26-key fanout-64 tree at fill 6; upsert batches of 2^16 / 2^20 keys, 74.25% existing, "
+                        "one epoch each, device buffers (first batch includes scratch allocation)",
+            "batches": out, "results_ok": ok}
+
+
 def served(args):
     """Run the served-load driver (hm_serve_bench_main in harmonia_amd/libhm_servebench.so, called
     in-process): plain path closed-loop capacity, then the serving loop open-loop at three Poisson
@@ -270,6 +438,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the uneven-batches comparison")
     ap.add_argument("--no-served", action="store_true", help="skip the served-load runs")
     ap.add_argument("--no-ablation", action="store_true", help="skip the layout / PSA / NTG ablation")
+    ap.add_argument("--no-ranges", action="store_true", help="skip the fanout-128 Zipf search and range scans")
+    ap.add_argument("--no-updates", action="store_true", help="skip the batched-update epochs")
     ap.add_argument("--served-keys-log2", type=int, default=27)
     ap.add_argument("--served-requests", type=int, default=20000)
     ap.add_argument("--calibrate", action="store_true",
@@ -408,6 +578,17 @@ def main():
     del dq, out_v, out_f
     torch.cuda.empty_cache()
 
+    skewed = None
+    if not args.no_ranges:
+        log("skewed search + ranges ...")
+        skewed = skewed_and_ranges(dev, lib, max(3, args.steps // 2), 2)
+        torch.cuda.empty_cache()
+    updates = None
+    if not args.no_updates:
+        log("batched updates ...")
+        updates = batched_updates(dev, lib)
+        torch.cuda.empty_cache()
+
     batched = None
     if not args.no_batched:
         log("uneven batches ...")
@@ -434,6 +615,8 @@ def main():
         "build_ms": round(build_ms, 1),
         "tree": {"height": st.height, "n_inner": st.n_inner, "n_leaves": st.n_leaves, "lds_levels": st.lds_levels},
         "ablation": abl,
+        "skewed_and_ranges": skewed,
+        "updates": updates,
         "batched_uneven": batched,
         "served_mixed": serve,
     }

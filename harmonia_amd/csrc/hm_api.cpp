@@ -692,6 +692,8 @@ int hm_baseline_search(hm_tree* t, int kind, uint32_t group_size, const uint64_t
 }  // extern "C"
 
 namespace hm {
+size_t range_ws_bytes(size_t n) { return align256((n + 1) * 8) + align256(scan_tmp_bytes(n + 1)); }
+
 static int range_common(hm_tree* t, const uint64_t* start, const uint64_t* hi, const uint32_t* len, uint32_t cap_const,
                         uint64_t n, uint64_t* out_offsets, uint64_t* out_keys, uint64_t* out_vals,
                         uint32_t* out_counts, uint64_t capacity, hipStream_t s, bool mapped) {

@@ -6,8 +6,16 @@
 // The reference emits at most Range_Size = 8 records per range (H/src/range.cu:32); here a range
 // emits up to its own length, or every key of a closed interval [lo, hi] as
 // find_range(start, end) does (H/regular_bpt/regular_bpt.cpp:320-338).
-// One group of 8 lanes per range: lane s owns slot s of the current leaf, and a ballot over the
-// group packs the qualifying keys into the output in order.
+// One group of 8 lanes per range: lane s owns slot s of each leaf, and a ballot over the group
+// packs the qualifying keys into the output in order.  The walk loads up to kWalk consecutive leaves
+// per step (their addresses are known: i, i + 1, ...), so each group keeps several loads in flight
+// instead of one dependent load per leaf; the leaves are then consumed in order exactly as one at a
+// time.  A step loads no more leaves than the range can still use (ceil(rest / 8) + 1: a leaf holds
+// at most 8 keys, and the first one may hold keys below start).
+// Ranges run in input order: consecutive groups then write consecutive output regions.  (Walking
+// them in PSA order of their starts, as the reference sorts range starts, H/src/range.cu:280-285,
+// was measured: the leaf reads gain locality, the output writes lose it, and the kernel time did
+// not change.)
 #include "hm_device.h"
 
 namespace hm {
@@ -15,6 +23,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int G = 8;
+constexpr int kWalk = 8;
 
 template <int F>
 __global__ __launch_bounds__(kThreads) void range_kernel(TreeView tv, const uint64_t* __restrict__ start,
@@ -41,22 +50,34 @@ __global__ __launch_bounds__(kThreads) void range_kernel(TreeView tv, const uint
     uint32_t got = 0;
     bool done = !valid || L == 0 || s > h || s == KEY_MAX;
     while (!done) {  // uniform inside the group
-      const uint64_t k = tv.leaves[leaf].key[gl];
-      const uint64_t v = tv.leaves[leaf].val[gl];
-      const bool take = k != KEY_MAX && k >= s && k <= h;
-      const uint64_t m = (__ballot(take) >> gbase) & 0xFFull;
-      const uint64_t beyond = (__ballot(k != KEY_MAX && k > h) >> gbase) & 0xFFull;
-      const uint32_t rank = (uint32_t)__popcll(m & lanes_below);
-      if (take && got + rank < L) {
-        const uint64_t o = off + got + rank;
-        if (o < capacity) {
-          out_keys[o] = k;
-          out_vals[o] = v;
+      uint64_t k[kWalk], v[kWalk];
+      const uint32_t need = (L - got + LEAF_KEYS - 1) / LEAF_KEYS + 1;  // got < L here
+      const uint32_t step = need < (uint32_t)kWalk ? need : (uint32_t)kWalk;
+#pragma unroll
+      for (int u = 0; u < kWalk; ++u) {
+        const bool in = (uint32_t)u < step && leaf + u < tv.n_leaves;
+        k[u] = in ? tv.leaves[leaf + u].key[gl] : KEY_MAX;
+        v[u] = in ? tv.leaves[leaf + u].val[gl] : 0ull;
+      }
+#pragma unroll
+      for (int u = 0; u < kWalk; ++u) {
+        if (!done && (uint32_t)u < step) {  // group-uniform
+          const bool take = k[u] != KEY_MAX && k[u] >= s && k[u] <= h;
+          const uint64_t m = (__ballot(take) >> gbase) & 0xFFull;
+          const uint64_t beyond = (__ballot(k[u] != KEY_MAX && k[u] > h) >> gbase) & 0xFFull;
+          const uint32_t rank = (uint32_t)__popcll(m & lanes_below);
+          if (take && got + rank < L) {
+            const uint64_t o = off + got + rank;
+            if (o < capacity) {
+              out_keys[o] = k[u];
+              out_vals[o] = v[u];
+            }
+          }
+          got += (uint32_t)__popcll(m);
+          done = got >= L || beyond != 0 || leaf + u + 1 >= tv.n_leaves;
         }
       }
-      got += (uint32_t)__popcll(m);
-      ++leaf;
-      done = got >= L || beyond != 0 || leaf >= tv.n_leaves;
+      leaf += step;
     }
     if (valid && gl == 0) out_counts[ri] = got < L ? got : L;
   }

@@ -311,7 +311,7 @@ int reserve_worker(hm_server* sv, Worker& w) {
   HM_TRY_RC(w.hk.ensure(b * 16));
   HM_TRY_RC(w.doff.ensure((b + 1) * 8));
   // the tree's workspace for this stream: the largest search pipeline or range scan of a batch
-  const size_t ws = std::max(pipe_bytes(b, 32, HM_OUT_UNPERMUTE), align256(b * 8) + align256(scan_tmp_bytes(b + 1)));
+  const size_t ws = std::max(pipe_bytes(b, 32, HM_OUT_UNPERMUTE), range_ws_bytes(b));
   return reserve_workspace(sv->t, w.s, ws);
 }
 

@@ -135,6 +135,8 @@ int stage_updates(hm_tree* t, const uint64_t* keys, const uint64_t* vals, uint64
 // returns without waiting for the stream.
 int search_impl(hm_tree* t, const uint64_t* q, uint64_t n, uint64_t* out_vals, uint8_t* out_found,
                 const hm_search_opts* o, hipStream_t s, bool mapped);
+// Stream workspace of a range batch of n ranges with device (or mapped) buffers.
+size_t range_ws_bytes(size_t n);
 int range_impl(hm_tree* t, const uint64_t* start, const uint32_t* len, uint64_t n, uint64_t* out_offsets,
                uint64_t* out_keys, uint64_t* out_vals, uint32_t* out_counts, uint64_t capacity, hipStream_t s,
                bool mapped);

@@ -188,10 +188,12 @@ def test_psa_sort_bit_exact(gpu, n, bits):
 
 
 # ---------------------------------------------------------------- range
-@pytest.mark.parametrize("F,fill", [(64, 8), (128, 8), (32, 3)])
-def test_range_parity(gpu, F, fill):
-    t, keys, vals, sk, sv = make(60000, F, fill, seed=41 + F)
-    starts, lens = W.range_queries(20000, 41)
+@pytest.mark.parametrize("F,fill,psa", [(64, 8, 16), (128, 8, 16), (32, 3, 16), (64, 8, 0), (128, 5, 12)])
+def test_range_parity(gpu, F, fill, psa):
+    """40000 ranges (with starts on keys, past the last key, 0, KEY_MAX and a zero length) against the
+    oracle's leaf walk and the closed form, for several fanouts, leaf fills and PSA settings."""
+    t, keys, vals, sk, sv = make(60000, F, fill, seed=41 + F, psa_bits=psa)
+    starts, lens = W.range_queries(40000, 41)
     starts[:100] = sk[W.rng(1, 1).integers(0, len(sk), 100)]
     starts[100] = sk[-1] + U64(1)
     starts[101] = 0
