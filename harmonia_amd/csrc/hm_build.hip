@@ -21,7 +21,11 @@ __global__ void validate_sorted_kernel(const uint64_t* __restrict__ k, size_t n,
     if (x == KEY_MAX) f |= 1u;
     if (i > 0 && k[i - 1] >= x) f |= 2u;
   }
-  if (f) atomicOr(flags, f);
+  // one atomic per wave: unsorted input sets a flag in nearly every lane, and atomics on one
+  // address serialize
+  const uint64_t b1 = __ballot((f & 1u) != 0), b2 = __ballot((f & 2u) != 0);
+  const uint32_t w = (b1 ? 1u : 0u) | (b2 ? 2u : 0u);
+  if ((threadIdx.x & 63) == 0 && w) atomicOr(flags, w);
 }
 
 __global__ void key_bits_kernel(const uint64_t* __restrict__ k, size_t n, uint64_t* __restrict__ orand) {

@@ -25,6 +25,60 @@ int grid_for(size_t work, int threads) {
   return (int)g;
 }
 
+// Register-array access with a run-time index, as unrolled selects: a dynamic index into a local
+// array would put the whole leaf in scratch memory.
+__device__ __forceinline__ uint64_t pick(const uint64_t (&a)[LEAF_KEYS], int i) {
+  // masks rather than selects: the compiler folds a select chain back into one indexed load
+  uint64_t r = 0;
+#pragma unroll
+  for (int s = 0; s < LEAF_KEYS; ++s) r |= a[s] & (0ull - (uint64_t)(i == s));
+  return r;
+}
+
+__device__ __forceinline__ void put(uint64_t (&a)[LEAF_KEYS], int i, uint64_t v) {
+#pragma unroll
+  for (int s = 0; s < LEAF_KEYS; ++s) a[s] = i == s ? v : a[s];
+}
+
+// A leaf held in registers (a Leaf struct copy would live in scratch memory).
+struct LeafRegs {
+  uint64_t key[LEAF_KEYS];
+  uint64_t val[LEAF_KEYS];
+};
+
+__device__ __forceinline__ LeafRegs load_leaf(const Leaf* p) {
+  const ulonglong2* q = reinterpret_cast<const ulonglong2*>(p);
+  LeafRegs r;
+#pragma unroll
+  for (int i = 0; i < LEAF_KEYS / 2; ++i) {
+    const ulonglong2 a = q[i];
+    r.key[2 * i] = a.x;
+    r.key[2 * i + 1] = a.y;
+  }
+#pragma unroll
+  for (int i = 0; i < LEAF_KEYS / 2; ++i) {
+    const ulonglong2 b = q[LEAF_KEYS / 2 + i];
+    r.val[2 * i] = b.x;
+    r.val[2 * i + 1] = b.y;
+  }
+  return r;
+}
+
+__device__ __forceinline__ void store_leaf(Leaf* p, const LeafRegs& r) {
+  ulonglong2* q = reinterpret_cast<ulonglong2*>(p);
+#pragma unroll
+  for (int i = 0; i < LEAF_KEYS / 2; ++i) q[i] = make_ulonglong2(r.key[2 * i], r.key[2 * i + 1]);
+#pragma unroll
+  for (int i = 0; i < LEAF_KEYS / 2; ++i) q[LEAF_KEYS / 2 + i] = make_ulonglong2(r.val[2 * i], r.val[2 * i + 1]);
+}
+
+__device__ __forceinline__ int leaf_count(const LeafRegs& l) {
+  int c = 0;
+#pragma unroll
+  for (int i = 0; i < LEAF_KEYS; ++i) c += l.key[i] != KEY_MAX ? 1 : 0;
+  return c;
+}
+
 __device__ __forceinline__ int leaf_count(const Leaf& l) {
   int c = 0;
 #pragma unroll
@@ -60,16 +114,17 @@ __global__ void count_inserts_kernel(const uint32_t* __restrict__ leaf, const ui
 // found and map to this leaf; they are contiguous among the not-found keys because the batch is
 // sorted) in key order, calling emit(i, key, value) for i = 0, 1, ...; returns the merged count.
 template <typename Emit>
-__device__ int merge_leaf(const Leaf& l, uint32_t n_ins, uint32_t start, const uint64_t* __restrict__ bk,
+__device__ __forceinline__ int merge_leaf(const LeafRegs& l, uint32_t n_ins, uint32_t start, const uint64_t* __restrict__ bk,
                           const uint64_t* __restrict__ bv, const uint8_t* __restrict__ found, Emit emit) {
   const int cur = leaf_count(l);
   int a = 0, c = 0;
   uint32_t b = start, taken = 0;
   while (taken < n_ins && found[b]) ++b;  // skip existing keys interleaved with the inserts
   while (a < cur || taken < n_ins) {
-    const bool take_leaf = taken >= n_ins || (a < cur && l.key[a] < bk[b]);
+    const uint64_t ka = pick(l.key, a);
+    const bool take_leaf = taken >= n_ins || (a < cur && ka < bk[b]);
     if (take_leaf) {
-      emit(c, l.key[a], l.val[a]);
+      emit(c, ka, pick(l.val, a));
       ++a;
     } else {
       emit(c, bk[b], bv[b]);
@@ -140,13 +195,13 @@ __global__ void relayout_touched_kernel(const Leaf* __restrict__ old_leaves, con
     if (h > 0 && bleaf[h] == bleaf[h - 1]) continue;  // not a run head
     const uint32_t L = bleaf[h];
     const uint32_t a = ins[L];
-    Leaf l = old_leaves[L];
+    LeafRegs l = load_leaf(old_leaves + L);
     for (size_t j = h; j < m && bleaf[j] == L; ++j)
-      if (found[j]) l.val[bslot[j]] = bv[j];
+      if (found[j]) put(l.val, (int)bslot[j], bv[j]);
     Leaf* dst = new_leaves + new_start[L];
     uint64_t* dmin = new_mins + new_start[L];
     if (!a) {
-      *dst = l;
+      store_leaf(dst, l);
       *dmin = l.key[0];
       continue;
     }
@@ -250,16 +305,15 @@ __device__ __forceinline__ bool run_head(const uint32_t* __restrict__ leaf, uint
 __global__ void plan_runs_kernel(const Leaf* __restrict__ leaves, const uint32_t* __restrict__ leaf,
                                  const uint8_t* __restrict__ found, size_t n_max, const uint32_t* __restrict__ dm,
                                  uint32_t* __restrict__ summary) {
+  __shared__ uint32_t block_new;
+  if (threadIdx.x == 0) block_new = 0;
+  __syncthreads();
   const uint32_t m = dm ? *dm : (uint32_t)n_max;
   const uint32_t stride = gridDim.x * blockDim.x;
-  // grid-stride loop with a wave-uniform trip count, so the ballot below sees the whole wave
-  const uint32_t m_up = (m + 63u) & ~63u;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m_up; i += stride) {
-    const bool valid = i < m;
-    const bool is_new = valid && !found[i];
-    const uint64_t nb = __ballot(is_new);
-    if ((threadIdx.x & 63) == 0 && nb) atomicAdd(&summary[1], (uint32_t)__popcll(nb));
-    if (valid && run_head(leaf, i)) {
+  uint32_t my_new = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
+    my_new += found[i] ? 0u : 1u;
+    if (run_head(leaf, i)) {
       // count this leaf's new keys, stopping once it cannot fit
       const uint32_t L = leaf[i];
       const uint32_t room = (uint32_t)(LEAF_KEYS - leaf_count(leaves[L]));
@@ -268,6 +322,13 @@ __global__ void plan_runs_kernel(const Leaf* __restrict__ leaves, const uint32_t
       if (ins > room) atomicOr(&summary[0], 1u);
     }
   }
+  // new-key count: wave sum, then one LDS atomic per wave and one global atomic per block (a
+  // global atomic per wave on one address serializes)
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) my_new += __shfl_xor(my_new, o);
+  if ((threadIdx.x & 63) == 0 && my_new) atomicAdd(&block_new, my_new);
+  __syncthreads();
+  if (threadIdx.x == 0 && block_new) atomicAdd(&summary[1], block_new);
 }
 
 __global__ void apply_runs_kernel(Leaf* __restrict__ leaves, const uint32_t* __restrict__ leaf,
@@ -287,8 +348,8 @@ __global__ void apply_runs_kernel(Leaf* __restrict__ leaves, const uint32_t* __r
         ++ins;
     }
     if (!ins) continue;
-    const Leaf l = leaves[L];  // this thread's value writes above are visible to its own load
-    Leaf o;
+    const LeafRegs l = load_leaf(leaves + L);  // this thread's value writes above are visible to its own load
+    LeafRegs o;
 #pragma unroll
     for (int s = 0; s < LEAF_KEYS; ++s) {
       o.key[s] = KEY_MAX;
@@ -296,12 +357,10 @@ __global__ void apply_runs_kernel(Leaf* __restrict__ leaves, const uint32_t* __r
     }
     // the plan guarantees at most LEAF_KEYS merged keys
     merge_leaf(l, ins, i, uk, uv, found, [&](int idx, uint64_t k, uint64_t v) {
-      if (idx < LEAF_KEYS) {
-        o.key[idx] = k;
-        o.val[idx] = v;
-      }
+      put(o.key, idx, k);  // idx >= LEAF_KEYS matches no slot
+      put(o.val, idx, v);
     });
-    leaves[L] = o;
+    store_leaf(leaves + L, o);
   }
 }
 
