@@ -132,6 +132,9 @@ class HmServerStats(ctypes.Structure):
         ("max_update_us", ctypes.c_double),
         ("max_prepare_us", ctypes.c_double),
         ("max_commit_wait_us", ctypes.c_double),
+        ("p99_lookup_us", ctypes.c_double),
+        ("p99_range_us", ctypes.c_double),
+        ("p99_update_us", ctypes.c_double),
     ]
 
     def as_dict(self):

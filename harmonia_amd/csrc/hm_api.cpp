@@ -51,6 +51,9 @@ int host_alloc_mapped(void** p, size_t bytes) {
     *p = nullptr;
     return set_err(HM_EDEVICE, "pinned host memory is not mapped at its host address");
   }
+  // touch every page now: the CPU mapping is filled in on first access, one fault per 4-KiB page,
+  // which made the first batches through a fresh serving buffer copy at ~2 GB/s
+  std::memset(*p, 0, bytes);
   return HM_OK;
 }
 
@@ -1079,9 +1082,9 @@ int publish_prepare(hm_tree* t, hipStream_t s, PublishPlan* plan) {
   uint64_t* new_mins = (uint64_t*)((char*)t->rl_buf + ib_off);
   HM_TRY(relayout_touched(t->d_leaves, ins, b.leaf, b.slot, b.uk, b.uv, b.found, plan->m, nstart, t->fill,
                           t->sb_leaves, new_mins, s));
-  // the streaming copy goes in paced chunks of ~0.5 GiB: a search whose stream shares a hardware
-  // queue with this one then waits behind one chunk, not the whole copy
-  constexpr uint64_t kChunk = 1ull << 22;
+  // the streaming copy goes in paced chunks of 256 MiB (~0.1 ms): a search whose stream shares a
+  // hardware queue with this one then waits behind one chunk, not the whole copy
+  constexpr uint64_t kChunk = 1ull << 21;
   for (uint64_t lo = 0; lo < nl; lo += kChunk) {
     const uint64_t hi = std::min<uint64_t>(nl, lo + kChunk);
     HM_TRY(relayout_copy(t->d_leaves, lo, hi, first, nstart, t->sb_leaves, new_mins, s));

@@ -239,19 +239,24 @@ extern "C" int hm_serve_bench_main(int argc, char** argv, char* out_json, uint64
       "\"lookups\": %llu, \"ranges\": %llu, \"updates\": %llu, \"bad_results\": %llu, \"keys\": %llu, "
       "\"inflight\": %u, \"build_s\": %.2f, \"leaf_fill\": %u, \"relayouts\": %llu, "
       "\"max_read_batch_us\": %.1f, \"max_update_us\": %.1f, \"max_prepare_us\": %.1f, "
-      "\"max_commit_wait_us\": %.1f}\n",
+      "\"max_commit_wait_us\": %.1f, \"p99_lookup_us\": %.1f, \"p99_range_us\": %.1f, "
+      "\"p99_update_us\": %.1f}\n",
       plain ? "plain" : "served", closed ? "closed" : "poisson", rate, (unsigned long long)n_req,
       (unsigned long long)st.queries, st.span_s, submit_s, st.queries / st.span_s, st.completed / st.span_s,
       st.p50_us, st.p99_us, st.mean_us, st.max_us, (unsigned long long)st.launches, (unsigned long long)st.epochs,
       (unsigned long long)st.lookups, (unsigned long long)st.ranges, (unsigned long long)st.updates,
       (unsigned long long)bad, (unsigned long long)n, sp.max_inflight, build_s, p.leaf_fill,
       (unsigned long long)ts.relayouts, st.max_read_batch_us, st.max_update_us, st.max_prepare_us,
-      st.max_commit_wait_us);
+      st.max_commit_wait_us, st.p99_lookup_us, st.p99_range_us, st.p99_update_us);
   return bad ? 3 : 0;
 }
 
 #ifdef HM_SERVE_BENCH_EXE
 int main(int argc, char** argv) {
+  // one hardware queue per worker stream plus the tree's own and the null stream: a read whose
+  // stream shares a queue with an update's re-layout would wait behind it (read by the HIP runtime
+  // when it initializes, so before the first HIP call; an explicit setting wins)
+  setenv("GPU_MAX_HW_QUEUES", "8", 0);
   char out[4096];
   out[0] = 0;
   const int rc = hm_serve_bench_main(argc, argv, out, sizeof out);

@@ -130,6 +130,7 @@ struct hm_server {
   std::vector<std::unique_ptr<Worker>> workers;
   // statistics
   std::vector<double> lat_us;
+  std::vector<double> lat_kind_us[3];  // by request kind (REQ_LOOKUP, REQ_RANGE, REQ_UPDATE)
   uint64_t n_lookups = 0, n_ranges = 0, n_updates = 0, n_queries = 0, n_rejected = 0, n_launches = 0,
            n_epochs = 0;
   bool have_first = false;
@@ -150,7 +151,9 @@ void complete_all(hm_server* sv, const std::vector<ReqPtr>& rs, const int* statu
       r.status = status ? status[i] : status_all;
       r.epoch = epoch;
       r.done = true;
-      sv->lat_us.push_back(std::chrono::duration<double, std::micro>(now - r.t_submit).count());
+      const double us = std::chrono::duration<double, std::micro>(now - r.t_submit).count();
+      sv->lat_us.push_back(us);
+      sv->lat_kind_us[r.type].push_back(us);
       if (r.type == REQ_LOOKUP) ++sv->n_lookups, sv->n_queries += r.n;
       if (r.type == REQ_RANGE) ++sv->n_ranges, sv->n_queries += r.n;
       if (r.type == REQ_UPDATE) ++sv->n_updates;
@@ -167,22 +170,28 @@ uint64_t tree_epoch(hm_tree* t) {
   return st.epoch;
 }
 
-// Run one coalesced read batch on worker w.  Called with the epoch lock held shared until the
-// device work is enqueued.
-// HM_SERVER_TRACE=1 in the environment: report every worker step slower than 2 ms on stderr.
-bool trace_on() {
-  static const bool on = [] {
+// HM_SERVER_TRACE=1 prints every step slower than 2 ms; HM_SERVER_TRACE=<n> (n > 1) every step
+// slower than n microseconds.  Times are since the first traced step.
+double trace_threshold_us() {
+  static const double th = [] {
     const char* e = std::getenv("HM_SERVER_TRACE");
-    return e && *e && *e != '0';
+    if (!e || !*e || *e == '0') return -1.0;
+    const double v = std::atof(e);
+    return v > 1.0 ? v : 2000.0;
   }();
-  return on;
+  return th;
 }
+bool trace_on() { return trace_threshold_us() > 0; }
 void trace_step(const char* what, Clock::time_point a, Clock::time_point b, uint64_t size) {
+  static const Clock::time_point t0 = Clock::now();
   const double us = std::chrono::duration<double, std::micro>(b - a).count();
-  if (us > 2000.0)
-    std::fprintf(stderr, "[hm_server] %s took %.0f us (size %llu)\n", what, us, (unsigned long long)size);
+  if (us > trace_threshold_us())
+    std::fprintf(stderr, "[hm_server] t=%.3f ms %s took %.0f us (size %llu)\n",
+                 std::chrono::duration<double, std::milli>(a - t0).count(), what, us, (unsigned long long)size);
 }
 
+// Run one coalesced read batch on worker w.  Called with the epoch lock held shared until the
+// device work is enqueued.
 int run_reads(hm_server* sv, Worker& w, std::vector<ReqPtr>& lookups, std::vector<ReqPtr>& ranges,
               std::shared_lock<std::shared_mutex>& epoch_lock, uint64_t* epoch) {
   hm_tree* t = sv->t;
@@ -191,6 +200,7 @@ int run_reads(hm_server* sv, Worker& w, std::vector<ReqPtr>& lookups, std::vecto
   uint64_t nr = 0, rtot = 0;
   for (auto& r : ranges) nr += r->n, rtot += r->range_total;
   int rc = HM_OK;
+  const auto t_pack = Clock::now();
   // pack the inputs into pinned staging first: host copies do not hold the epoch lock
   if (nq) {
     if ((rc = w.hq.ensure(nq * 8)) || (rc = w.hv.ensure(nq * 8)) || (rc = w.hf.ensure(nq))) return rc;
@@ -213,8 +223,11 @@ int run_reads(hm_server* sv, Worker& w, std::vector<ReqPtr>& lookups, std::vecto
       o += r->n;
     }
   }
+  if (trace_on()) trace_step("read pack", t_pack, Clock::now(), nq + rtot);
   // then enqueue the device work under the shared epoch lock: the batch sees one whole epoch
+  const auto t_lock = Clock::now();
   epoch_lock.lock();
+  if (trace_on()) trace_step("read wait for the epoch lock", t_lock, Clock::now(), nq + rtot);
   const auto t_enq = Clock::now();
   *epoch = tree_epoch(t);
   if (nq) {
@@ -240,6 +253,7 @@ int run_reads(hm_server* sv, Worker& w, std::vector<ReqPtr>& lookups, std::vecto
     trace_step("read device work", t_sync, Clock::now(), nq + rtot);
   }
   // hand results back
+  const auto t_scatter = Clock::now();
   uint64_t o = 0;
   for (auto& r : lookups) {
     const size_t n = r->n;
@@ -256,6 +270,7 @@ int run_reads(hm_server* sv, Worker& w, std::vector<ReqPtr>& lookups, std::vecto
     ro += n;
     rk += r->range_total;
   }
+  if (trace_on()) trace_step("read scatter", t_scatter, Clock::now(), nq + rtot);
   return HM_OK;
 }
 
@@ -315,6 +330,9 @@ int reserve_worker(hm_server* sv, Worker& w) {
   return reserve_workspace(sv->t, w.s, ws);
 }
 
+// A backlog is split among idle workers down to batches of this many queries.
+constexpr uint64_t kMinShare = 1u << 15;
+
 // The first pending read, or nullptr.
 Request* first_read(hm_server* sv) {
   for (auto& r : sv->pending)
@@ -361,7 +379,14 @@ void worker_loop(hm_server* sv, Worker* w) {
         sv->pending.swap(rest);
         sv->updating = true;
       } else {
-        // every pending read that fits the batch and memory limits
+        // every pending read that fits the batch and memory limits -- but a backlog is shared with
+        // the workers still idle: each worker packs and unpacks its batch on the host, and one
+        // worker taking it all would serialize that work (and grow the next backlog)
+        uint64_t pending_q = 0;
+        for (auto& h : sv->pending)
+          if (h->type != REQ_UPDATE) pending_q += h->n + h->range_total;
+        const uint64_t share = (pending_q + sv->idle) / (sv->idle + 1);
+        const uint64_t cap = std::min<uint64_t>(sv->p.max_batch, std::max<uint64_t>(share, kMinShare));
         uint64_t nq = 0;
         std::deque<ReqPtr> rest;
         bool full = false;
@@ -373,7 +398,7 @@ void worker_loop(hm_server* sv, Worker* w) {
           const uint64_t b = request_bytes(*h);
           const uint64_t m = h->n + h->range_total;
           if ((!lookups.empty() || !ranges.empty()) &&
-              (nq + m > sv->p.max_batch || sv->inflight_bytes + bytes + b > sv->p.mem_budget)) {
+              (nq + m > cap || sv->inflight_bytes + bytes + b > sv->p.mem_budget)) {
             full = true;  // keep arrival order among the reads left behind
             rest.push_back(h);
             continue;
@@ -403,6 +428,12 @@ void worker_loop(hm_server* sv, Worker* w) {
       complete_all(sv, ups, status.data(), HM_OK, epoch);
       sv->cv_work.notify_all();
     } else {
+      if (trace_on()) {
+        Clock::time_point oldest = tb;
+        for (auto* v : {&lookups, &ranges})
+          for (auto& r : *v) oldest = std::min(oldest, r->t_submit);
+        trace_step("queue wait of a batch's oldest request", oldest, tb, lookups.size() + ranges.size());
+      }
       const int rc = run_reads(sv, *w, lookups, ranges, epoch_lock, &epoch);
       if (epoch_lock.owns_lock()) epoch_lock.unlock();
       {
@@ -565,6 +596,14 @@ int hm_server_get_stats(hm_server* s, hm_server_stats* st) {
     st->max_us = v.back();
     st->span_s = std::chrono::duration<double>(s->last_complete - s->first_submit).count();
   }
+  double* p99_kind[3] = {&st->p99_lookup_us, &st->p99_range_us, &st->p99_update_us};
+  for (int k = 0; k < 3; ++k) {
+    *p99_kind[k] = 0;
+    if (s->lat_kind_us[k].empty()) continue;
+    std::vector<double> v = s->lat_kind_us[k];
+    std::sort(v.begin(), v.end());
+    *p99_kind[k] = v[std::min(v.size() - 1, (size_t)(0.99 * (double)(v.size() - 1) + 0.5))];
+  }
   st->max_read_batch_us = s->max_read_batch_us;
   st->max_update_us = s->max_update_us;
   st->max_prepare_us = s->max_prepare_us;
@@ -576,6 +615,7 @@ int hm_server_reset_stats(hm_server* s) {
   if (!s) return set_err(HM_EINVAL, "server is NULL");
   std::lock_guard<std::mutex> g(s->mu);
   s->lat_us.clear();
+  for (auto& v : s->lat_kind_us) v.clear();
   s->n_lookups = s->n_ranges = s->n_updates = s->n_queries = s->n_rejected = s->n_launches = s->n_epochs = 0;
   s->max_read_batch_us = s->max_update_us = s->max_prepare_us = s->max_commit_wait_us = 0;
   s->have_first = false;

@@ -197,6 +197,7 @@ typedef struct hm_server_stats {
   /* worker-side durations (longest since the last reset): one read batch from taking it to its
    * results written; one update epoch, its prepare phase, and the wait for the exclusive commit */
   double max_read_batch_us, max_update_us, max_prepare_us, max_commit_wait_us;
+  double p99_lookup_us, p99_range_us, p99_update_us; /* request latency p99 by kind */
 } hm_server_stats;
 int hm_server_create(hm_tree* t, const hm_server_params* p, hm_server** out);
 /* Requests: every buffer (inputs and outputs, host memory) must stay valid and unchanged until the

@@ -40,6 +40,7 @@ def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(_lib.HmParams) == 48
     assert ctypes.sizeof(_lib.HmSearchOpts) == 32
     assert ctypes.sizeof(_lib.HmStats) == 8 * 3 + 4 * 4 + 8 * 17 + 8 * 6
+    assert ctypes.sizeof(_lib.HmServerStats) == 8 * 8 + 8 * 5 + 8 * 4 + 8 * 3
 
 
 def test_params_defaults():
