@@ -13,7 +13,16 @@
 namespace hm {
 namespace {
 
-__global__ void validate_sorted_kernel(const uint64_t* __restrict__ k, size_t n, uint32_t* __restrict__ flags) {
+// Both reductions below end in atomics on one address, which the memory system serializes (~10 ns
+// each): every block reduces in LDS first and the grid is capped (kReduceBlocks), so a launch issues
+// at most a few thousand of them.
+constexpr int kReduceBlocks = 1024;
+
+__global__ __launch_bounds__(256) void validate_sorted_kernel(const uint64_t* __restrict__ k, size_t n,
+                                                              uint32_t* __restrict__ flags) {
+  __shared__ uint32_t block_f;
+  if (threadIdx.x == 0) block_f = 0;
+  __syncthreads();
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   uint32_t f = 0;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -21,14 +30,21 @@ __global__ void validate_sorted_kernel(const uint64_t* __restrict__ k, size_t n,
     if (x == KEY_MAX) f |= 1u;
     if (i > 0 && k[i - 1] >= x) f |= 2u;
   }
-  // one atomic per wave: unsorted input sets a flag in nearly every lane, and atomics on one
-  // address serialize
   const uint64_t b1 = __ballot((f & 1u) != 0), b2 = __ballot((f & 2u) != 0);
   const uint32_t w = (b1 ? 1u : 0u) | (b2 ? 2u : 0u);
-  if ((threadIdx.x & 63) == 0 && w) atomicOr(flags, w);
+  if ((threadIdx.x & 63) == 0 && w) atomicOr(&block_f, w);
+  __syncthreads();
+  if (threadIdx.x == 0 && block_f) atomicOr(flags, block_f);
 }
 
-__global__ void key_bits_kernel(const uint64_t* __restrict__ k, size_t n, uint64_t* __restrict__ orand) {
+__global__ __launch_bounds__(256) void key_bits_kernel(const uint64_t* __restrict__ k, size_t n,
+                                                       uint64_t* __restrict__ orand) {
+  __shared__ unsigned long long block_o, block_a;
+  if (threadIdx.x == 0) {
+    block_o = 0ull;
+    block_a = ~0ull;
+  }
+  __syncthreads();
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   uint64_t o = 0, a = ~0ull;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
@@ -36,18 +52,21 @@ __global__ void key_bits_kernel(const uint64_t* __restrict__ k, size_t n, uint64
     o |= x;
     a &= x;
   }
-  // wave reduce then one atomic per wave
   for (int off = 32; off > 0; off >>= 1) {
     o |= __shfl_xor(o, off, 64);
     a &= __shfl_xor(a, off, 64);
   }
   if ((threadIdx.x & 63) == 0) {
-    atomicOr((unsigned long long*)&orand[0], (unsigned long long)o);
-    atomicAnd((unsigned long long*)&orand[1], (unsigned long long)a);
+    atomicOr(&block_o, (unsigned long long)o);
+    atomicAnd(&block_a, (unsigned long long)a);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicOr((unsigned long long*)&orand[0], block_o);
+    atomicAnd((unsigned long long*)&orand[1], block_a);
   }
 }
 
-// One thread per leaf slot: leaf = t / 8, slot = t % 8.
 __global__ void fill_leaves_kernel(const uint64_t* __restrict__ keys, const uint64_t* __restrict__ vals, size_t n,
                                    int fill, Leaf* __restrict__ leaves, size_t n_leaves) {
   const size_t total = n_leaves * LEAF_KEYS;
@@ -110,7 +129,9 @@ int grid_for(size_t work, int threads) {
 hipError_t validate_sorted(const uint64_t* keys, size_t n, uint32_t* d_flags, hipStream_t s) {
   if (n == 0) return hipSuccess;
   ProfScope prof("build", s);
-  hipLaunchKernelGGL(validate_sorted_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, keys, n, d_flags);
+  int g = grid_for(n, 256);
+  if (g > kReduceBlocks) g = kReduceBlocks;
+  hipLaunchKernelGGL(validate_sorted_kernel, dim3(g), dim3(256), 0, s, keys, n, d_flags);
   return hipGetLastError();
 }
 
@@ -118,7 +139,7 @@ hipError_t key_bit_summary(const uint64_t* keys, size_t n, uint64_t* d_orand, hi
   if (n == 0) return hipSuccess;
   ProfScope prof("build", s);
   int g = grid_for(n, 256);
-  if (g > 4096) g = 4096;
+  if (g > kReduceBlocks) g = kReduceBlocks;
   hipLaunchKernelGGL(key_bits_kernel, dim3(g), dim3(256), 0, s, keys, n, d_orand);
   return hipGetLastError();
 }

@@ -305,8 +305,8 @@ __device__ __forceinline__ bool run_head(const uint32_t* __restrict__ leaf, uint
 __global__ void plan_runs_kernel(const Leaf* __restrict__ leaves, const uint32_t* __restrict__ leaf,
                                  const uint8_t* __restrict__ found, size_t n_max, const uint32_t* __restrict__ dm,
                                  uint32_t* __restrict__ summary) {
-  __shared__ uint32_t block_new;
-  if (threadIdx.x == 0) block_new = 0;
+  __shared__ uint32_t block_new, block_over;
+  if (threadIdx.x == 0) block_new = block_over = 0;
   __syncthreads();
   const uint32_t m = dm ? *dm : (uint32_t)n_max;
   const uint32_t stride = gridDim.x * blockDim.x;
@@ -319,7 +319,7 @@ __global__ void plan_runs_kernel(const Leaf* __restrict__ leaves, const uint32_t
       const uint32_t room = (uint32_t)(LEAF_KEYS - leaf_count(leaves[L]));
       uint32_t ins = 0;
       for (uint32_t j = i; j < m && leaf[j] == L && ins <= room; ++j) ins += found[j] ? 0u : 1u;
-      if (ins > room) atomicOr(&summary[0], 1u);
+      if (ins > room) block_over = 1u;  // a benign race: every writer stores 1
     }
   }
   // new-key count: wave sum, then one LDS atomic per wave and one global atomic per block (a
@@ -329,6 +329,7 @@ __global__ void plan_runs_kernel(const Leaf* __restrict__ leaves, const uint32_t
   if ((threadIdx.x & 63) == 0 && my_new) atomicAdd(&block_new, my_new);
   __syncthreads();
   if (threadIdx.x == 0 && block_new) atomicAdd(&summary[1], block_new);
+  if (threadIdx.x == 0 && block_over) atomicOr(&summary[0], 1u);
 }
 
 __global__ void apply_runs_kernel(Leaf* __restrict__ leaves, const uint32_t* __restrict__ leaf,
