@@ -1007,7 +1007,17 @@ int publish_prepare(hm_tree* t, hipStream_t s, PublishPlan* plan) {
   ProfScope prof("relayout", s);
   const uint64_t nl = t->n_leaves;
   const size_t a4 = align256((nl + 1) * 4);
-  rc = ensure_rl(t, 4 * a4 + align256(scan_tmp_bytes(nl + 1)), s);
+  {
+    // size the scratch for the counts and the inner rebuild at once (growing it later means a
+    // device-wide free and a copy): a touched leaf becomes at most 8 more leaves (fill 1), so
+    // nl + 8 * n_new bounds the new leaf count
+    const uint64_t bound = nl + 8 * (uint64_t)plan->n_new;
+    uint32_t ls[MAX_LEVELS + 1], ni = 0;
+    const int h = plan_levels(bound, t->F, ls, &ni);
+    const size_t counts = align256(4 * a4 + align256(scan_tmp_bytes(nl + 1)));
+    rc = ensure_rl(t, counts + (h >= 0 ? inner_scratch_bytes(bound, t->F, ni) : 0), s);
+    if (rc == HM_EBUDGET) rc = ensure_rl(t, counts, s);  // the bound is loose: try the exact size later
+  }
   if (rc) return rc;
   char* cur = (char*)t->rl_buf;
   uint32_t* ins = (uint32_t*)cur;
@@ -1039,7 +1049,7 @@ int publish_prepare(hm_tree* t, hipStream_t s, PublishPlan* plan) {
       t->device_bytes() + grow + (ib_need > t->rl_bytes ? ib_need - t->rl_bytes : 0) > t->p.mem_budget)
     return set_err(HM_EBUDGET, "re-layout to %llu leaves exceeds the device-memory budget",
                    (unsigned long long)new_leaves);
-  if (ib_need > t->rl_bytes) {  // keep the counts: grow into a new buffer and copy them over
+  if (ib_need > t->rl_bytes) {  // not expected (sized above); keep the counts: grow and copy them over
     void* nb = nullptr;
     HM_TRY(hipMalloc(&nb, ib_need));
     HM_TRY(copy_words((uint64_t*)nb, (const uint64_t*)t->rl_buf, ib_off / 8, s));

@@ -16,7 +16,8 @@ namespace {
 // Both reductions below end in atomics on one address, which the memory system serializes (~10 ns
 // each): every block reduces in LDS first and the grid is capped (kReduceBlocks), so a launch issues
 // at most a few thousand of them.
-constexpr int kReduceBlocks = 1024;
+constexpr int kReduceBlocks = 4096;
+constexpr int kUnroll = 4;  // independent loads in flight per thread
 
 __global__ __launch_bounds__(256) void validate_sorted_kernel(const uint64_t* __restrict__ k, size_t n,
                                                               uint32_t* __restrict__ flags) {
@@ -25,10 +26,22 @@ __global__ __launch_bounds__(256) void validate_sorted_kernel(const uint64_t* __
   __syncthreads();
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   uint32_t f = 0;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const uint64_t x = k[i];
-    if (x == KEY_MAX) f |= 1u;
-    if (i > 0 && k[i - 1] >= x) f |= 2u;
+  for (size_t b = (size_t)blockIdx.x * blockDim.x + threadIdx.x; b < n; b += kUnroll * stride) {
+    uint64_t x[kUnroll], y[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const size_t i = b + u * stride;
+      x[u] = i < n ? k[i] : 0ull;
+      y[u] = (i < n && i > 0) ? k[i - 1] : 0ull;
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const size_t i = b + u * stride;
+      if (i < n) {
+        if (x[u] == KEY_MAX) f |= 1u;
+        if (i > 0 && y[u] >= x[u]) f |= 2u;
+      }
+    }
   }
   const uint64_t b1 = __ballot((f & 1u) != 0), b2 = __ballot((f & 2u) != 0);
   const uint32_t w = (b1 ? 1u : 0u) | (b2 ? 2u : 0u);
@@ -47,10 +60,18 @@ __global__ __launch_bounds__(256) void key_bits_kernel(const uint64_t* __restric
   __syncthreads();
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   uint64_t o = 0, a = ~0ull;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    const uint64_t x = k[i];
-    o |= x;
-    a &= x;
+  for (size_t b = (size_t)blockIdx.x * blockDim.x + threadIdx.x; b < n; b += kUnroll * stride) {
+    uint64_t x[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const size_t i = b + u * stride;
+      x[u] = i < n ? k[i] : k[b];  // k[b] leaves the or / and unchanged
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      o |= x[u];
+      a &= x[u];
+    }
   }
   for (int off = 32; off > 0; off >>= 1) {
     o |= __shfl_xor(o, off, 64);
