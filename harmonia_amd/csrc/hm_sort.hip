@@ -176,31 +176,41 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(const K* __rest
 }
 
 // Undo one recorded pass for per-element results: (vin, fin) are in the pass's output order,
-// (vout, fout) receive them in its input order.
-__global__ __launch_bounds__(kThreads) void radix_unscatter_kernel(const uint64_t* __restrict__ vin,
-                                                                   const uint8_t* __restrict__ fin,
-                                                                   uint64_t* __restrict__ vout,
-                                                                   uint8_t* __restrict__ fout,
-                                                                   const uint16_t* __restrict__ lp, size_t n,
-                                                                   const uint32_t* __restrict__ offsets, uint32_t nb,
-                                                                   const uint32_t* __restrict__ dn) {
+// (vout, fout) receive them in its input order.  A tile takes 512 threads x 8 items (the forward
+// passes use 256 x 16): the tile's LDS staging (36 KiB) caps a CU at 4 blocks, and the gathers below
+// are latency bound, so twice the waves per block keep twice the loads in flight.
+constexpr int kUThreads = 512;
+constexpr int kUItems = kTile / kUThreads;
+
+__global__ __launch_bounds__(kUThreads) void radix_unscatter_kernel(const uint64_t* __restrict__ vin,
+                                                                    const uint8_t* __restrict__ fin,
+                                                                    uint64_t* __restrict__ vout,
+                                                                    uint8_t* __restrict__ fout,
+                                                                    const uint16_t* __restrict__ lp, size_t n,
+                                                                    const uint32_t* __restrict__ offsets, uint32_t nb,
+                                                                    const uint32_t* __restrict__ dn) {
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   if (dn) n = *dn;
   if (tile >= nb || (size_t)tile * kTile >= n) return;  // block-uniform
   __shared__ uint32_t goff[256];
   __shared__ uint32_t boff[257];
-  __shared__ uint32_t scan_sh[kWaves + 1];
+  __shared__ uint32_t scan_sh[kUThreads / 64 + 1];
   __shared__ uint64_t sv[kTile];
   __shared__ uint8_t sf[kTile];
   {
     // this tile's count of digit d = next scanned offset - this one (digit-major, exclusive scan)
     const uint32_t d = threadIdx.x;
-    const size_t idx = (size_t)d * nb + tile;
-    const uint32_t g = offsets[idx];
-    const uint32_t next = idx + 1 < (size_t)256 * nb ? offsets[idx + 1] : (uint32_t)n;
-    goff[d] = g;
+    uint32_t c = 0;
+    if (d < 256) {
+      const size_t idx = (size_t)d * nb + tile;
+      const uint32_t g = offsets[idx];
+      const uint32_t next = idx + 1 < (size_t)256 * nb ? offsets[idx + 1] : (uint32_t)n;
+      goff[d] = g;
+      c = next - g;
+    }
     uint32_t total;
-    boff[d] = block_exclusive_scan<uint32_t, kThreads>(next - g, scan_sh, total);
+    const uint32_t e = block_exclusive_scan<uint32_t, kUThreads>(c, scan_sh, total);
+    if (d < 256) boff[d] = e;
     if (d == 0) boff[256] = total;
   }
   __syncthreads();
@@ -209,11 +219,11 @@ __global__ __launch_bounds__(kThreads) void radix_unscatter_kernel(const uint64_
   // gather the tile's digit runs in digit order (consecutive i of one digit read consecutive
   // addresses); the digit of staging index i is the last d with boff[d] <= i.  Every load of the
   // tile is issued before the first one is used, so their latencies overlap.
-  uint32_t src[kItems];
-  uint16_t lpv[kItems];
+  uint32_t src[kUItems];
+  uint16_t lpv[kUItems];
 #pragma unroll
-  for (int r = 0; r < kItems; ++r) {
-    const uint32_t i = threadIdx.x + r * kThreads;
+  for (int r = 0; r < kUItems; ++r) {
+    const uint32_t i = threadIdx.x + r * kUThreads;
     uint32_t lo = 0;
 #pragma unroll
     for (uint32_t step = 128; step; step >>= 1)
@@ -221,17 +231,17 @@ __global__ __launch_bounds__(kThreads) void radix_unscatter_kernel(const uint64_
     src[r] = goff[lo] + (i - boff[lo]);
     lpv[r] = i < cnt ? lp[tile0 + i] : (uint16_t)0;
   }
-  uint64_t v[kItems];
-  uint8_t f[kItems];
+  uint64_t v[kUItems];
+  uint8_t f[kUItems];
 #pragma unroll
-  for (int r = 0; r < kItems; ++r) {
-    const uint32_t i = threadIdx.x + r * kThreads;
+  for (int r = 0; r < kUItems; ++r) {
+    const uint32_t i = threadIdx.x + r * kUThreads;
     v[r] = i < cnt ? vin[src[r]] : 0ull;
     f[r] = i < cnt ? fin[src[r]] : (uint8_t)0;
   }
 #pragma unroll
-  for (int r = 0; r < kItems; ++r) {
-    const uint32_t i = threadIdx.x + r * kThreads;
+  for (int r = 0; r < kUItems; ++r) {
+    const uint32_t i = threadIdx.x + r * kUThreads;
     if (i < cnt) {
       sv[i] = v[r];
       sf[i] = f[r];
@@ -239,8 +249,8 @@ __global__ __launch_bounds__(kThreads) void radix_unscatter_kernel(const uint64_
   }
   __syncthreads();
 #pragma unroll
-  for (int r = 0; r < kItems; ++r) {
-    const uint32_t j = threadIdx.x + r * kThreads;
+  for (int r = 0; r < kUItems; ++r) {
+    const uint32_t j = threadIdx.x + r * kUThreads;
     if (j < cnt) {
       vout[tile0 + j] = sv[lpv[r]];
       fout[tile0 + j] = sf[lpv[r]];
@@ -384,7 +394,7 @@ hipError_t unsort_results(uint64_t* vals, uint8_t* found, size_t n, int bits, co
       dv = vals, df = found;
     }
     ProfScope prof("unsort", s);
-    hipLaunchKernelGGL(radix_unscatter_kernel, dim3(grid_for(nb)), dim3(kThreads), 0, s, sv, sf, dv, df,
+    hipLaunchKernelGGL(radix_unscatter_kernel, dim3(grid_for(nb)), dim3(kUThreads), 0, s, sv, sf, dv, df,
                        reinterpret_cast<const uint16_t*>(rec + counts_bytes(n)), n,
                        reinterpret_cast<const uint32_t*>(rec), (uint32_t)nb, dn);
     sv = dv;
