@@ -507,6 +507,25 @@ int hm_server_create(hm_tree* t, const hm_server_params* p, hm_server** out) {
       return rc;
     }
   }
+  // run a one-key lookup and a one-key range now: the first launch of a kernel in a process loads
+  // the code object (milliseconds), which would otherwise land on the first requests
+  {
+    Worker& w = *sv->workers[0];
+    uint64_t* hq = (uint64_t*)w.hq.p;
+    uint32_t* hlen = (uint32_t*)((char*)w.hl.p + 8);
+    hq[0] = 0;
+    ((uint64_t*)w.hl.p)[0] = 0;
+    hlen[0] = 1;
+    int rc = search_impl(t, hq, 1, (uint64_t*)w.hv.p, (uint8_t*)w.hf.p, nullptr, w.s, true);
+    if (!rc)
+      rc = range_impl(t, (const uint64_t*)w.hl.p, hlen, 1, (uint64_t*)w.doff.p, (uint64_t*)w.hk.p,
+                      (uint64_t*)w.hk.p + 1, (uint32_t*)w.hc.p, 1, w.s, true);
+    if (!rc && hipStreamSynchronize(w.s) != hipSuccess) rc = set_err(HM_EDEVICE, "warm-up failed");
+    if (rc) {
+      hm_server_destroy(sv);
+      return rc;
+    }
+  }
   for (auto& w : sv->workers) w->th = std::thread(worker_loop, sv, w.get());
   *out = sv;
   return HM_OK;
