@@ -5,6 +5,19 @@
 
 namespace hm {
 
+// Lanes (among `valid`) of this wave whose 8-bit digit equals this lane's digit: eight ballots,
+// one per digit bit.  Every lane of the wave must execute it.
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t valid) {
+  uint64_t peers = valid;
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const bool bit = (d >> b) & 1u;
+    const uint64_t m = __ballot(bit);
+    peers &= bit ? m : ~m;
+  }
+  return peers;
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_inclusive_scan(T x) {
   const int lane = threadIdx.x & 63;

@@ -67,10 +67,38 @@ __global__ __launch_bounds__(kThreads) void scan_down_kernel(const Tin* __restri
   }
 }
 
+// Small inputs (n <= kSingleMax): one block of 1024 threads, each scanning a contiguous chunk --
+// one launch instead of three (the radix sorts of update batches and range batches scan a few
+// thousand counters at a time, where launches dominate).
+constexpr int kSingleThreads = 1024;
+constexpr size_t kSingleMax = 65536;
+
+template <typename Tin, typename T>
+__global__ __launch_bounds__(kSingleThreads) void scan_single_kernel(const Tin* __restrict__ in, T* __restrict__ out,
+                                                                     size_t n, T init) {
+  __shared__ T sh[kSingleThreads / 64 + 1];
+  const size_t chunk = (n + kSingleThreads - 1) / kSingleThreads;
+  const size_t b = (size_t)threadIdx.x * chunk;
+  const size_t e = b + chunk < n ? b + chunk : n;
+  T s = 0;
+  for (size_t i = b; i < e; ++i) s += (T)in[i];
+  T total;
+  T run = block_exclusive_scan<T, kSingleThreads>(s, sh, total) + init;
+  for (size_t i = b; i < e; ++i) {
+    const T v = (T)in[i];  // read before the write: in and out may be the same array
+    out[i] = run;
+    run += v;
+  }
+}
+
 template <typename Tin, typename T>
 hipError_t scan_impl(const Tin* in, T* out, size_t n, T init, void* tmp, hipStream_t s) {
   if (n == 0) return hipSuccess;
   ProfScope prof("scan", s);
+  if (n <= kSingleMax) {
+    hipLaunchKernelGGL((scan_single_kernel<Tin, T>), dim3(1), dim3(kSingleThreads), 0, s, in, out, n, init);
+    return hipGetLastError();
+  }
   const size_t nb = (n + kTile - 1) / kTile;
   T* partials = reinterpret_cast<T*>(tmp);
   hipLaunchKernelGGL((scan_reduce_kernel<Tin, T>), dim3((unsigned)nb), dim3(kThreads), 0, s, in, n, partials);

@@ -40,18 +40,6 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t grid) { return
 unsigned grid_for(size_t nb) { return (unsigned)((nb + 7) / 8 * 8); }
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
-// Lanes (among `valid`) whose digit equals this lane's digit.
-__device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t valid) {
-  uint64_t peers = valid;
-#pragma unroll
-  for (int b = 0; b < 8; ++b) {
-    const bool bit = (d >> b) & 1u;
-    const uint64_t m = __ballot(bit);
-    peers &= bit ? m : ~m;
-  }
-  return peers;
-}
-
 template <typename K>
 __global__ __launch_bounds__(kThreads) void radix_hist_kernel(const K* __restrict__ keys, size_t n, int shift,
                                                               uint32_t mask, uint32_t* __restrict__ counts,
