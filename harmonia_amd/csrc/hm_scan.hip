@@ -71,7 +71,7 @@ __global__ __launch_bounds__(kThreads) void scan_down_kernel(const Tin* __restri
 // one launch instead of three (the radix sorts of update batches and range batches scan a few
 // thousand counters at a time, where launches dominate).
 constexpr int kSingleThreads = 1024;
-constexpr size_t kSingleMax = 65536;
+constexpr size_t kSingleMax = 16384;
 
 template <typename Tin, typename T>
 __global__ __launch_bounds__(kSingleThreads) void scan_single_kernel(const Tin* __restrict__ in, T* __restrict__ out,
