@@ -348,9 +348,11 @@ def batched_updates(dev, lib):
     g = W.rng(seed, 31)
     out = []
     step = 0
-    # (batch size, share of existing keys, reported): the first two warm the scratch buffers up
-    plan = [(1 << 16, 0.7425, False), (1 << 20, 0.7425, False), (1 << 16, 1.0, True), (1 << 20, 1.0, True),
-            (1 << 16, 0.7425, True), (1 << 20, 0.7425, True), (1 << 20, 0.7425, True)]
+    # (batch size, share of existing keys, reported): the first two warm the scratch buffers up; each
+    # reported configuration runs three epochs (fresh keys each) and reports the median
+    plan = [(1 << 16, 0.7425, False), (1 << 20, 0.7425, False)]
+    for cfg in ((1 << 16, 1.0), (1 << 20, 1.0), (1 << 16, 0.7425), (1 << 20, 0.7425)):
+        plan += [(cfg[0], cfg[1], True)] * 3
     for batch, share, reported in plan:
         n_old = int(round(batch * share))
         old = cur[g.choice(len(cur), n_old, replace=False)]
@@ -382,6 +384,13 @@ def batched_updates(dev, lib):
         if reported:
             out.append(r)
         step += 1
+    batches = []
+    for i in range(0, len(out), 3):  # median epoch of each configuration (by time), with the spread
+        reps = sorted(out[i:i + 3], key=lambda r: r["ms"])
+        med = dict(reps[1])
+        med["ms_min_max"] = [reps[0]["ms"], reps[-1]["ms"]]
+        med["relaid_out_epochs"] = sum(1 for r in reps if r["relaid_out"])
+        batches.append(med)
     v, f = tree.search_batch(bk, psa_bits=0)
     ok = bool(f.all()) and np.array_equal(v, bv) and tree.stats().n_keys == len(cur)
     tree.close()
@@ -389,9 +398,9 @@ def batched_updates(dev, lib):
         log("UPDATE RESULT CHECK FAILED")
         sys.exit(1)
     return {"workload": "2^26-key fanout-64 tree at fill 6; upsert batches of 2^16 / 2^20 keys, all existing "
-                        "(values only, in place) or 74.25% existing, one epoch each, device buffers, after one "
-                        "warm-up batch of each size",
-            "batches": out, "results_ok": ok}
+                        "(values only, in place) or 74.25% existing, one epoch per batch, device buffers, after "
+                        "one warm-up batch of each size; median of 3 epochs per configuration",
+            "batches": batches, "results_ok": ok}
 
 
 def served(args):
@@ -569,7 +578,7 @@ def main():
             traffic = None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
-                "kernel": "search_kernel (PSA-sorted queries, 2^24 per launch)",
+                "kernel": "search_shared_kernel<64,OUT_DIRECT> (PSA-sorted queries, 2^24 per launch)",
                 "bytes_per_query": BYTES_PER_QUERY}
     e2e_achieved = BYTES_PER_QUERY_PSA * n_q / (head["ms_per_step"] * 1e-3) / 1e9
 

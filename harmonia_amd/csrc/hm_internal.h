@@ -112,7 +112,7 @@ struct SearchLaunch {
   int lds_levels = 2;   // levels staged in LDS
   int blocks_per_cu = 8;
   int queries_per_group = 2;  // independent descents interleaved per group
-  bool sorted = false;        // queries are PSA-sorted: with group 1, waves share node reads
+  bool sorted = false;        // queries are PSA-sorted: waves share node reads (group 1, or group 2 x 4)
 };
 // Where a search writes result i of its (possibly PSA-sorted) query array:
 enum SearchOut {

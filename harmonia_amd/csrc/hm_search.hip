@@ -20,13 +20,13 @@ size_t sorted_lds_bytes(const TreeView& tv, int levels, int Q) {
   return (top_words + (size_t)(kThreads / 64) * Q * node_slots(tv.fanout)) * sizeof(uint64_t);
 }
 
-template <int F, int G, int Q, int OUT>
-__global__ __launch_bounds__(kThreads) void search_kernel(TreeView tv, int lds_levels,
-                                                          const uint64_t* __restrict__ q,
-                                                          const uint32_t* __restrict__ pos,
-                                                          uint64_t* __restrict__ out_vals,
-                                                          uint8_t* __restrict__ out_found, uint32_t n,
-                                                          const uint32_t* __restrict__ dn) {
+// SHARED: the queries are PSA-sorted and the upper global levels go through shared_node_level
+// (wave buffers after the staged top levels, sized by sorted_lds_bytes).
+template <int F, int G, int Q, int OUT, bool SHARED>
+__device__ __forceinline__ void search_body(const TreeView& tv, int lds_levels, const uint64_t* __restrict__ q,
+                                            const uint32_t* __restrict__ pos, uint64_t* __restrict__ out_vals,
+                                            uint8_t* __restrict__ out_found, uint32_t n,
+                                            const uint32_t* __restrict__ dn) {
   extern __shared__ __align__(16) uint64_t smem[];
   uint32_t* spre;
   stage_top_levels<F>(tv, lds_levels, smem, spre);
@@ -48,6 +48,13 @@ __global__ __launch_bounds__(kThreads) void search_kernel(TreeView tv, int lds_l
   const uint32_t vb = xcd_block(blockIdx.x, gridDim.x);
   const uint32_t begin = vb * chunk;
   const uint32_t end = begin + chunk < n ? begin + chunk : n;
+  uint64_t* wslot = nullptr;
+  if constexpr (SHARED) {
+    constexpr int NS = node_slots(F);
+    const uint32_t nl = lds_levels > 0 ? tv.level_start[lds_levels] : 0u;
+    const uint32_t top_words = (nl * NS + (nl + 1) / 2 + 1) & ~1u;  // 16-byte aligned
+    wslot = smem + top_words + (threadIdx.x >> 6) * (Q * NS);
+  }
 
   for (uint32_t qb = begin; qb < end; qb += tile) {  // uniform across the block
     uint64_t t[Q], td[Q];
@@ -58,7 +65,10 @@ __global__ __launch_bounds__(kThreads) void search_kernel(TreeView tv, int lds_l
       t[i] = q[qi < end ? qi : end - 1];  // branch-free: lanes past the end repeat the last query
       td[i] = t[i] == KEY_MAX ? KEY_MAX - 1 : t[i];
     }
-    descend_q<F, G, Q>(tv, lds_levels, smem, spre, td, leaf, gl, gbase);
+    if constexpr (SHARED)
+      descend_shared<F, G, Q>(tv, lds_levels, smem, spre, wslot, td, leaf, gl, gbase, threadIdx.x & 63);
+    else
+      descend_q<F, G, Q>(tv, lds_levels, smem, spre, td, leaf, gl, gbase);
     uint64_t k[Q][W], v[Q][W];
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
@@ -93,6 +103,26 @@ __global__ __launch_bounds__(kThreads) void search_kernel(TreeView tv, int lds_l
       }
     }
   }
+}
+
+template <int F, int G, int Q, int OUT>
+__global__ __launch_bounds__(kThreads) void search_kernel(TreeView tv, int lds_levels,
+                                                          const uint64_t* __restrict__ q,
+                                                          const uint32_t* __restrict__ pos,
+                                                          uint64_t* __restrict__ out_vals,
+                                                          uint8_t* __restrict__ out_found, uint32_t n,
+                                                          const uint32_t* __restrict__ dn) {
+  search_body<F, G, Q, OUT, false>(tv, lds_levels, q, pos, out_vals, out_found, n, dn);
+}
+
+// PSA-sorted queries, two lanes per query, four queries per group (the shape it was measured faster
+// on; G = 4 and Q = 2 lost).  Five waves per SIMD as the plain kernel of that shape: without the
+// hint the shared levels cost four more VGPRs and a wave of occupancy.
+template <int F, int OUT>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) void search_shared_kernel(
+    TreeView tv, int lds_levels, const uint64_t* __restrict__ q, const uint32_t* __restrict__ pos,
+    uint64_t* __restrict__ out_vals, uint8_t* __restrict__ out_found, uint32_t n, const uint32_t* __restrict__ dn) {
+  search_body<F, 2, 4, OUT, true>(tv, lds_levels, q, pos, out_vals, out_found, n, dn);
 }
 
 // Search over PSA-sorted queries, one lane per query, Q per lane: the global levels go through
@@ -204,6 +234,22 @@ hipError_t launch_fgq(const TreeView& tv, const SearchLaunch& sl, const uint64_t
   size_t need = (n + gpb - 1) / gpb;
   size_t nb = need < max_blocks ? need : max_blocks;
   nb = (nb + 7) / 8 * 8;
+  // PSA-sorted input, two lanes per query, four queries per group: the wave shares the upper global
+  // levels' node reads.  Fanout 128 is left out (its kernel spills at five waves per SIMD).
+  if constexpr (G == 2 && Q == 4 && F <= 64) {
+    if (sl.sorted) {
+      int levels = sl.lds_levels;  // the wave buffers come on top of the staged levels: stay within 64 KiB
+      while (levels > 0 && sorted_lds_bytes(tv, levels, Q) > 65536) --levels;
+      const size_t lds = sorted_lds_bytes(tv, levels, Q);
+      if (mode == OUT_DIRECT)
+        hipLaunchKernelGGL((search_shared_kernel<F, OUT_DIRECT>), dim3((unsigned)nb), dim3(kThreads), lds, s, tv,
+                           levels, q, pos, out_vals, out_found, (uint32_t)n, dn);
+      else
+        hipLaunchKernelGGL((search_shared_kernel<F, OUT_SCATTER>), dim3((unsigned)nb), dim3(kThreads), lds, s, tv,
+                           levels, q, pos, out_vals, out_found, (uint32_t)n, dn);
+      return hipGetLastError();
+    }
+  }
   const size_t lds = lds_bytes_for(tv, sl.lds_levels);
   switch (mode) {
     case OUT_DIRECT:

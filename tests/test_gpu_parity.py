@@ -91,6 +91,23 @@ def test_search_all_launch_shapes(gpu, F):
                         assert np.array_equal(v, ev), (g, qpg, lds, psa, mode)
 
 
+@pytest.mark.parametrize("F", [32, 64])
+def test_search_shared_levels_deep_tree(gpu, F):
+    """PSA-sorted batches with two lanes x four queries share each wave's upper-level node reads.
+    A tall tree runs that over several levels; few PSA bits leave query slots straddling nodes
+    (the per-group fallback), many bits keep them in one node."""
+    t, keys, vals, sk, sv = make(1 << 21 if F == 32 else 1 << 22, F, 8, seed=31 + F)
+    q = queries(sk, 1 << 18, 5 + F)
+    ev, ef = oracle.search_sorted(sk, sv, q)
+    for lds in (0, 1, 2):
+        for bits in (3, 8, 16, 30):
+            for mode in (1, 2):
+                v, f = t.search_batch(q, group_size=2, queries_per_group=4, lds_levels=lds, psa_bits=bits,
+                                      output_mode=mode)
+                assert np.array_equal(f, ef), (lds, bits, mode)
+                assert np.array_equal(v, ev), (lds, bits, mode)
+
+
 @pytest.mark.parametrize("n", [0, 1, 2, 7, 8, 9, 64, 65, 511, 4097])
 def test_search_small_and_ragged(gpu, n):
     t, keys, vals, sk, sv = make(n, 64, 8, seed=n)
@@ -152,7 +169,7 @@ def test_output_modes(gpu, n_q):
     q = queries(sk, n_q, n_q + 1)
     ev, ef = oracle.search_sorted(sk, sv, q)
     for mode in (1, 2):
-        for g, qpg in ((1, 1), (1, 2), (4, 2), (8, 4)):
+        for g, qpg in ((1, 1), (1, 2), (2, 4), (4, 2), (8, 4)):
             for bits in (16, 24):
                 v, f = t.search_batch(q, psa_bits=bits, output_mode=mode, group_size=g, queries_per_group=qpg)
                 assert np.array_equal(f, ef), (mode, g, qpg, bits)
