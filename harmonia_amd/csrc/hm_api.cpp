@@ -206,11 +206,11 @@ int resolve_search(const hm_tree* t, uint64_t n, const hm_search_opts* o, Search
 size_t pipe_bytes(size_t n, int psa_bits, int out_mode) {
   if (!psa_bits) return 0;
   const size_t kq = align256(n * 8), kp = align256(n * 4), kf = align256(n);
-  size_t b = 2 * kq + align256(radix_tmp_bytes(n));
+  size_t b = 2 * kq;
   if (out_mode == HM_OUT_UNPERMUTE)
-    b += align256(replay_bytes(n, psa_bits)) + 2 * (kq + kf);
+    b += align256(replay_bytes(n, psa_bits)) + 2 * (kq + kf);  // the counts live in the replay records
   else
-    b += 2 * kp;
+    b += align256(radix_tmp_bytes(n)) + 2 * kp;
   return b;
 }
 
@@ -220,7 +220,6 @@ PipeBufs carve_pipe(char*& cur, size_t n, int psa_bits, int out_mode) {
   const size_t kq = align256(n * 8), kp = align256(n * 4), kf = align256(n);
   b.k0 = (uint64_t*)cur, cur += kq;
   b.k1 = (uint64_t*)cur, cur += kq;
-  b.tmp = cur, cur += align256(radix_tmp_bytes(n));
   if (out_mode == HM_OUT_UNPERMUTE) {
     b.replay = cur, cur += align256(replay_bytes(n, psa_bits));
     b.rv = (uint64_t*)cur, cur += kq;
@@ -228,6 +227,7 @@ PipeBufs carve_pipe(char*& cur, size_t n, int psa_bits, int out_mode) {
     b.vt = (uint64_t*)cur, cur += kq;
     b.ft = (uint8_t*)cur, cur += kf;
   } else {
+    b.tmp = cur, cur += align256(radix_tmp_bytes(n));
     b.p0 = (uint32_t*)cur, cur += kp;
     b.p1 = (uint32_t*)cur, cur += kp;
   }
@@ -245,7 +245,7 @@ int enqueue_pipeline(const TreeView& tv, const SearchLaunch& sl, int psa_bits, i
   if (out_mode == HM_OUT_UNPERMUTE) {
     // keys-only sort recording its passes; search in PSA order (coalesced result writes); then the
     // passes run backwards to put the results in input order
-    HM_TRY(radix_sort_keys_replay(dq, n, 64 - psa_bits, 64, b.k0, b.k1, b.replay, b.tmp, s, &which, dn));
+    HM_TRY(radix_sort_keys_replay(dq, n, 64 - psa_bits, 64, b.k0, b.k1, b.replay, s, &which, dn));
     const uint64_t* sk = which ? b.k1 : b.k0;
     SearchLaunch ss = sl;
     ss.sorted = true;

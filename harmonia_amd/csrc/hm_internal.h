@@ -79,11 +79,11 @@ hipError_t radix_sort_u64pay(const uint64_t* keys_in, const uint64_t* vals_in, s
                              uint64_t* k0, uint64_t* v0, uint64_t* k1, uint64_t* v1, void* tmp, hipStream_t s,
                              int* which);
 // Keys-only stable LSD sort over bits [begin_bit, end_bit) that records how to undo it: per pass,
-// the scanned (digit, tile) offsets and each element's index in its tile's digit-ordered staging
-// (16 bits).  `replay` must hold replay_bytes(n, end_bit - begin_bit); tmp holds radix_tmp_bytes(n).
+// the per-digit scanned (digit, tile) offsets with the digit totals, and each element's index in its
+// tile's digit-ordered staging (16 bits).  `replay` must hold replay_bytes(n, end_bit - begin_bit).
 size_t replay_bytes(size_t n, int bits);
 hipError_t radix_sort_keys_replay(const uint64_t* keys_in, size_t n, int begin_bit, int end_bit, uint64_t* k0,
-                                  uint64_t* k1, void* replay, void* tmp, hipStream_t s, int* which,
+                                  uint64_t* k1, void* replay, hipStream_t s, int* which,
                                   const uint32_t* dn = nullptr);
 // Route per-element results (vals, found) of the sorted order back to input order by running the
 // recorded passes backwards: each pass gathers its tile's digit runs (coalesced) into LDS and writes

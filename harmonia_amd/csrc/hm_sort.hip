@@ -8,8 +8,9 @@
 // order, so the result equals a stable sort of (key, pos) by key >> begin_bit.  The same kernels
 // sort the bulk-build input (u64 value payload over every varying bit).
 //
-// One pass = histogram kernel (per-tile digit counts, digit-major) -> exclusive scan over
-// (digit, tile) -> scatter kernel (wave-level digit match via 8 ballots gives each item its
+// One pass = histogram kernel (per-tile digit counts, digit-major) -> digit scan (one block per
+// digit: exclusive scan of the digit's counts along the tiles, and the digit's total) -> scatter
+// kernel (the digit starts are the exclusive scan of the 256 totals, done in each block) (wave-level digit match via 8 ballots gives each item its
 // stable rank inside its wave; per-wave counters are scanned across the block; the tile is then
 // staged in LDS in digit order so the global writes of each digit run are coalesced).  Tiles are
 // mapped to blocks so that each XCD processes a contiguous range of tiles: the digit runs of
@@ -40,7 +41,10 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t grid) { return
 unsigned grid_for(size_t nb) { return (unsigned)((nb + 7) / 8 * 8); }
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
-template <typename K>
+// Per-tile digit counts, one block per tile.  VEC: the keys are 16-byte aligned and full tiles are
+// read with 16-byte loads (two keys per lane per load; the counts do not depend on which lane counts
+// which key).  (A persistent grid that loads tile t + 1 while counting tile t measured no faster.)
+template <typename K, bool VEC>
 __global__ __launch_bounds__(kThreads) void radix_hist_kernel(const K* __restrict__ keys, size_t n, int shift,
                                                               uint32_t mask, uint32_t* __restrict__ counts,
                                                               uint32_t nb, const uint32_t* __restrict__ dn) {
@@ -55,18 +59,31 @@ __global__ __launch_bounds__(kThreads) void radix_hist_kernel(const K* __restric
   __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const size_t base = (size_t)tile * kTile + (size_t)w * 64 * kItems;
-  const size_t last = n ? n - 1 : 0;
-  // issue every load of the tile before the first use, so the loads overlap
-  K kk[kItems];
+  if (VEC && (size_t)(tile + 1) * kTile <= n) {  // block-uniform
+    static_assert(sizeof(K) == 8, "16-byte loads take two keys");
+    const ulonglong2* src = reinterpret_cast<const ulonglong2*>(keys + base);
+    ulonglong2 kk[kItems / 2];
 #pragma unroll
-  for (int r = 0; r < kItems; ++r) {
-    const size_t i = base + (size_t)r * 64 + lane;
-    kk[r] = keys[i < last ? i : last];
-  }
+    for (int r = 0; r < kItems / 2; ++r) kk[r] = src[r * 64 + lane];
 #pragma unroll
-  for (int r = 0; r < kItems; ++r) {
-    const size_t i = base + (size_t)r * 64 + lane;
-    if (i < n) atomicAdd(&hist[w][(uint32_t)(kk[r] >> shift) & mask], 1u);
+    for (int r = 0; r < kItems / 2; ++r) {
+      atomicAdd(&hist[w][(uint32_t)(kk[r].x >> shift) & mask], 1u);
+      atomicAdd(&hist[w][(uint32_t)(kk[r].y >> shift) & mask], 1u);
+    }
+  } else {
+    const size_t last = n ? n - 1 : 0;
+    // issue every load of the tile before the first use, so the loads overlap
+    K kk[kItems];
+#pragma unroll
+    for (int r = 0; r < kItems; ++r) {
+      const size_t i = base + (size_t)r * 64 + lane;
+      kk[r] = keys[i < last ? i : last];
+    }
+#pragma unroll
+    for (int r = 0; r < kItems; ++r) {
+      const size_t i = base + (size_t)r * 64 + lane;
+      if (i < n) atomicAdd(&hist[w][(uint32_t)(kk[r] >> shift) & mask], 1u);
+    }
   }
   __syncthreads();
   uint32_t c = 0;
@@ -94,7 +111,14 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(const K* __rest
   __shared__ K skeys[kTile];
   __shared__ V svals[HAS_VAL ? kTile : 1];
   for (int i = threadIdx.x; i < kWaves * 256; i += kThreads) (&wcnt[0][0])[i] = 0;
-  goff[threadIdx.x] = offsets[(size_t)threadIdx.x * nb + tile];
+  {
+    // digit d starts at the digits below it (scan of the totals after the 256 x nb counts) plus its
+    // count in the tiles before this one
+    uint32_t all;
+    const uint32_t dbase = block_exclusive_scan<uint32_t, kThreads>(offsets[(size_t)256 * nb + threadIdx.x], scan_sh,
+                                                                    all);
+    goff[threadIdx.x] = dbase + offsets[(size_t)threadIdx.x * nb + tile];
+  }
   __syncthreads();
 
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -186,16 +210,19 @@ __global__ __launch_bounds__(kUThreads) void radix_unscatter_kernel(const uint64
   __shared__ uint64_t sv[kTile];
   __shared__ uint8_t sf[kTile];
   {
-    // this tile's count of digit d = next scanned offset - this one (digit-major, exclusive scan)
+    // offsets hold, per digit, the exclusive scan of its counts along the tiles, then the 256 digit
+    // totals: this tile's count of digit d = the next tile's offset (or the total) - this one
     const uint32_t d = threadIdx.x;
-    uint32_t c = 0;
+    uint32_t c = 0, g = 0, tot = 0;
     if (d < 256) {
       const size_t idx = (size_t)d * nb + tile;
-      const uint32_t g = offsets[idx];
-      const uint32_t next = idx + 1 < (size_t)256 * nb ? offsets[idx + 1] : (uint32_t)n;
-      goff[d] = g;
-      c = next - g;
+      tot = offsets[(size_t)256 * nb + d];
+      g = offsets[idx];
+      c = (tile + 1 < nb ? offsets[idx + 1] : tot) - g;
     }
+    uint32_t all;
+    const uint32_t dbase = block_exclusive_scan<uint32_t, kUThreads>(tot, scan_sh, all);
+    if (d < 256) goff[d] = dbase + g;
     uint32_t total;
     const uint32_t e = block_exclusive_scan<uint32_t, kUThreads>(c, scan_sh, total);
     if (d < 256) boff[d] = e;
@@ -257,20 +284,72 @@ __global__ void copy_with_pos_kernel(const uint64_t* __restrict__ kin, const V* 
 }
 
 
-// One digit pass from (sk, sv) into (dk, dv).  counts (256 * nb u32) receive the scanned
-// (digit, tile) offsets; scan_tmp holds scan_tmp_bytes(256 * nb).
+// Per digit (one block each): exclusive scan of counts[d * nb .. d * nb + nb) in place (the digit's
+// offsets inside its run, tile by tile); the digit's total goes to counts[256 * nb + d].  One launch
+// in place of a three-kernel scan over all 256 * nb counters: the block-level scan of the 256
+// totals that the digit starts need is done by every scatter / un-scatter block.
+constexpr int kDPer = 16;
+__global__ __launch_bounds__(kThreads) void digit_scan_kernel(uint32_t* __restrict__ counts, uint32_t nb) {
+  __shared__ uint32_t wsum[kWaves];
+  uint32_t* row = counts + (size_t)blockIdx.x * nb;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < nb; base += kThreads * kDPer) {  // uniform across the block
+    // wave w scans its segment of 64 * kDPer counters in order, 64 at a time (coalesced)
+    const uint32_t wb = base + (uint32_t)w * 64 * kDPer;
+    uint32_t v[kDPer], ex[kDPer];
+#pragma unroll
+    for (int i = 0; i < kDPer; ++i) {
+      const uint32_t idx = wb + i * 64 + lane;
+      v[i] = idx < nb ? row[idx] : 0u;
+    }
+    uint32_t run = 0;
+#pragma unroll
+    for (int i = 0; i < kDPer; ++i) {
+      const uint32_t inc = wave_inclusive_scan(v[i]);
+      ex[i] = run + inc - v[i];
+      run += (uint32_t)__shfl((int)inc, 63);
+    }
+    if (lane == 0) wsum[w] = run;
+    __syncthreads();
+    uint32_t before = carry, tot = 0;
+#pragma unroll
+    for (int ww = 0; ww < kWaves; ++ww) {
+      const uint32_t x = wsum[ww];
+      before += ww < w ? x : 0u;
+      tot += x;
+    }
+#pragma unroll
+    for (int i = 0; i < kDPer; ++i) {
+      const uint32_t idx = wb + i * 64 + lane;
+      if (idx < nb) row[idx] = before + ex[i];
+    }
+    carry += tot;
+    __syncthreads();  // wsum is rewritten by the next round
+  }
+  if (threadIdx.x == 0) counts[(size_t)256 * nb + blockIdx.x] = carry;
+}
+
+// One digit pass from (sk, sv) into (dk, dv).  counts (256 * (nb + 1) u32) receive the per-digit
+// scanned (digit, tile) offsets and the digit totals.
 template <typename V, bool HAS_VAL, bool SAVE_LP>
 hipError_t radix_pass(const uint64_t* sk, const V* sv, uint64_t* dk, V* dv, uint16_t* lp, size_t n, const uint32_t* dn,
-                      int shift, int bits, bool gen, uint32_t* counts, void* scan_tmp, hipStream_t s) {
+                      int shift, int bits, bool gen, uint32_t* counts, hipStream_t s) {
   const size_t nb = (n + kTile - 1) / kTile;
   const uint32_t mask = (1u << bits) - 1u;
   {
     ProfScope prof("psa_hist", s);
-    hipLaunchKernelGGL((radix_hist_kernel<uint64_t>), dim3(grid_for(nb)), dim3(kThreads), 0, s, sk, n, shift, mask,
-                       counts, (uint32_t)nb, dn);
+    if ((reinterpret_cast<uintptr_t>(sk) & 15) == 0)
+      hipLaunchKernelGGL((radix_hist_kernel<uint64_t, true>), dim3(grid_for(nb)), dim3(kThreads), 0, s, sk, n, shift,
+                         mask, counts, (uint32_t)nb, dn);
+    else
+      hipLaunchKernelGGL((radix_hist_kernel<uint64_t, false>), dim3(grid_for(nb)), dim3(kThreads), 0, s, sk, n, shift,
+                         mask, counts, (uint32_t)nb, dn);
   }
-  hipError_t e = exclusive_scan_u32(counts, counts, 256 * nb, 0u, scan_tmp, s);
-  if (e != hipSuccess) return e;
+  {
+    ProfScope prof("scan", s);
+    hipLaunchKernelGGL(digit_scan_kernel, dim3(256), dim3(kThreads), 0, s, counts, (uint32_t)nb);
+  }
   {
     ProfScope prof("psa_scatter", s);
     hipLaunchKernelGGL((radix_scatter_kernel<uint64_t, V, HAS_VAL, SAVE_LP>), dim3(grid_for(nb)), dim3(kThreads), 0, s,
@@ -294,7 +373,6 @@ hipError_t radix_sort_impl(const uint64_t* keys_in, const V* vals_in, size_t n, 
     return hipGetLastError();
   }
   uint32_t* counts = reinterpret_cast<uint32_t*>(tmp);
-  void* scan_tmp = reinterpret_cast<char*>(tmp) + counts_bytes(n);
   const uint64_t* sk = keys_in;
   const V* sv = vals_in;
   int out = 0;
@@ -303,7 +381,7 @@ hipError_t radix_sort_impl(const uint64_t* keys_in, const V* vals_in, size_t n, 
     const int bits = end_bit - shift < 8 ? end_bit - shift : 8;
     uint64_t* dk = out == 0 ? k0 : k1;
     V* dv = out == 0 ? v0 : v1;
-    hipError_t e = radix_pass<V, true, false>(sk, sv, dk, dv, nullptr, n, dn, shift, bits, gen, counts, scan_tmp, s);
+    hipError_t e = radix_pass<V, true, false>(sk, sv, dk, dv, nullptr, n, dn, shift, bits, gen, counts, s);
     if (e != hipSuccess) return e;
     sk = dk;
     sv = dv;
@@ -318,7 +396,7 @@ int n_passes(int bits) { return (bits + 7) / 8; }
 
 }  // namespace
 
-size_t radix_tmp_bytes(size_t n) { return counts_bytes(n) + scan_tmp_bytes(256 * ((n + kTile - 1) / kTile + 1)) + 256; }
+size_t radix_tmp_bytes(size_t n) { return counts_bytes(n) + 256; }
 
 hipError_t radix_sort_pos(const uint64_t* keys_in, size_t n, int begin_bit, int end_bit, uint64_t* k0, uint32_t* v0,
                           uint64_t* k1, uint32_t* v1, void* tmp, hipStream_t s, int* which, const uint32_t* dn) {
@@ -338,8 +416,7 @@ size_t replay_bytes(size_t n, int bits) {
 }
 
 hipError_t radix_sort_keys_replay(const uint64_t* keys_in, size_t n, int begin_bit, int end_bit, uint64_t* k0,
-                                  uint64_t* k1, void* replay, void* tmp, hipStream_t s, int* which,
-                                  const uint32_t* dn) {
+                                  uint64_t* k1, void* replay, hipStream_t s, int* which, const uint32_t* dn) {
   *which = 0;
   if (n == 0) return hipSuccess;
   if (begin_bit < 0 || end_bit > 64 || begin_bit >= end_bit) return hipErrorInvalidValue;
@@ -352,7 +429,7 @@ hipError_t radix_sort_keys_replay(const uint64_t* keys_in, size_t n, int begin_b
     uint64_t* dk = out == 0 ? k0 : k1;
     hipError_t e = radix_pass<uint32_t, false, true>(sk, nullptr, dk, nullptr,
                                                      reinterpret_cast<uint16_t*>(rec + counts_bytes(n)), n, dn, shift,
-                                                     bits, false, reinterpret_cast<uint32_t*>(rec), tmp, s);
+                                                     bits, false, reinterpret_cast<uint32_t*>(rec), s);
     if (e != hipSuccess) return e;
     sk = dk;
     *which = out;

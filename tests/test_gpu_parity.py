@@ -160,6 +160,16 @@ def test_search_device_pointers(gpu):
         s.synchronize()
         assert np.array_equal(df.cpu().numpy(), ef)
         assert np.array_equal(dv.cpu().numpy().view(U64), ev)
+    # device queries and results at an 8-byte but not 16-byte aligned address (a tensor slice): the
+    # PSA histogram then reads the queries with 8-byte loads
+    dq1 = torch.from_numpy(np.concatenate([np.zeros(1, U64), q]).view(np.int64)).to(gpu)[1:]
+    dv1 = torch.empty(len(q) + 1, dtype=torch.int64, device=gpu)[1:]
+    df1 = torch.empty(len(q) + 1, dtype=torch.uint8, device=gpu)[1:]
+    assert dq1.data_ptr() % 16 == 8
+    for mode in (1, 2):
+        t.search_batch(dq1, out_vals=dv1, out_found=df1, stream=s, psa_bits=16, output_mode=mode)
+        s.synchronize()
+        assert np.array_equal(df1.cpu().numpy(), ef) and np.array_equal(dv1.cpu().numpy().view(U64), ev)
 
 
 @pytest.mark.parametrize("n_q", [1, 2, 255, 256, 257, 65537, 300001])
