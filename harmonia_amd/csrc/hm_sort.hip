@@ -194,6 +194,11 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(const K* __rest
 constexpr int kUThreads = 512;
 constexpr int kUItems = kTile / kUThreads;
 
+// VEC: vout is 16-byte and fout 4-byte aligned.  The write phase then gives each lane 4 consecutive
+// outputs: one 8-byte load of their 4 staging indices, two 16-byte value stores and one 4-byte flag
+// store, in place of 12 narrow loads and stores (the kernel is bound by load/store instructions, not
+// bytes; the 1-byte flag accesses cost as much as the 8-byte values).
+template <bool VEC>
 __global__ __launch_bounds__(kUThreads) void radix_unscatter_kernel(const uint64_t* __restrict__ vin,
                                                                     const uint8_t* __restrict__ fin,
                                                                     uint64_t* __restrict__ vout,
@@ -235,7 +240,15 @@ __global__ __launch_bounds__(kUThreads) void radix_unscatter_kernel(const uint64
   // addresses); the digit of staging index i is the last d with boff[d] <= i.  Every load of the
   // tile is issued before the first one is used, so their latencies overlap.
   uint32_t src[kUItems];
-  uint16_t lpv[kUItems];
+  uint16_t lpv[VEC ? 1 : kUItems];
+  uint2 lpw[VEC ? kUItems / 4 : 1];  // VEC: staging indices of outputs 4 j0 .. 4 j0 + 3
+  if constexpr (VEC) {
+#pragma unroll
+    for (int r = 0; r < kUItems / 4; ++r) {
+      const uint32_t j0 = (threadIdx.x + r * kUThreads) * 4;
+      lpw[r] = j0 + 3 < cnt ? *reinterpret_cast<const uint2*>(lp + tile0 + j0) : make_uint2(0u, 0u);
+    }
+  }
 #pragma unroll
   for (int r = 0; r < kUItems; ++r) {
     const uint32_t i = threadIdx.x + r * kUThreads;
@@ -244,7 +257,7 @@ __global__ __launch_bounds__(kUThreads) void radix_unscatter_kernel(const uint64
     for (uint32_t step = 128; step; step >>= 1)
       if (boff[lo + step] <= i) lo += step;
     src[r] = goff[lo] + (i - boff[lo]);
-    lpv[r] = i < cnt ? lp[tile0 + i] : (uint16_t)0;
+    if constexpr (!VEC) lpv[r] = i < cnt ? lp[tile0 + i] : (uint16_t)0;
   }
   uint64_t v[kUItems];
   uint8_t f[kUItems];
@@ -263,12 +276,36 @@ __global__ __launch_bounds__(kUThreads) void radix_unscatter_kernel(const uint64
     }
   }
   __syncthreads();
+  if constexpr (VEC) {
 #pragma unroll
-  for (int r = 0; r < kUItems; ++r) {
-    const uint32_t j = threadIdx.x + r * kUThreads;
-    if (j < cnt) {
-      vout[tile0 + j] = sv[lpv[r]];
-      fout[tile0 + j] = sf[lpv[r]];
+    for (int r = 0; r < kUItems / 4; ++r) {
+      const uint32_t j0 = (threadIdx.x + r * kUThreads) * 4;
+      if (j0 + 3 < cnt) {
+        const uint32_t l0 = lpw[r].x & 0xFFFFu, l1 = lpw[r].x >> 16, l2 = lpw[r].y & 0xFFFFu, l3 = lpw[r].y >> 16;
+        *reinterpret_cast<ulonglong2*>(vout + tile0 + j0) = make_ulonglong2(sv[l0], sv[l1]);
+        *reinterpret_cast<ulonglong2*>(vout + tile0 + j0 + 2) = make_ulonglong2(sv[l2], sv[l3]);
+        *reinterpret_cast<uint32_t*>(fout + tile0 + j0) =
+            (uint32_t)sf[l0] | ((uint32_t)sf[l1] << 8) | ((uint32_t)sf[l2] << 16) | ((uint32_t)sf[l3] << 24);
+      } else {
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+          const uint32_t j = j0 + k;
+          if (j < cnt) {
+            const uint32_t l = lp[tile0 + j];
+            vout[tile0 + j] = sv[l];
+            fout[tile0 + j] = sf[l];
+          }
+        }
+      }
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < kUItems; ++r) {
+      const uint32_t j = threadIdx.x + r * kUThreads;
+      if (j < cnt) {
+        vout[tile0 + j] = sv[lpv[r]];
+        fout[tile0 + j] = sf[lpv[r]];
+      }
     }
   }
 }
@@ -459,9 +496,14 @@ hipError_t unsort_results(uint64_t* vals, uint8_t* found, size_t n, int bits, co
       dv = vals, df = found;
     }
     ProfScope prof("unsort", s);
-    hipLaunchKernelGGL(radix_unscatter_kernel, dim3(grid_for(nb)), dim3(kUThreads), 0, s, sv, sf, dv, df,
-                       reinterpret_cast<const uint16_t*>(rec + counts_bytes(n)), n,
-                       reinterpret_cast<const uint32_t*>(rec), (uint32_t)nb, dn);
+    const uint16_t* lp = reinterpret_cast<const uint16_t*>(rec + counts_bytes(n));
+    const uint32_t* offs = reinterpret_cast<const uint32_t*>(rec);
+    if ((reinterpret_cast<uintptr_t>(dv) & 15) == 0 && (reinterpret_cast<uintptr_t>(df) & 3) == 0)
+      hipLaunchKernelGGL(radix_unscatter_kernel<true>, dim3(grid_for(nb)), dim3(kUThreads), 0, s, sv, sf, dv, df, lp,
+                         n, offs, (uint32_t)nb, dn);
+    else
+      hipLaunchKernelGGL(radix_unscatter_kernel<false>, dim3(grid_for(nb)), dim3(kUThreads), 0, s, sv, sf, dv, df, lp,
+                         n, offs, (uint32_t)nb, dn);
     sv = dv;
     sf = df;
   }
