@@ -29,11 +29,17 @@
 namespace hm {
 namespace {
 
+// Tiles are NT threads x kItems keys: 256 x 16 for the sorts with a payload (their LDS staging of keys
+// and payload then allows several blocks per CU), 512 x 16 for the keys-only sort that PSA replays:
+// its 256 digit runs per tile are then ~32 keys long instead of ~16, and the runs are what the scatter
+// writes and the un-scatter gathers (each a separate stretch of HBM).
 constexpr int kThreads = 256;
+constexpr int kRThreads = 512;
 constexpr int kItems = 16;
 constexpr int kTile = kThreads * kItems;
+constexpr int kRTile = kRThreads * kItems;
 constexpr int kWaves = kThreads / 64;
-static_assert(kTile <= 65536, "staging indices are 16-bit");
+static_assert(kRTile <= 65536, "staging indices are 16-bit");
 
 // Block -> tile: blocks are dispatched round-robin over the 8 XCDs; XCD x gets tiles
 // [x * grid / 8, (x + 1) * grid / 8).  grid % 8 == 0; tiles >= nb are idle.
@@ -44,22 +50,22 @@ size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 // Per-tile digit counts, one block per tile.  VEC: the keys are 16-byte aligned and full tiles are
 // read with 16-byte loads (two keys per lane per load; the counts do not depend on which lane counts
 // which key).  (A persistent grid that loads tile t + 1 while counting tile t measured no faster.)
-template <typename K, bool VEC>
-__global__ __launch_bounds__(kThreads) void radix_hist_kernel(const K* __restrict__ keys, size_t n, int shift,
-                                                              uint32_t mask, uint32_t* __restrict__ counts,
-                                                              uint32_t nb, const uint32_t* __restrict__ dn) {
+template <typename K, bool VEC, int NT>
+__global__ __launch_bounds__(NT) void radix_hist_kernel(const K* __restrict__ keys, size_t n, int shift, uint32_t mask,
+                                                        uint32_t* __restrict__ counts, uint32_t nb,
+                                                        const uint32_t* __restrict__ dn) {
+  constexpr int TILE = NT * kItems, NW = NT / 64;
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   if (tile >= nb) return;  // block-uniform
   if (dn) n = *dn;  // device-side count (graph replays): tiles past n count nothing
   // one histogram per wave (LDS atomics; lanes of a wave that hit one bin are serialized by the LDS,
   // which is cheaper than ranking them when only counts are needed)
-  __shared__ uint32_t hist[kWaves][256];
-#pragma unroll
-  for (int ww = 0; ww < kWaves; ++ww) hist[ww][threadIdx.x] = 0;
+  __shared__ uint32_t hist[NW][256];
+  for (int i = threadIdx.x; i < NW * 256; i += NT) (&hist[0][0])[i] = 0;
   __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const size_t base = (size_t)tile * kTile + (size_t)w * 64 * kItems;
-  if (VEC && (size_t)(tile + 1) * kTile <= n) {  // block-uniform
+  const size_t base = (size_t)tile * TILE + (size_t)w * 64 * kItems;
+  if (VEC && (size_t)(tile + 1) * TILE <= n) {  // block-uniform
     static_assert(sizeof(K) == 8, "16-byte loads take two keys");
     const ulonglong2* src = reinterpret_cast<const ulonglong2*>(keys + base);
     ulonglong2 kk[kItems / 2];
@@ -86,44 +92,47 @@ __global__ __launch_bounds__(kThreads) void radix_hist_kernel(const K* __restric
     }
   }
   __syncthreads();
-  uint32_t c = 0;
+  if (threadIdx.x < 256) {
+    uint32_t c = 0;
 #pragma unroll
-  for (int ww = 0; ww < kWaves; ++ww) c += hist[ww][threadIdx.x];
-  counts[(size_t)threadIdx.x * nb + tile] = c;
+    for (int ww = 0; ww < NW; ++ww) c += hist[ww][threadIdx.x];
+    counts[(size_t)threadIdx.x * nb + tile] = c;
+  }
 }
 
 // Write (key, value) to (kout, vout); without HAS_VAL keys only.  With SAVE_LP, lp_out[i] = the
 // index of element i in its tile's digit-ordered staging (what un-sorting needs).
-template <typename K, typename V, bool HAS_VAL, bool SAVE_LP>
-__global__ __launch_bounds__(kThreads) void radix_scatter_kernel(const K* __restrict__ kin, const V* __restrict__ vin,
-                                                                 K* __restrict__ kout, V* __restrict__ vout,
-                                                                 uint16_t* __restrict__ lp_out, size_t n, int shift,
-                                                                 uint32_t mask, const uint32_t* __restrict__ offsets,
-                                                                 uint32_t nb, int gen_pos,
-                                                                 const uint32_t* __restrict__ dn) {
+template <typename K, typename V, bool HAS_VAL, bool SAVE_LP, int NT>
+__global__ __launch_bounds__(NT) void radix_scatter_kernel(const K* __restrict__ kin, const V* __restrict__ vin,
+                                                           K* __restrict__ kout, V* __restrict__ vout,
+                                                           uint16_t* __restrict__ lp_out, size_t n, int shift,
+                                                           uint32_t mask, const uint32_t* __restrict__ offsets,
+                                                           uint32_t nb, int gen_pos, const uint32_t* __restrict__ dn) {
+  constexpr int TILE = NT * kItems, NW = NT / 64;
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   if (dn) n = *dn;
-  if (tile >= nb || (size_t)tile * kTile >= n) return;  // whole tile past the end (block-uniform)
-  __shared__ uint32_t wcnt[kWaves][256];
+  if (tile >= nb || (size_t)tile * TILE >= n) return;  // whole tile past the end (block-uniform)
+  __shared__ uint32_t wcnt[NW][256];
   __shared__ uint32_t goff[256];
   __shared__ uint32_t boff[256];
-  __shared__ uint32_t scan_sh[kWaves + 1];
-  __shared__ K skeys[kTile];
-  __shared__ V svals[HAS_VAL ? kTile : 1];
-  for (int i = threadIdx.x; i < kWaves * 256; i += kThreads) (&wcnt[0][0])[i] = 0;
+  __shared__ uint32_t scan_sh[NW + 1];
+  __shared__ K skeys[TILE];
+  __shared__ V svals[HAS_VAL ? TILE : 1];
+  for (int i = threadIdx.x; i < NW * 256; i += NT) (&wcnt[0][0])[i] = 0;
   {
     // digit d starts at the digits below it (scan of the totals after the 256 x nb counts) plus its
-    // count in the tiles before this one
+    // count in the tiles before this one (threads >= 256 take part in the scans with zeros)
+    const bool dig = threadIdx.x < 256;
     uint32_t all;
-    const uint32_t dbase = block_exclusive_scan<uint32_t, kThreads>(offsets[(size_t)256 * nb + threadIdx.x], scan_sh,
-                                                                    all);
-    goff[threadIdx.x] = dbase + offsets[(size_t)threadIdx.x * nb + tile];
+    const uint32_t dbase =
+        block_exclusive_scan<uint32_t, NT>(dig ? offsets[(size_t)256 * nb + threadIdx.x] : 0u, scan_sh, all);
+    if (dig) goff[threadIdx.x] = dbase + offsets[(size_t)threadIdx.x * nb + tile];
   }
   __syncthreads();
 
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const uint64_t lt = (1ull << lane) - 1ull;
-  const size_t base = (size_t)tile * kTile + (size_t)w * 64 * kItems;
+  const size_t base = (size_t)tile * TILE + (size_t)w * 64 * kItems;
   K kk[kItems];
   V vv[HAS_VAL ? kItems : 1];
   uint32_t pk[kItems];  // (rank within wave << 8) | digit
@@ -152,14 +161,17 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(const K* __rest
   {
     const int d = threadIdx.x;
     uint32_t run = 0;
+    if (d < 256) {
 #pragma unroll
-    for (int ww = 0; ww < kWaves; ++ww) {
-      const uint32_t t = wcnt[ww][d];
-      wcnt[ww][d] = run;
-      run += t;
+      for (int ww = 0; ww < NW; ++ww) {
+        const uint32_t t = wcnt[ww][d];
+        wcnt[ww][d] = run;
+        run += t;
+      }
     }
     uint32_t total;
-    boff[d] = block_exclusive_scan<uint32_t, kThreads>(run, scan_sh, total);
+    const uint32_t e = block_exclusive_scan<uint32_t, NT>(run, scan_sh, total);
+    if (d < 256) boff[d] = e;
   }
   __syncthreads();
   // stage the tile in LDS in digit order ...
@@ -176,9 +188,9 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(const K* __rest
   }
   __syncthreads();
   // ... and write it out with consecutive lanes on consecutive addresses of each digit run
-  const size_t tile0 = (size_t)tile * kTile;
-  const uint32_t cnt = (uint32_t)(n - tile0 < (size_t)kTile ? n - tile0 : (size_t)kTile);
-  for (uint32_t i = threadIdx.x; i < cnt; i += kThreads) {
+  const size_t tile0 = (size_t)tile * TILE;
+  const uint32_t cnt = (uint32_t)(n - tile0 < (size_t)TILE ? n - tile0 : (size_t)TILE);
+  for (uint32_t i = threadIdx.x; i < cnt; i += NT) {
     const K k = skeys[i];
     const uint32_t d = (uint32_t)(k >> shift) & mask;
     const uint32_t p = goff[d] + (i - boff[d]);
@@ -187,12 +199,11 @@ __global__ __launch_bounds__(kThreads) void radix_scatter_kernel(const K* __rest
   }
 }
 
-// Undo one recorded pass for per-element results: (vin, fin) are in the pass's output order,
-// (vout, fout) receive them in its input order.  A tile takes 512 threads x 8 items (the forward
-// passes use 256 x 16): the tile's LDS staging (36 KiB) caps a CU at 4 blocks, and the gathers below
-// are latency bound, so twice the waves per block keep twice the loads in flight.
-constexpr int kUThreads = 512;
-constexpr int kUItems = kTile / kUThreads;
+// Undo one recorded pass of the keys-only (replay) sort for per-element results: (vin, fin) are in
+// the pass's output order, (vout, fout) receive them in its input order.  One block per forward tile
+// (512 threads x 16 items); its LDS staging of 8192 results (72 KiB) allows two blocks per CU.
+constexpr int kUThreads = kRThreads;
+constexpr int kUItems = kRTile / kUThreads;
 
 // VEC: vout is 16-byte and fout 4-byte aligned.  The write phase then gives each lane 4 consecutive
 // outputs: one 8-byte load of their 4 staging indices, two 16-byte value stores and one 4-byte flag
@@ -208,12 +219,12 @@ __global__ __launch_bounds__(kUThreads) void radix_unscatter_kernel(const uint64
                                                                     const uint32_t* __restrict__ dn) {
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
   if (dn) n = *dn;
-  if (tile >= nb || (size_t)tile * kTile >= n) return;  // block-uniform
+  if (tile >= nb || (size_t)tile * kRTile >= n) return;  // block-uniform
   __shared__ uint32_t goff[256];
   __shared__ uint32_t boff[257];
   __shared__ uint32_t scan_sh[kUThreads / 64 + 1];
-  __shared__ uint64_t sv[kTile];
-  __shared__ uint8_t sf[kTile];
+  __shared__ uint64_t sv[kRTile];
+  __shared__ uint8_t sf[kRTile];
   {
     // offsets hold, per digit, the exclusive scan of its counts along the tiles, then the 256 digit
     // totals: this tile's count of digit d = the next tile's offset (or the total) - this one
@@ -234,7 +245,7 @@ __global__ __launch_bounds__(kUThreads) void radix_unscatter_kernel(const uint64
     if (d == 0) boff[256] = total;
   }
   __syncthreads();
-  const size_t tile0 = (size_t)tile * kTile;
+  const size_t tile0 = (size_t)tile * kRTile;
   const uint32_t cnt = boff[256];
   // gather the tile's digit runs in digit order (consecutive i of one digit read consecutive
   // addresses); the digit of staging index i is the last d with boff[d] <= i.  Every load of the
@@ -372,15 +383,17 @@ __global__ __launch_bounds__(kThreads) void digit_scan_kernel(uint32_t* __restri
 template <typename V, bool HAS_VAL, bool SAVE_LP>
 hipError_t radix_pass(const uint64_t* sk, const V* sv, uint64_t* dk, V* dv, uint16_t* lp, size_t n, const uint32_t* dn,
                       int shift, int bits, bool gen, uint32_t* counts, hipStream_t s) {
-  const size_t nb = (n + kTile - 1) / kTile;
+  constexpr int NT = SAVE_LP ? kRThreads : kThreads;  // the replay sort takes 8192-key tiles
+  constexpr int TILE = NT * kItems;
+  const size_t nb = (n + TILE - 1) / TILE;
   const uint32_t mask = (1u << bits) - 1u;
   {
     ProfScope prof("psa_hist", s);
     if ((reinterpret_cast<uintptr_t>(sk) & 15) == 0)
-      hipLaunchKernelGGL((radix_hist_kernel<uint64_t, true>), dim3(grid_for(nb)), dim3(kThreads), 0, s, sk, n, shift,
+      hipLaunchKernelGGL((radix_hist_kernel<uint64_t, true, NT>), dim3(grid_for(nb)), dim3(NT), 0, s, sk, n, shift,
                          mask, counts, (uint32_t)nb, dn);
     else
-      hipLaunchKernelGGL((radix_hist_kernel<uint64_t, false>), dim3(grid_for(nb)), dim3(kThreads), 0, s, sk, n, shift,
+      hipLaunchKernelGGL((radix_hist_kernel<uint64_t, false, NT>), dim3(grid_for(nb)), dim3(NT), 0, s, sk, n, shift,
                          mask, counts, (uint32_t)nb, dn);
   }
   {
@@ -389,12 +402,13 @@ hipError_t radix_pass(const uint64_t* sk, const V* sv, uint64_t* dk, V* dv, uint
   }
   {
     ProfScope prof("psa_scatter", s);
-    hipLaunchKernelGGL((radix_scatter_kernel<uint64_t, V, HAS_VAL, SAVE_LP>), dim3(grid_for(nb)), dim3(kThreads), 0, s,
+    hipLaunchKernelGGL((radix_scatter_kernel<uint64_t, V, HAS_VAL, SAVE_LP, NT>), dim3(grid_for(nb)), dim3(NT), 0, s,
                        sk, sv, dk, dv, lp, n, shift, mask, (const uint32_t*)counts, (uint32_t)nb, gen ? 1 : 0, dn);
   }
   return hipGetLastError();
 }
 
+// Sized for the smaller tile (4096 keys): the replay sort's 8192-key tiles need half of it.
 size_t counts_bytes(size_t n) { return align_up(256 * ((n + kTile - 1) / kTile + 1) * sizeof(uint32_t), 256); }
 
 template <typename V>
@@ -480,7 +494,7 @@ hipError_t unsort_results(uint64_t* vals, uint8_t* found, size_t n, int bits, co
                           const uint32_t* dn) {
   if (n == 0) return hipSuccess;
   const int P = n_passes(bits);
-  const size_t nb = (n + kTile - 1) / kTile;
+  const size_t nb = (n + kRTile - 1) / kRTile;
   const size_t stride = counts_bytes(n) + align_up(n * sizeof(uint16_t), 256);
   const uint64_t* sv = vals;
   const uint8_t* sf = found;
