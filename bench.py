@@ -199,12 +199,12 @@ def ablation(tree, dq, out_v, out_f, stream, steps, warmup):
         ("prefix_index_g8", "+ prefix-sum layout: the HB+ kernel with child = prefix[node] + slot",
          lambda: tree.search_baseline(dq, out_v, out_f, kind="prefix", group_size=8, stream=stream)),
         ("prefix_g8_persistent", "the production kernel at the same shape (8 lanes, 1 query per group, no LDS, "
-                                 "no PSA): persistent XCD-mapped grid",
+                                 "no PSA): persistent XCD-mapped grid, 8 blocks per CU (one round at this shape)",
          lambda: tree.search_batch(dq, out_vals=out_v, out_found=out_f, stream=stream, group_size=8,
-                                   queries_per_group=1, lds_levels=0, psa_bits=0)),
+                                   queries_per_group=1, lds_levels=0, psa_bits=0, blocks_per_cu=8)),
         ("prefix_g8_psa16", "+ PSA partial sort (16 bits), results back in input order",
          lambda: tree.search_batch(dq, out_vals=out_v, out_found=out_f, stream=stream, group_size=8,
-                                   queries_per_group=1, lds_levels=0, psa_bits=16, output_mode=1)),
+                                   queries_per_group=1, lds_levels=0, psa_bits=16, output_mode=1, blocks_per_cu=8)),
         ("prefix_psa16_ntg", "+ the tree's narrowed-group shape for PSA batches (lanes per query, queries in "
                              "flight per group)",
          lambda: tree.search_batch(dq, out_vals=out_v, out_found=out_f, stream=stream, lds_levels=0, psa_bits=16,

@@ -74,7 +74,7 @@ int tune_tree(hm_tree* t, const uint64_t* sample, uint64_t n, uint32_t flags, hm
   const int maxl = std::min(3, t->height);
   const int groups[] = {1, 2, 4, 8};
   const int qpgs[] = {1, 2, 4};
-  const int bpcs[] = {4, 8};
+  const int bpcs[] = {4, 5, 8, 10};  // persistent grids: whole rounds of resident blocks matter
   uint32_t trials = 0;
   Best bd, bp;
   int rc = 0;
