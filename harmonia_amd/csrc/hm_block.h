@@ -7,15 +7,18 @@ namespace hm {
 
 // Lanes (among `valid`) of this wave whose 8-bit digit equals this lane's digit: eight ballots,
 // one per digit bit.  Every lane of the wave must execute it.
+// Per bit: s = 0 or all ones (the bit sign-extended), m = the wave's ballot of that bit, and the lane
+// keeps the peers whose bit agrees with its own: peers &= ~(m ^ s), one 3-input logic op per half.
 __device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t valid) {
-  uint64_t peers = valid;
+  uint32_t lo = (uint32_t)valid, hi = (uint32_t)(valid >> 32);
 #pragma unroll
   for (int b = 0; b < 8; ++b) {
-    const bool bit = (d >> b) & 1u;
-    const uint64_t m = __ballot(bit);
-    peers &= bit ? m : ~m;
+    const uint32_t s = (uint32_t)((int32_t)(d << (31 - b)) >> 31);
+    const uint64_t m = __ballot(s != 0u);
+    lo &= ~((uint32_t)m ^ s);
+    hi &= ~((uint32_t)(m >> 32) ^ s);
   }
-  return peers;
+  return ((uint64_t)hi << 32) | lo;
 }
 
 template <typename T>
