@@ -984,10 +984,17 @@ int publish_prepare(hm_tree* t, hipStream_t s, PublishPlan* plan) {
   if (n <= kSmallUpdate) {
     HM_TRY(small_sort_dedupe(t->pend_k, t->pend_v, (uint32_t)n, b.uk, b.uv, dm, s));
   } else {
-    int which = 0;
-    HM_TRY(radix_sort_u64pay(t->pend_k, t->pend_v, n, 0, 64, b.k0, b.v0, b.k1, b.v1, b.rtmp, s, &which));
-    const uint64_t* sk = which ? b.k1 : b.k0;
-    const uint64_t* sv = which ? b.v1 : b.v0;
+    const uint64_t* sk;
+    const uint64_t* sv;
+    if (n <= kMergeUpdate) {  // the index ping-pong uses keep / keep_scan before they are needed
+      HM_TRY(merge_sort_pairs(t->pend_k, t->pend_v, (uint32_t)n, b.k0, b.k1, b.keep, b.keep_scan, b.v0, s, &sk));
+      sv = b.v0;
+    } else {
+      int which = 0;
+      HM_TRY(radix_sort_u64pay(t->pend_k, t->pend_v, n, 0, 64, b.k0, b.v0, b.k1, b.v1, b.rtmp, s, &which));
+      sk = which ? b.k1 : b.k0;
+      sv = which ? b.v1 : b.v0;
+    }
     HM_TRY(mark_last_of_runs(sk, n, b.keep, s));
     HM_TRY(exclusive_scan_u32(b.keep, b.keep_scan, n, 0u, b.stmp, s));
     HM_TRY(compact_pairs(sk, sv, b.keep, b.keep_scan, n, b.uk, b.uv, s));

@@ -161,6 +161,13 @@ hipError_t count_inserts(const uint32_t* leaf, const uint8_t* found, size_t n, u
 constexpr uint32_t kSmallUpdate = 4096;
 hipError_t small_sort_dedupe(const uint64_t* pk, const uint64_t* pv, uint32_t n, uint64_t* uk, uint64_t* uv,
                              uint32_t* dm, hipStream_t s);
+// Mid-size batches (kSmallUpdate < n <= kMergeUpdate): stable sort of the staged pairs by key, as a
+// merge sort -- chunks of kSmallUpdate sorted in LDS, then log2(n / kSmallUpdate) merge passes over
+// (key, staging index) in (k0, i0) / (k1, i1), then the values gathered into vout.  *sorted_keys is
+// k0 or k1.  1 + log2(n / 4096) + 1 launches in place of the radix sort's 24.
+constexpr uint32_t kMergeUpdate = 1u << 22;
+hipError_t merge_sort_pairs(const uint64_t* pk, const uint64_t* pv, uint32_t n, uint64_t* k0, uint64_t* k1,
+                            uint32_t* i0, uint32_t* i1, uint64_t* vout, hipStream_t s, const uint64_t** sorted_keys);
 // *dm = keep_scan[n - 1] + keep[n - 1] (unique count after compact_pairs).
 hipError_t unique_count(const uint32_t* keep, const uint32_t* keep_scan, size_t n, uint32_t* dm, hipStream_t s);
 // Over the m located unique keys (sorted, so each leaf's keys form one run):

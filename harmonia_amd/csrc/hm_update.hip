@@ -12,6 +12,8 @@
 // The common serving case (a few thousand keys, no leaf overflows) stays O(batch): one block sorts
 // and de-duplicates the batch in LDS, the keys are located, each leaf's keys form one run of the
 // sorted batch, and one thread per run plans (overflow? how many new keys?) and then applies it.
+#include <utility>
+
 #include "hm_block.h"
 #include "hm_internal.h"
 
@@ -234,24 +236,9 @@ __global__ void relayout_touched_kernel(const Leaf* __restrict__ old_leaves, con
 
 constexpr int kSortThreads = 1024;
 
-// Bitonic sort of (key, staging index) in LDS; the index breaks ties, so the last write of a key
-// ends its run of equal keys.  Padding (KEY_MAX, ~0) sorts after every real key (KEY_MAX is never
-// staged).
-__global__ __launch_bounds__(kSortThreads) void small_sort_dedupe_kernel(const uint64_t* __restrict__ pk,
-                                                                         const uint64_t* __restrict__ pv, uint32_t n,
-                                                                         uint64_t* __restrict__ uk,
-                                                                         uint64_t* __restrict__ uv,
-                                                                         uint32_t* __restrict__ dm) {
-  __shared__ uint64_t sk[kSmallUpdate];
-  __shared__ uint32_t si[kSmallUpdate];
-  __shared__ uint32_t scan_sh[kSortThreads / 64 + 1];
-  uint32_t N = 1;
-  while (N < n) N <<= 1;
-  for (uint32_t i = threadIdx.x; i < N; i += kSortThreads) {
-    sk[i] = i < n ? pk[i] : KEY_MAX;
-    si[i] = i < n ? i : 0xFFFFFFFFu;
-  }
-  __syncthreads();
+// Bitonic network over N (a power of two, <= kSmallUpdate) (key, index) pairs in LDS, ascending by
+// key, then index.  Ends with a barrier.
+__device__ __forceinline__ void bitonic_pairs(uint64_t* sk, uint32_t* si, uint32_t N) {
   for (uint32_t k = 2; k <= N; k <<= 1) {
     for (uint32_t j = k >> 1; j > 0; j >>= 1) {
       for (uint32_t i = threadIdx.x; i < N; i += kSortThreads) {
@@ -271,6 +258,84 @@ __global__ __launch_bounds__(kSortThreads) void small_sort_dedupe_kernel(const u
       __syncthreads();
     }
   }
+}
+
+// First stage of the merge sort of mid-size batches: chunk c of kSmallUpdate staged keys sorted by
+// (key, staging index) in LDS, written with the indices.
+__global__ __launch_bounds__(kSortThreads) void chunk_sort_kernel(const uint64_t* __restrict__ pk, uint32_t n,
+                                                                  uint64_t* __restrict__ ko,
+                                                                  uint32_t* __restrict__ io) {
+  __shared__ uint64_t sk[kSmallUpdate];
+  __shared__ uint32_t si[kSmallUpdate];
+  const uint32_t c0 = blockIdx.x * kSmallUpdate;
+  const uint32_t cnt = n - c0 < kSmallUpdate ? n - c0 : kSmallUpdate;
+  for (uint32_t i = threadIdx.x; i < kSmallUpdate; i += kSortThreads) {
+    sk[i] = i < cnt ? pk[c0 + i] : KEY_MAX;
+    si[i] = i < cnt ? c0 + i : 0xFFFFFFFFu;
+  }
+  __syncthreads();
+  bitonic_pairs(sk, si, kSmallUpdate);
+  for (uint32_t i = threadIdx.x; i < cnt; i += kSortThreads) {
+    ko[c0 + i] = sk[i];
+    io[c0 + i] = si[i];
+  }
+}
+
+// One merge pass over sorted runs of length L (pairs [2gL, 2gL + L) and [2gL + L, 2gL + 2L)): each
+// element goes to its own rank in the run plus the number of elements of the other run that order
+// before it.  (key, index) pairs are all distinct, so the ranks never collide and the merge keeps
+// the staging order of equal keys.
+__global__ __launch_bounds__(256) void merge_pass_kernel(const uint64_t* __restrict__ kin,
+                                                         const uint32_t* __restrict__ iin,
+                                                         uint64_t* __restrict__ kout, uint32_t* __restrict__ iout,
+                                                         uint32_t n, uint32_t L) {
+  const uint32_t e = blockIdx.x * 256u + threadIdx.x;
+  if (e >= n) return;
+  const uint32_t base = e / (2 * L) * (2 * L);
+  const uint32_t mid = base + L < n ? base + L : n;
+  const uint32_t end = mid + L < n ? mid + L : n;
+  const uint64_t k = kin[e];
+  const uint32_t ix = iin[e];
+  const bool in_a = e < mid;
+  uint32_t lo = in_a ? mid : base, hi = in_a ? end : mid;  // search the other run
+  while (lo < hi) {
+    const uint32_t m = (lo + hi) >> 1;
+    const uint64_t km = kin[m];
+    if (km < k || (km == k && iin[m] < ix))
+      lo = m + 1;
+    else
+      hi = m;
+  }
+  const uint32_t pos = in_a ? base + (e - base) + (lo - mid) : base + (e - mid) + (lo - base);
+  kout[pos] = k;
+  iout[pos] = ix;
+}
+
+__global__ void gather_by_index_kernel(const uint64_t* __restrict__ v, const uint32_t* __restrict__ ix, uint32_t n,
+                                       uint64_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = v[ix[i]];
+}
+
+// Bitonic sort of (key, staging index) in LDS; the index breaks ties, so the last write of a key
+// ends its run of equal keys.  Padding (KEY_MAX, ~0) sorts after every real key (KEY_MAX is never
+// staged).
+__global__ __launch_bounds__(kSortThreads) void small_sort_dedupe_kernel(const uint64_t* __restrict__ pk,
+                                                                         const uint64_t* __restrict__ pv, uint32_t n,
+                                                                         uint64_t* __restrict__ uk,
+                                                                         uint64_t* __restrict__ uv,
+                                                                         uint32_t* __restrict__ dm) {
+  __shared__ uint64_t sk[kSmallUpdate];
+  __shared__ uint32_t si[kSmallUpdate];
+  __shared__ uint32_t scan_sh[kSortThreads / 64 + 1];
+  uint32_t N = 1;
+  while (N < n) N <<= 1;
+  for (uint32_t i = threadIdx.x; i < N; i += kSortThreads) {
+    sk[i] = i < n ? pk[i] : KEY_MAX;
+    si[i] = i < n ? i : 0xFFFFFFFFu;
+  }
+  __syncthreads();
+  bitonic_pairs(sk, si, N);
   constexpr int kPer = kSmallUpdate / kSortThreads;  // consecutive elements per thread
   const uint32_t b0 = threadIdx.x * kPer;
   bool keep[kPer];
@@ -372,6 +437,25 @@ hipError_t small_sort_dedupe(const uint64_t* pk, const uint64_t* pv, uint32_t n,
   if (n == 0 || n > kSmallUpdate) return hipErrorInvalidValue;
   ProfScope prof("update", s);
   hipLaunchKernelGGL(small_sort_dedupe_kernel, dim3(1), dim3(kSortThreads), 0, s, pk, pv, n, uk, uv, dm);
+  return hipGetLastError();
+}
+
+hipError_t merge_sort_pairs(const uint64_t* pk, const uint64_t* pv, uint32_t n, uint64_t* k0, uint64_t* k1,
+                            uint32_t* i0, uint32_t* i1, uint64_t* vout, hipStream_t s, const uint64_t** sorted_keys) {
+  if (n == 0) return hipErrorInvalidValue;
+  ProfScope prof("update", s);
+  const uint32_t chunks = (n + kSmallUpdate - 1) / kSmallUpdate;
+  hipLaunchKernelGGL(chunk_sort_kernel, dim3(chunks), dim3(kSortThreads), 0, s, pk, n, k0, i0);
+  uint64_t *ka = k0, *kb = k1;
+  uint32_t *ia = i0, *ib = i1;
+  const unsigned grid = (unsigned)((n + 255) / 256);
+  for (uint64_t L = kSmallUpdate; L < n; L *= 2) {
+    hipLaunchKernelGGL(merge_pass_kernel, dim3(grid), dim3(256), 0, s, ka, ia, kb, ib, n, (uint32_t)L);
+    std::swap(ka, kb);
+    std::swap(ia, ib);
+  }
+  hipLaunchKernelGGL(gather_by_index_kernel, dim3(grid), dim3(256), 0, s, pv, ia, n, vout);
+  *sorted_keys = ka;
   return hipGetLastError();
 }
 

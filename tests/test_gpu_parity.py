@@ -297,11 +297,12 @@ def test_update_in_place_and_relayout(gpu, fill):
         assert rebuilt_any  # full leaves force the re-layout path
 
 
-@pytest.mark.parametrize("size", [1, 4095, 4096, 4097, 30000, 65537])
+@pytest.mark.parametrize("size", [1, 4095, 4096, 4097, 30000, 65537, (1 << 22) + 1])
 @pytest.mark.parametrize("fill", [8, 5])
 def test_update_batch_sizes(gpu, size, fill):
-    """Batches on both sides of the one-block sort (<= 4096 staged keys) and the multi-block radix
-    sort; each mixes new keys, existing keys and repeats (the last write wins), staged in two calls."""
+    """Batches on all sides of the three staged-key sorts: one block (<= 4096), the merge sort (up to
+    2^22) and the radix sort; each mixes new keys, existing keys and repeats (the last write wins,
+    also across sort chunks), staged in two calls."""
     t, keys, vals, sk, sv = make(60000, 64, fill, seed=71 + size)
     g = W.rng(71, size)
     n_new = (size + 1) // 2
