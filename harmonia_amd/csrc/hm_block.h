@@ -5,6 +5,15 @@
 
 namespace hm {
 
+// Make this wave's LDS writes visible to its own later LDS reads (and keep the compiler from moving
+// LDS accesses across): the LDS executes one wave's operations in order, so no block barrier is
+// needed for a wave-private buffer.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Lanes (among `valid`) of this wave whose 8-bit digit equals this lane's digit: eight ballots,
 // one per digit bit.  Every lane of the wave must execute it.
 // Per bit: s = 0 or all ones (the bit sign-extended), m = the wave's ballot of that bit, and the lane

@@ -7,6 +7,7 @@
 // group's bits of the 64-bit lane mask.  The reference does the same with __ballot_sync over
 // 32-lane warps plus shared-memory flag arrays (H/regular_bpt/prefix-sort.cu:147-173).
 #pragma once
+#include "hm_block.h"
 #include "hm_internal.h"
 
 namespace hm {
@@ -166,14 +167,6 @@ __device__ __forceinline__ void descend_q(const TreeView& tv, int lds_levels, co
   for (int q = 0; q < Q; ++q) node[q] -= tv.n_inner;
 }
 
-// Make this wave's LDS writes visible to its own later LDS reads (and keep the compiler from moving
-// LDS accesses across): the LDS executes one wave's operations in order, so no block barrier is
-// needed for a wave-private buffer.
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // One global level for PSA-sorted queries with G lanes per query.  At the upper levels the 64/G
 // queries of a wave's query slot almost always share their node, so the wave loads the node of its

@@ -237,22 +237,25 @@ __global__ void relayout_touched_kernel(const Leaf* __restrict__ old_leaves, con
 constexpr int kSortThreads = 1024;
 
 // Bitonic network over N (a power of two, <= kSmallUpdate) (key, index) pairs in LDS, ascending by
-// key, then index.  Ends with a barrier.
+// key, then index.  Pair q (elements i, i + j) is handled by thread q % kSortThreads, so every lane
+// has work in every step.  Each step moves every pair through LDS (about 98 KB at N = 4096), and that
+// bounds it: wave-level syncs for the steps whose pairs stay inside one wave measured the same as block
+// barriers.  Ends with a block barrier.
 __device__ __forceinline__ void bitonic_pairs(uint64_t* sk, uint32_t* si, uint32_t N) {
   for (uint32_t k = 2; k <= N; k <<= 1) {
     for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      for (uint32_t i = threadIdx.x; i < N; i += kSortThreads) {
-        const uint32_t p = i ^ j;
-        if (p > i) {
-          const uint64_t a = sk[i], b = sk[p];
-          const uint32_t ia = si[i], ib = si[p];
-          const bool gt = a > b || (a == b && ia > ib);
-          if (gt == ((i & k) == 0)) {
-            sk[i] = b;
-            sk[p] = a;
-            si[i] = ib;
-            si[p] = ia;
-          }
+      // pair q: i = q with a zero bit inserted at j's position, partner i + j (every lane busy)
+      for (uint32_t q = threadIdx.x; q < N / 2; q += kSortThreads) {
+        const uint32_t i = ((q & ~(j - 1)) << 1) | (q & (j - 1));
+        const uint32_t p = i + j;
+        const uint64_t a = sk[i], b = sk[p];
+        const uint32_t ia = si[i], ib = si[p];
+        const bool gt = a > b || (a == b && ia > ib);
+        if (gt == ((i & k) == 0)) {
+          sk[i] = b;
+          sk[p] = a;
+          si[i] = ib;
+          si[p] = ia;
         }
       }
       __syncthreads();
