@@ -263,24 +263,93 @@ __device__ __forceinline__ void bitonic_pairs(uint64_t* sk, uint32_t* si, uint32
   }
 }
 
+// LSD radix sort, in LDS, of kSmallUpdate (key, index) pairs held in (sk[0], si[0]), over the 8 key
+// bytes; stable, so equal keys keep their index order.  8 passes that each move the pairs through LDS
+// once (a bitonic network over the same pairs takes 78 steps).  Wave w owns positions
+// [256 w, 256 w + 256), element r of lane l being 256 w + 64 r + l; ranks within a wave come from the
+// digit ballots (match_digit), in that order.  Returns the buffer (0 or 1) holding the result; ends
+// with a block barrier.
+constexpr int kChunkWaves = kSortThreads / 64;
+constexpr int kChunkItems = kSmallUpdate / kSortThreads;
+static_assert(kChunkItems * 64 * kChunkWaves == (int)kSmallUpdate, "each wave owns 64 x kChunkItems positions");
+struct RadixLds {
+  uint64_t sk[2][kSmallUpdate];
+  uint32_t si[2][kSmallUpdate];
+  uint32_t wcnt[kChunkWaves][256];
+  uint32_t boff[256];
+  uint32_t scan_sh[kChunkWaves + 1];
+};
+__device__ __forceinline__ int radix_sort_lds(RadixLds& L) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint64_t lt = (1ull << lane) - 1ull;
+  const uint32_t base = (uint32_t)w * 64 * kChunkItems;
+  int cur = 0;
+  for (int shift = 0; shift < 64; shift += 8) {  // uniform across the block
+    for (int i = threadIdx.x; i < kChunkWaves * 256; i += kSortThreads) (&L.wcnt[0][0])[i] = 0;
+    __syncthreads();  // the pairs (first pass) and the cleared counters
+    uint64_t k[kChunkItems];
+    uint32_t ix[kChunkItems], pkd[kChunkItems];
+#pragma unroll
+    for (int r = 0; r < kChunkItems; ++r) {
+      k[r] = L.sk[cur][base + r * 64 + lane];
+      ix[r] = L.si[cur][base + r * 64 + lane];
+    }
+#pragma unroll
+    for (int r = 0; r < kChunkItems; ++r) {
+      const uint32_t d = (uint32_t)(k[r] >> shift) & 255u;
+      const uint64_t peers = match_digit(d, ~0ull);
+      const uint32_t prior = L.wcnt[w][d];
+      if ((peers & lt) == 0) L.wcnt[w][d] = prior + (uint32_t)__popcll(peers);
+      pkd[r] = ((prior + (uint32_t)__popcll(peers & lt)) << 8) | d;
+    }
+    __syncthreads();
+    {
+      // per digit: each wave's start inside the digit, then the digit's start
+      const int d = threadIdx.x;
+      uint32_t run = 0;
+      if (d < 256) {
+#pragma unroll
+        for (int ww = 0; ww < kChunkWaves; ++ww) {
+          const uint32_t t = L.wcnt[ww][d];
+          L.wcnt[ww][d] = run;
+          run += t;
+        }
+      }
+      uint32_t total;
+      const uint32_t e = block_exclusive_scan<uint32_t, kSortThreads>(run, L.scan_sh, total);
+      if (d < 256) L.boff[d] = e;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kChunkItems; ++r) {
+      const uint32_t d = pkd[r] & 255u;
+      const uint32_t np = L.boff[d] + L.wcnt[w][d] + (pkd[r] >> 8);
+      L.sk[cur ^ 1][np] = k[r];
+      L.si[cur ^ 1][np] = ix[r];
+    }
+    cur ^= 1;
+    __syncthreads();  // every thread has read wcnt / boff before the next pass clears them
+  }
+  return cur;
+}
+
 // First stage of the merge sort of mid-size batches: chunk c of kSmallUpdate staged keys sorted by
-// (key, staging index) in LDS, written with the indices.
+// key with their staging indices.  Padding (KEY_MAX, ~0) has digit 255 in every pass and sorts after
+// every staged key.
 __global__ __launch_bounds__(kSortThreads) void chunk_sort_kernel(const uint64_t* __restrict__ pk, uint32_t n,
                                                                   uint64_t* __restrict__ ko,
                                                                   uint32_t* __restrict__ io) {
-  __shared__ uint64_t sk[kSmallUpdate];
-  __shared__ uint32_t si[kSmallUpdate];
+  __shared__ RadixLds L;
   const uint32_t c0 = blockIdx.x * kSmallUpdate;
   const uint32_t cnt = n - c0 < kSmallUpdate ? n - c0 : kSmallUpdate;
   for (uint32_t i = threadIdx.x; i < kSmallUpdate; i += kSortThreads) {
-    sk[i] = i < cnt ? pk[c0 + i] : KEY_MAX;
-    si[i] = i < cnt ? c0 + i : 0xFFFFFFFFu;
+    L.sk[0][i] = i < cnt ? pk[c0 + i] : KEY_MAX;
+    L.si[0][i] = i < cnt ? c0 + i : 0xFFFFFFFFu;
   }
-  __syncthreads();
-  bitonic_pairs(sk, si, kSmallUpdate);
+  const int cur = radix_sort_lds(L);
   for (uint32_t i = threadIdx.x; i < cnt; i += kSortThreads) {
-    ko[c0 + i] = sk[i];
-    io[c0 + i] = si[i];
+    ko[c0 + i] = L.sk[cur][i];
+    io[c0 + i] = L.si[cur][i];
   }
 }
 
@@ -320,25 +389,35 @@ __global__ void gather_by_index_kernel(const uint64_t* __restrict__ v, const uin
   if (i < n) out[i] = v[ix[i]];
 }
 
-// Bitonic sort of (key, staging index) in LDS; the index breaks ties, so the last write of a key
-// ends its run of equal keys.  Padding (KEY_MAX, ~0) sorts after every real key (KEY_MAX is never
-// staged).
+// Sort of (key, staging index) in LDS, stable in the index, so the last write of a key ends its run
+// of equal keys; then the last of each run is kept.  Padding (KEY_MAX, ~0) sorts after every real key
+// (KEY_MAX is never staged).
 __global__ __launch_bounds__(kSortThreads) void small_sort_dedupe_kernel(const uint64_t* __restrict__ pk,
                                                                          const uint64_t* __restrict__ pv, uint32_t n,
                                                                          uint64_t* __restrict__ uk,
                                                                          uint64_t* __restrict__ uv,
                                                                          uint32_t* __restrict__ dm) {
-  __shared__ uint64_t sk[kSmallUpdate];
-  __shared__ uint32_t si[kSmallUpdate];
+  __shared__ RadixLds L;
   __shared__ uint32_t scan_sh[kSortThreads / 64 + 1];
+  // up to 1024 keys: the bitonic network over the next power of two; above: the radix sort (8 passes
+  // over all kSmallUpdate slots, padding included), which is cheaper there
+  const bool radix = n > 1024;
   uint32_t N = 1;
   while (N < n) N <<= 1;
+  if (radix) N = kSmallUpdate;
   for (uint32_t i = threadIdx.x; i < N; i += kSortThreads) {
-    sk[i] = i < n ? pk[i] : KEY_MAX;
-    si[i] = i < n ? i : 0xFFFFFFFFu;
+    L.sk[0][i] = i < n ? pk[i] : KEY_MAX;
+    L.si[0][i] = i < n ? i : 0xFFFFFFFFu;
   }
-  __syncthreads();
-  bitonic_pairs(sk, si, N);
+  int cur = 0;
+  if (radix) {  // block-uniform
+    cur = radix_sort_lds(L);
+  } else {
+    __syncthreads();
+    bitonic_pairs(L.sk[0], L.si[0], N);
+  }
+  const uint64_t* sk = L.sk[cur];
+  const uint32_t* si = L.si[cur];
   constexpr int kPer = kSmallUpdate / kSortThreads;  // consecutive elements per thread
   const uint32_t b0 = threadIdx.x * kPer;
   bool keep[kPer];
