@@ -10,11 +10,12 @@
 //
 // One pass = histogram kernel (per-tile digit counts, digit-major) -> digit scan (one block per
 // digit: exclusive scan of the digit's counts along the tiles, and the digit's total) -> scatter
-// kernel (the digit starts are the exclusive scan of the 256 totals, done in each block) (wave-level digit match via 8 ballots gives each item its
-// stable rank inside its wave; per-wave counters are scanned across the block; the tile is then
-// staged in LDS in digit order so the global writes of each digit run are coalesced).  Tiles are
-// mapped to blocks so that each XCD processes a contiguous range of tiles: the digit runs of
-// neighbouring tiles share cache lines at their ends, and those partial lines then meet in one L2.
+// kernel.  In the scatter, the digit starts are the exclusive scan of the 256 totals (done in each
+// block); a wave-level digit match (8 ballots) gives each item its stable rank inside its wave;
+// per-wave counters are scanned across the block; the tile is then staged in LDS in digit order so
+// the global writes of each digit run are coalesced.  Tiles are mapped to blocks so that each XCD
+// processes a contiguous range of tiles: the digit runs of neighbouring tiles share cache lines at
+// their ends, and those partial lines then meet in one L2.
 //
 // Returning results to input order: a search in PSA order produces one result per sorted query.
 // Instead of sorting the results by position (three more passes over 24 position bits) or
