@@ -400,6 +400,7 @@ void free_tree(hm_tree* t) {
   if (t->rl_buf) (void)hipFree(t->rl_buf);
   if (t->bl_child) (void)hipFree(t->bl_child);
   if (t->publish_ev) (void)hipEventDestroy(t->publish_ev);
+  if (t->na_ev) (void)hipEventDestroy(t->na_ev);
   if (t->own) (void)hipStreamDestroy(t->own);
   delete t;
 }
@@ -450,7 +451,8 @@ static int create_tree(const hm_params* p, hm_tree** out) {
     rc = set_err(HM_EDEVICE, "no HIP device");
   else if (hipStreamCreateWithFlags(&t->own, hipStreamNonBlocking) != hipSuccess)
     rc = set_err(HM_EDEVICE, "hipStreamCreate failed");
-  else if (hipEventCreateWithFlags(&t->publish_ev, hipEventDisableTiming) != hipSuccess)
+  else if (hipEventCreateWithFlags(&t->publish_ev, hipEventDisableTiming) != hipSuccess ||
+           hipEventCreateWithFlags(&t->na_ev, hipEventDisableTiming) != hipSuccess)
     rc = set_err(HM_EDEVICE, "hipEventCreate failed");
   if (rc) {
     free_tree(t);
@@ -956,6 +958,7 @@ int hm_update(hm_tree* t, const uint64_t* keys, const uint64_t* vals, uint64_t n
 static int wait_for_readers(hm_tree* t, hipStream_t s) {
   for (auto& kv : t->ws)
     if (kv.second.last_use) HM_TRY(hipStreamWaitEvent(s, kv.second.last_use, 0));
+  if (t->na_valid) HM_TRY(hipStreamWaitEvent(s, t->na_ev, 0));  // the last allocation-free search
   return HM_OK;
 }
 
@@ -1179,6 +1182,13 @@ int reserve_relayout(hm_tree* t, hipStream_t s) {
   return ensure_rl(t, rl, s);
 }
 
+// Drop the staged batch of an epoch that failed.  The caller holds t->pub_mu; the staging copies
+// still in flight complete first, so the slots can be reused.
+void discard_staged(hm_tree* t) {
+  if (t->stage_valid) (void)hipEventSynchronize(t->stage_ev);
+  t->pend_n = 0;
+}
+
 // Make the prepared epoch visible: merge in place (after every earlier search) or swap in the
 // standby layout; later searches wait for it on the device.  The caller holds t->pub_mu.
 int publish_commit(hm_tree* t, hipStream_t s, const PublishPlan& plan, int* rebuilt) {
@@ -1218,8 +1228,9 @@ int hm_epoch_publish(hm_tree* t, void* stream, int* out_rebuilt) {
   std::lock_guard<std::mutex> g(t->pub_mu);
   PublishPlan plan;
   int rc = publish_prepare(t, s, &plan);
-  if (rc) return rc;
-  return publish_commit(t, s, plan, out_rebuilt);
+  if (!rc) rc = publish_commit(t, s, plan, out_rebuilt);
+  if (rc) discard_staged(t);  // the failed epoch is dropped, not carried into the next one
+  return rc;
 }
 
 // ---- binary snapshot ------------------------------------------------------------------------
@@ -1255,8 +1266,18 @@ struct File {
 };
 }  // namespace
 
+static int tree_save_impl(const hm_tree* t, const char* path);
+
 int hm_tree_save(const hm_tree* t, const char* path) {
   if (!t || !path) return set_err(HM_EINVAL, "NULL argument");
+  try {
+    return tree_save_impl(t, path);
+  } catch (const std::bad_alloc&) {
+    return set_err(HM_ENOMEM, "%s: host allocation failed", path);
+  }
+}
+
+static int tree_save_impl(const hm_tree* t, const char* path) {
   if (t->pend_n) return set_err(HM_EINVAL, "%llu staged updates: publish before saving", (unsigned long long)t->pend_n);
   std::vector<uint64_t> leaves(t->n_leaves * 16), inner((size_t)t->n_inner * t->NS);
   std::vector<uint32_t> prefix(t->n_inner);
@@ -1287,9 +1308,7 @@ int hm_tree_save(const hm_tree* t, const char* path) {
   return HM_OK;
 }
 
-int hm_tree_load(const char* path, const hm_params* p, hm_tree** out) {
-  if (!path || !out) return set_err(HM_EINVAL, "NULL argument");
-  *out = nullptr;
+static int tree_load_impl(const char* path, const hm_params* p, hm_tree** out) {
   hm_params q;
   hm_params_init(&q);
   if (p) q = *p;
@@ -1306,6 +1325,25 @@ int hm_tree_load(const char* path, const hm_params* p, hm_tree** out) {
   if (h.node_slots != (uint32_t)node_slots((int)h.fanout) || h.n_leaves == 0 || h.n_leaves >= (1ull << 32) ||
       h.n_keys > h.n_leaves * LEAF_KEYS)
     return set_err(HM_EINVAL, "%s: inconsistent snapshot header", path);
+  // no hash covers the header: before anything is sized from it, the file must be exactly as long
+  // as the header says, and the level table must be the one the leaf count defines
+  {
+    uint32_t ls[MAX_LEVELS + 1], ni = 0;
+    const int height = plan_levels(h.n_leaves, (int)h.fanout, ls, &ni);
+    if (height < 0 || (uint32_t)height != h.height || ni != h.n_inner)
+      return set_err(HM_EINVAL, "%s: header level table does not match %llu leaves", path,
+                     (unsigned long long)h.n_leaves);
+    for (int i = 0; i <= MAX_LEVELS; ++i)
+      if (ls[i] != h.level_start[i]) return set_err(HM_EINVAL, "%s: header level table is corrupt", path);
+    const uint64_t want = sizeof(SnapHeader) + h.n_leaves * sizeof(Leaf) +
+                          (uint64_t)h.n_inner * ((uint64_t)h.node_slots * 8 + 4);
+    if (fseeko(f.f, 0, SEEK_END) != 0) return set_err(HM_EINVAL, "%s: cannot seek", path);
+    const off_t size = ftello(f.f);
+    if (size < 0 || (uint64_t)size != want)
+      return set_err(HM_EINVAL, "%s: %lld bytes, the header describes %llu", path, (long long)size,
+                     (unsigned long long)want);
+    if (fseeko(f.f, (off_t)sizeof(SnapHeader), SEEK_SET) != 0) return set_err(HM_EINVAL, "%s: cannot seek", path);
+  }
   std::vector<uint64_t> leaves(h.n_leaves * 16);
   if (fread(leaves.data(), 8, leaves.size(), f.f) != leaves.size() ||
       fnv1a(leaves.data(), leaves.size() * 8) != h.fnv_leaves)
@@ -1332,6 +1370,9 @@ int hm_tree_load(const char* path, const hm_params* p, hm_tree** out) {
   if (q.mem_budget && need > q.mem_budget)
     return set_err(HM_EBUDGET, "tree needs %llu device bytes, budget is %llu", (unsigned long long)need,
                    (unsigned long long)q.mem_budget);
+  // host buffers first: nothing below may throw once the tree object exists
+  std::vector<uint64_t> inner((size_t)h.n_inner * h.node_slots), finner(inner.size());
+  std::vector<uint32_t> prefix(h.n_inner), fprefix(prefix.size());
   hm_tree* t = nullptr;
   rc = create_tree(&q, &t);
   if (rc) return rc;
@@ -1349,8 +1390,6 @@ int hm_tree_load(const char* path, const hm_params* p, hm_tree** out) {
   rc = rebuild_inner(t, t->own);
   if (rc) return fail(rc);
   // the stored inner levels and prefix must be the ones the leaves define
-  std::vector<uint64_t> inner((size_t)t->n_inner * t->NS), finner(inner.size());
-  std::vector<uint32_t> prefix(t->n_inner), fprefix(prefix.size());
   if (t->n_inner != h.n_inner || (uint32_t)t->height != h.height ||
       fread(finner.data(), 8, finner.size(), f.f) != finner.size() ||
       fread(fprefix.data(), 4, fprefix.size(), f.f) != fprefix.size())
@@ -1364,6 +1403,18 @@ int hm_tree_load(const char* path, const hm_params* p, hm_tree** out) {
   t->epoch = h.epoch;
   *out = t;
   return HM_OK;
+}
+
+int hm_tree_load(const char* path, const hm_params* p, hm_tree** out) {
+  if (!path || !out) return set_err(HM_EINVAL, "NULL argument");
+  *out = nullptr;
+  try {
+    return tree_load_impl(path, p, out);
+  } catch (const std::bad_alloc&) {
+    return set_err(HM_ENOMEM, "%s: host allocation failed", path);
+  } catch (const std::exception& e) {
+    return set_err(HM_EINVAL, "%s: %s", path, e.what());
+  }
 }
 
 int hm_tree_stats(const hm_tree* t, hm_stats* st) {
@@ -1406,9 +1457,19 @@ int hm_psa_sort(const uint64_t* keys, uint64_t n, int begin_bit, int end_bit, ui
   if (n > kMaxBatch) return set_err(HM_EINVAL, "too many keys");
   hipStream_t s = (hipStream_t)stream;
   const size_t kq = align256(n * 8), kp = align256(n * 4);
-  char* buf = nullptr;
-  HM_TRY(hipMalloc(&buf, 3 * kq + 2 * kp + align256(radix_tmp_bytes(n))));
-  char* cur = buf;
+  struct Scratch {  // freed on every return path, after the stream's work on it
+    char* p = nullptr;
+    hipStream_t s = nullptr;
+    ~Scratch() {
+      if (p) {
+        (void)hipStreamSynchronize(s);
+        (void)hipFree(p);
+      }
+    }
+  } scratch;
+  scratch.s = s;
+  HM_TRY(hipMalloc(&scratch.p, 3 * kq + 2 * kp + align256(radix_tmp_bytes(n))));
+  char* cur = scratch.p;
   const uint64_t* dk = keys;
   if (!is_device_ptr(keys)) {
     HM_TRY(hipMemcpyAsync(cur, keys, n * 8, hipMemcpyHostToDevice, s));
@@ -1429,7 +1490,6 @@ int hm_psa_sort(const uint64_t* keys, uint64_t n, int begin_bit, int end_bit, ui
   HM_TRY(hipMemcpyAsync(keys_out, which ? k1 : k0, n * 8, dko ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
   HM_TRY(hipMemcpyAsync(pos_out, which ? p1 : p0, n * 4, dpo ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
   HM_TRY(hipStreamSynchronize(s));
-  HM_TRY(hipFree(buf));
   return HM_OK;
 }
 

@@ -311,6 +311,9 @@ void run_updates(hm_server* sv, Worker& w, std::vector<ReqPtr>& ups, std::vector
     rc = publish_commit(t, w.s, plan, &rebuilt);
     *epoch = tree_epoch(t);
   }
+  // a failed epoch is dropped (its requests are reported failed below), so it is not applied with
+  // the next one
+  if (rc) discard_staged(t);
   if (!rc && hipStreamSynchronize(w.s) != hipSuccess) rc = set_err(HM_EDEVICE, "update stream failed");
   if (rc)
     for (int& st : status)

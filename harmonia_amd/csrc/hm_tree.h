@@ -68,6 +68,10 @@ struct hm_tree {
   hipStream_t own = nullptr;
   hipEvent_t publish_ev = nullptr;
   bool publish_valid = false;
+  // the latest allocation-free search (hm_search_noalloc), so a publish waits for its reads; created
+  // with the tree because that entry point creates nothing
+  hipEvent_t na_ev = nullptr;
+  bool na_valid = false;
   uint64_t epoch = 0;
   uint64_t relayouts = 0;
   uint64_t* pend_k = nullptr;
@@ -145,6 +149,8 @@ int range_impl(hm_tree* t, const uint64_t* start, const uint32_t* len, uint64_t 
                bool mapped);
 int publish_prepare(hm_tree* t, hipStream_t s, PublishPlan* plan);
 int publish_commit(hm_tree* t, hipStream_t s, const PublishPlan& plan, int* rebuilt);
+// Drop the staged batch after a failed prepare or commit (caller holds t->pub_mu).
+void discard_staged(hm_tree* t);
 // Allocate the standby layout and re-layout scratch now (best effort under the memory budget).
 int reserve_relayout(hm_tree* t, hipStream_t s);
 // Size the update staging and publish scratch for batches of up to n keys now.

@@ -38,19 +38,23 @@ int time_pipeline(hm_tree* t, const SearchLaunch& sl, int bits, int mode, const 
   const TreeView tv = t->view();
   int rc = enqueue_pipeline(tv, sl, bits, mode, b.q, n, nullptr, pb, b.vals, b.found, nullptr, s);
   if (rc) return rc;
-  hipEvent_t e0, e1;
-  HM_TRY(hipEventCreate(&e0));
-  HM_TRY(hipEventCreate(&e1));
-  HM_TRY(hipEventRecord(e0, s));
+  struct Events {  // destroyed on every return path
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~Events() {
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+    }
+  } ev;
+  HM_TRY(hipEventCreate(&ev.e0));
+  HM_TRY(hipEventCreate(&ev.e1));
+  HM_TRY(hipEventRecord(ev.e0, s));
   for (int r = 0; r < reps && !rc; ++r) rc = enqueue_pipeline(tv, sl, bits, mode, b.q, n, nullptr, pb, b.vals,
                                                               b.found, nullptr, s);
-  HM_TRY(hipEventRecord(e1, s));
-  HM_TRY(hipEventSynchronize(e1));
-  float t_ms = 0;
-  HM_TRY(hipEventElapsedTime(&t_ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  HM_TRY(hipEventRecord(ev.e1, s));
+  HM_TRY(hipEventSynchronize(ev.e1));
   if (rc) return rc;
+  float t_ms = 0;
+  HM_TRY(hipEventElapsedTime(&t_ms, ev.e0, ev.e1));
   *ms = t_ms / reps;
   return HM_OK;
 }
@@ -110,9 +114,15 @@ int tune_tree(hm_tree* t, const uint64_t* sample, uint64_t n, uint32_t flags, hm
             ++trials;
             if (ms < bp.ms) bp = Best{sl, bits, mode, ms};
           }
-  // sort threshold: smallest power-of-two batch (<= n) where the PSA winner beats the direct winner
+  // sort threshold: the smallest batch where the PSA winner beats the direct winner, searched over
+  // 1024 * 2^(k/2) (a factor of ~1.41 per step, so a crossover between two powers of two is not
+  // rounded up to the next one) and then the sample size n itself; if no smaller batch wins but PSA
+  // won the grid at n, the threshold is n
   uint64_t threshold = ~0ull;
-  for (uint64_t m = 1024; m <= n; m *= 2) {
+  std::vector<uint64_t> sizes;
+  for (uint64_t m = 1024; m < n; m = (m * 181 + 127) / 128) sizes.push_back(m);  // 181/128 ~ sqrt(2)
+  sizes.push_back(n);
+  for (uint64_t m : sizes) {
     float md = 0, mp = 0;
     if ((rc = time_pipeline(t, bd.sl, 0, 0, b, m, s, reps, &md))) return rc;
     if ((rc = time_pipeline(t, bp.sl, bp.bits, bp.mode, b, m, s, reps, &mp))) return rc;
@@ -122,6 +132,7 @@ int tune_tree(hm_tree* t, const uint64_t* sample, uint64_t n, uint32_t flags, hm
       break;
     }
   }
+  if (threshold == ~0ull && bp.ms < bd.ms) threshold = n;
   // apply
   t->cfg_direct = SearchCfg{bd.sl.group, bd.sl.queries_per_group, bd.sl.blocks_per_cu, HM_OUT_UNPERMUTE};
   t->cfg_psa = SearchCfg{bp.sl.group, bp.sl.queries_per_group, bp.sl.blocks_per_cu, bp.mode};

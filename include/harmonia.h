@@ -86,7 +86,7 @@ typedef struct hm_search_opts {
   uint32_t* out_pos;       /* with HM_OUT_PSA_ORDER: out_pos[i] = input index of result i */
 } hm_search_opts;
 
-/* Fill *p with defaults: F=64, fill 8, LDS auto, group auto, PSA 16 bits, threshold 2^16. */
+/* Fill *p with defaults: F=64, fill 8, LDS auto, group auto, PSA 16 bits, threshold 2^20. */
 int hm_params_init(hm_params* p);
 
 /* Bulk build.  Replaces the insert loop + relayout of the reference:
