@@ -13,6 +13,9 @@ Parity pinning (what checks each function; details in tests/test_oracle.py):
                  device build compared slot for slot
   search_cfb     regular_bpt find + Harmonia BPlusTree::search (oracle/_ref), golden vectors,
                  closed form search_sorted, brute force
+  search_harmonia the same checks as search_cfb (reference BPlusTree::search and regular find via
+                 oracle/_ref, golden vectors, closed form), the AVX2 and scalar paths against each
+                 other, and a planted index-step mistake that those checks catch
   search_sorted  brute force (python set/dict) on tiny inputs
   psa_sort       python's stable sorted() on tiny inputs, permutation + stability invariants
   range_walk     regular find_range + BPlusTree::range_search (oracle/_ref), golden vectors,
@@ -52,6 +55,9 @@ def load():
                                          _vp, _vp, _vp]),
         "hmo_search_cfb": (None, [_vp, _vp, _vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _vp,
                                   ctypes.c_uint64, _vp, _vp, ctypes.c_int]),
+        "hmo_search_harmonia": (None, [_vp, _vp, _vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _vp,
+                                       ctypes.c_uint64, _vp, _vp, ctypes.c_int, ctypes.c_int]),
+        "hmo_have_avx2": (ctypes.c_int, []),
         "hmo_search_sorted": (None, [_vp, _vp, ctypes.c_uint64, _vp, ctypes.c_uint64, _vp, _vp]),
         "hmo_psa_sort": (ctypes.c_int, [_vp, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, _vp, _vp]),
         "hmo_range_walk": (None, [_vp, _vp, _vp, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int,
@@ -140,6 +146,22 @@ def search_cfb(layout, q, nthreads=1):
     lib.hmo_search_cfb(_p(layout.inner), _p(layout.prefix), _p(layout.leaves), layout.n_inner, layout.height,
                        layout.fanout, _p(q), len(q), _p(vals), _p(found), nthreads)
     return vals, found
+
+
+def search_harmonia(layout, q, nthreads=1, avx2=True):
+    """Harmonia's BPlusTree::search steps (index-then-key, 4 + 4 compares; backward leaf scan) over
+    the layout, OpenMP over queries as search_cpu_test does."""
+    lib = load()
+    q = _c(q, _u64)
+    vals = np.empty(len(q), dtype=_u64)
+    found = np.empty(len(q), dtype=np.uint8)
+    lib.hmo_search_harmonia(_p(layout.inner), _p(layout.prefix), _p(layout.leaves), layout.n_inner, layout.height,
+                            layout.fanout, _p(q), len(q), _p(vals), _p(found), nthreads, 1 if avx2 else 0)
+    return vals, found
+
+
+def have_avx2():
+    return bool(load().hmo_have_avx2())
 
 
 def search_sorted(sorted_keys, vals, q):
@@ -237,6 +259,8 @@ class _RefLib:
             L.ref_rbpt_insert.argtypes = [_vp, _vp, _vp, ctypes.c_uint64]
             L.ref_rbpt_find.restype = None
             L.ref_rbpt_find.argtypes = [_vp, _vp, ctypes.c_uint64, _vp, _vp]
+            L.ref_rbpt_find_omp.restype = None
+            L.ref_rbpt_find_omp.argtypes = [_vp, _vp, ctypes.c_uint64, _vp, _vp, ctypes.c_int]
             L.ref_rbpt_find_range.restype = ctypes.c_uint64
             L.ref_rbpt_find_range.argtypes = [_vp, ctypes.c_int64, ctypes.c_int64, _vp, _vp, ctypes.c_uint64,
                                               ctypes.c_uint64]
@@ -248,6 +272,8 @@ class _RefLib:
             L.ref_h_build.argtypes = [_vp, _vp, ctypes.c_uint64]
             L.ref_h_search.restype = None
             L.ref_h_search.argtypes = [_vp, _vp, ctypes.c_uint64, _vp]
+            L.ref_h_search_omp.restype = None
+            L.ref_h_search_omp.argtypes = [_vp, _vp, ctypes.c_uint64, _vp, ctypes.c_int]
             L.ref_h_range.restype = ctypes.c_uint64
             L.ref_h_range.argtypes = [_vp, ctypes.c_int64, ctypes.c_int64, _vp, _vp, ctypes.c_uint64]
             L.ref_h_height.restype = ctypes.c_int
@@ -284,11 +310,15 @@ class RefRegularTree:
         self.root = self.r.lib.ref_rbpt_insert(self.root, _p(k), _p(v), len(k))
         self.n += len(k)
 
-    def find(self, q):
+    def find(self, q, nthreads=0):
+        """find() per query; nthreads > 0 runs search_cpu_multithread's OpenMP loop."""
         q = _c(q, _u64).view(np.int64)
         ov = np.empty(len(q), dtype=np.int64)
         of = np.empty(len(q), dtype=np.uint8)
-        self.r.lib.ref_rbpt_find(self.root, _p(q), len(q), _p(ov), _p(of))
+        if nthreads > 0:
+            self.r.lib.ref_rbpt_find_omp(self.root, _p(q), len(q), _p(ov), _p(of), nthreads)
+        else:
+            self.r.lib.ref_rbpt_find(self.root, _p(q), len(q), _p(ov), _p(of))
         return ov.view(_u64), of
 
     def find_range(self, lo, hi, cap):
@@ -310,11 +340,15 @@ class RefHarmoniaTree:
         v = _c(vals, _u64).view(np.int64)
         self.t = self.r.lib.ref_h_build(_p(k), _p(v), len(k))
 
-    def search(self, q):
-        """Values as int64 (-1 = not found, the reference's sentinel)."""
+    def search(self, q, nthreads=0):
+        """Values as int64 (-1 = not found, the reference's sentinel); nthreads > 0 runs
+        search_cpu_test's OpenMP loop."""
         q = _c(q, _u64).view(np.int64)
         ov = np.empty(len(q), dtype=np.int64)
-        self.r.lib.ref_h_search(self.t, _p(q), len(q), _p(ov))
+        if nthreads > 0:
+            self.r.lib.ref_h_search_omp(self.t, _p(q), len(q), _p(ov), nthreads)
+        else:
+            self.r.lib.ref_h_search(self.t, _p(q), len(q), _p(ov))
         return ov
 
     def range_search(self, lo, hi, cap):

@@ -13,6 +13,16 @@
  *                       `while (i < F-1 && key >= k[i]) i++` (upper_bound), next = prefix[node] + i,
  *                       then an equality scan of the leaf (regular find, H/regular_bpt/regular_bpt.cpp:383-393);
  *                       the OpenMP form restates search_cpu_multithread (H/regular_bpt/main.cpp:92-116);
+ *   - hmo_search_harmonia: Harmonia's BPlusTree::search (H/src/bpt.cpp:673-680) over this layout:
+ *                       per inner node the index-then-key step of getChildIdx_avx2_version2
+ *                       (H/src/avx.cpp:2-54: 4 index slots compared at once, the next 4 only if all
+ *                       4 are <= key; then the same 4 + 4 over the chosen 8-key group), next =
+ *                       prefix[node] + slot (search_last_level_inner_node_PPItree,
+ *                       H/src/bpt-ppi.cpp:118-140), then findKey's backward equality scan of the
+ *                       leaf (H/src/bpt.h:252-274); the OpenMP form restates search_cpu_test
+ *                       (H/src/main.cpp:772-812).  The reference's keys are signed with padding
+ *                       LLONG_MAX; here keys are unsigned with padding 2^64-1, so the AVX2 signed
+ *                       compares run on keys with the top bit flipped (same order);
  *   - hmo_search_sorted: the closed form: lower_bound in the sorted key array + equality;
  *   - hmo_psa_sort:     cub::DeviceRadixSort::SortPairs(keys, pos, n, begin_bit, end_bit)
  *                       (H/regular_bpt/prefix-sort.cu:288-293) = stable sort of (key, pos) by the
@@ -184,6 +194,108 @@ void hmo_search_cfb(const uint64_t* inner, const uint32_t* prefix, const uint64_
     out_found[i] = (uint8_t)f;
   }
   (void)nthreads;
+}
+
+/* --- Harmonia's index-then-key node search (BPlusTree::search) --- */
+#if defined(__x86_64__)
+#include <immintrin.h>
+/* #entries <= key among 4 consecutive u64 (signed AVX2 compare after flipping the top bit) */
+__attribute__((target("avx2"))) static inline int le4_avx2(const uint64_t* p, __m256i vq) {
+  const __m256i flip = _mm256_set1_epi64x((long long)0x8000000000000000ull);
+  const __m256i v = _mm256_xor_si256(_mm256_loadu_si256((const __m256i*)p), flip);
+  const __m256i le = _mm256_or_si256(_mm256_cmpgt_epi64(vq, v), _mm256_cmpeq_epi64(vq, v));
+  return __builtin_popcount((unsigned)_mm256_movemask_epi8(le) & 0x01010101u);
+}
+/* rank inside a group of 4 + 4: the second half is compared only if the whole first half is <= key */
+__attribute__((target("avx2"))) static inline int le8_avx2(const uint64_t* p, __m256i vq) {
+  int k = le4_avx2(p, vq);
+  if (k == 4) k += le4_avx2(p + 4, vq);
+  return k;
+}
+__attribute__((target("avx2"))) static int harmonia_find_avx2(const hmo_view* t, uint64_t key, uint64_t* val) {
+  if (key == KEY_MAX) return 0;
+  const int IS = idx_slots(t->F), NS = node_slots(t->F);
+  const __m256i vq = _mm256_set1_epi64x((long long)(key ^ 0x8000000000000000ull));
+  uint64_t idx = 0;
+  for (uint32_t h = 0; h < t->height; ++h) {
+    const uint64_t* nd = t->inner + idx * NS;
+    int j = 0; /* index slots <= key, groups of 8 as 4 + 4 */
+    for (int g = 0; g < IS; g += 8) {
+      const int c = le8_avx2(nd + g, vq);
+      j += c;
+      if (c < 8) break;
+    }
+    const int s = 8 * j + le8_avx2(nd + IS + 8 * j, vq);
+    idx = t->prefix[idx] + (uint64_t)s;
+  }
+  const uint64_t* lf = t->leaves + (idx - t->n_inner) * 16;
+  for (int i = LEAF_KEYS - 1; i >= 0; --i) /* findKey scans backwards */
+    if (lf[i] == key) {
+      *val = lf[8 + i];
+      return 1;
+    }
+  return 0;
+}
+#endif
+
+/* the same steps without AVX2 */
+static int harmonia_find_scalar(const hmo_view* t, uint64_t key, uint64_t* val) {
+  if (key == KEY_MAX) return 0;
+  const int IS = idx_slots(t->F), NS = node_slots(t->F);
+  uint64_t idx = 0;
+  for (uint32_t h = 0; h < t->height; ++h) {
+    const uint64_t* nd = t->inner + idx * NS;
+    int j = 0;
+    while (j < IS && nd[j] <= key) ++j;
+    const uint64_t* g = nd + IS + 8 * j;
+    int c = 0;
+    while (c < 8 && g[c] <= key) ++c;
+    idx = t->prefix[idx] + (uint64_t)(8 * j + c);
+  }
+  const uint64_t* lf = t->leaves + (idx - t->n_inner) * 16;
+  for (int i = LEAF_KEYS - 1; i >= 0; --i)
+    if (lf[i] == key) {
+      *val = lf[8 + i];
+      return 1;
+    }
+  return 0;
+}
+
+/* use_avx2: 1 = AVX2 path (falls back to scalar on a CPU without AVX2), 0 = scalar path */
+void hmo_search_harmonia(const uint64_t* inner, const uint32_t* prefix, const uint64_t* leaves, uint32_t n_inner,
+                         uint32_t height, int F, const uint64_t* q, uint64_t m, uint64_t* out_vals,
+                         uint8_t* out_found, int nthreads, int use_avx2) {
+  hmo_view t = {inner, prefix, leaves, n_inner, height, F};
+#if defined(__x86_64__)
+  const int avx = use_avx2 && __builtin_cpu_supports("avx2");
+#else
+  const int avx = 0;
+  (void)use_avx2;
+#endif
+#ifdef _OPENMP
+  if (nthreads < 1) nthreads = 1;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+#endif
+  for (int64_t i = 0; i < (int64_t)m; ++i) {
+    uint64_t v = 0;
+#if defined(__x86_64__)
+    const int f = avx ? harmonia_find_avx2(&t, q[i], &v) : harmonia_find_scalar(&t, q[i], &v);
+#else
+    const int f = harmonia_find_scalar(&t, q[i], &v);
+#endif
+    out_vals[i] = f ? v : 0;
+    out_found[i] = (uint8_t)f;
+  }
+  (void)nthreads;
+}
+
+/* 1 if this CPU runs the AVX2 path */
+int hmo_have_avx2(void) {
+#if defined(__x86_64__)
+  return __builtin_cpu_supports("avx2") ? 1 : 0;
+#else
+  return 0;
+#endif
 }
 
 static uint64_t lower_bound_u64(const uint64_t* a, uint64_t n, uint64_t x) {

@@ -26,6 +26,14 @@ void ref_h_search(void* t, const int64_t* q, uint64_t m, int64_t* out_vals) {
   for (uint64_t i = 0; i < m; ++i) out_vals[i] = (int64_t)tree->search((key_t)q[i]);
 }
 
+// search_cpu_test (H/src/main.cpp:772-812): BPlusTree::search under an OpenMP parallel for over
+// the queries, `nthreads` threads.
+void ref_h_search_omp(void* t, const int64_t* q, uint64_t m, int64_t* out_vals, int nthreads) {
+  BPlusTree* tree = (BPlusTree*)t;
+#pragma omp parallel for num_threads(nthreads)
+  for (int64_t i = 0; i < (int64_t)m; ++i) out_vals[i] = (int64_t)tree->search((key_t)q[i]);
+}
+
 uint64_t ref_h_range(void* t, int64_t lo, int64_t hi, int64_t* out_keys, int64_t* out_vals, uint64_t cap) {
   std::vector<Record> v;
   ((BPlusTree*)t)->range_search((key_t)lo, (key_t)hi, &v);

@@ -34,6 +34,19 @@ void ref_rbpt_find(void* root, const int64_t* q, uint64_t m, int64_t* out_vals, 
   }
 }
 
+// search_cpu_multithread (H/regular_bpt/main.cpp:92-116): find() under an OpenMP parallel for
+// over the queries, `nthreads` threads.
+void ref_rbpt_find_omp(void* root, const int64_t* q, uint64_t m, int64_t* out_vals, uint8_t* out_found,
+                       int nthreads) {
+  node* r = (node*)root;
+#pragma omp parallel for num_threads(nthreads)
+  for (int64_t i = 0; i < (int64_t)m; ++i) {
+    record* rec = find(r, (key_t)q[i], false);
+    out_found[i] = rec ? 1 : 0;
+    out_vals[i] = rec ? (int64_t)rec->value : 0;
+  }
+}
+
 // Keys in [lo, hi]; writes at most cap pairs, returns the full count.
 uint64_t ref_rbpt_find_range(void* root, int64_t lo, int64_t hi, int64_t* out_keys, int64_t* out_vals,
                              uint64_t cap, uint64_t n_keys_total) {

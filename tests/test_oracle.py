@@ -38,6 +38,38 @@ def test_search_matches_reference_golden(name, F, fill):
         assert np.array_equal(np.where(f == 1, v.view(np.int64), -1), h)
 
 
+@pytest.mark.parametrize("name,F", [("search_rbpt64.npz", 64), ("search_rbpt128.npz", 128)])
+@pytest.mark.parametrize("fill", [8, 5, 1])
+@pytest.mark.parametrize("avx2", [True, False])
+def test_search_harmonia_matches_reference_golden(name, F, fill, avx2):
+    g = load_gold(name)
+    L = layout_for(g["keys"], g["vals"], F, fill)
+    v, f = oracle.search_harmonia(L, g["queries"], 2, avx2=avx2)
+    assert np.array_equal(f, g["rbpt_found"])
+    assert np.array_equal(v[f == 1], g["rbpt_vals"][f == 1])
+    assert np.all(v[f == 0] == 0)
+    if "harmonia_vals" in g:
+        assert np.array_equal(np.where(f == 1, v.view(np.int64), -1), g["harmonia_vals"])
+
+
+def test_search_harmonia_edge_keys():
+    """Queries equal to separators and index entries, one above and below them, and the key-space
+    ends: the AVX2 signed compares on top-bit-flipped keys must order like unsigned keys."""
+    keys = np.concatenate([np.array([0, 1, (1 << 63) - 1, 1 << 63, (1 << 64) - 2], dtype=U64),
+                           W.gen_keys(5000, 91)[0]])
+    keys = np.unique(keys)
+    sk, sv = keys, W.splitmix64_mix(keys)
+    for F in (32, 64, 128):
+        L = oracle.build_bfs(sk, sv, F, 3)
+        seps = L.inner[L.inner != KEY_MAX]
+        q = np.concatenate([seps, seps - U64(1), seps + U64(1), sk[:50], np.array([0, KEY_MAX, KEY_MAX - U64(1)],
+                                                                                 dtype=U64)])
+        ev, ef = oracle.search_sorted(sk, sv, q)
+        for avx2 in (True, False):
+            v, f = oracle.search_harmonia(L, q, 2, avx2=avx2)
+            assert np.array_equal(f, ef) and np.array_equal(v, ev)
+
+
 def test_search_sorted_matches_reference_golden():
     g = load_gold("search_rbpt64.npz")
     sk = np.sort(g["keys"])
@@ -102,6 +134,31 @@ def test_search_matches_reference_code(order, F):
     assert np.array_equal(f, rf) and np.array_equal(v[f == 1], rv[rf == 1])
     H = oracle.RefHarmoniaTree(keys, vals)
     assert np.array_equal(np.where(f == 1, v.view(np.int64), -1), H.search(q))
+    hv, hf = oracle.search_harmonia(L, q, 2)
+    assert np.array_equal(hf, f) and np.array_equal(hv, v)
+
+
+@needs_ref
+def test_c1_cpu_reference_paths_agree():
+    """The CPU configuration (2^20 keys, fanout 64, 2^20 queries: 90% hits, 10% misses, unsorted):
+    the reference's own regular_bpt find and Harmonia BPlusTree::search under their OpenMP loops
+    (search_cpu_multithread, search_cpu_test), the two restatements over the bulk-built layout and
+    the closed form give the same value / not-found per query."""
+    seed = W.config_seed(0)
+    keys, vals = W.gen_keys(1 << 20, seed)
+    sk = np.sort(keys)
+    sv = W.splitmix64_mix(sk)
+    q = W.point_queries(sk, 1 << 20, seed + 1)
+    ev, ef = oracle.search_sorted(sk, sv, q)
+    assert int(ef.sum()) == 943718
+    L = oracle.build_bfs(sk, sv, 64, 8)
+    for fn in (oracle.search_cfb, oracle.search_harmonia):
+        v, f = fn(L, q, 4)
+        assert np.array_equal(f, ef) and np.array_equal(v, ev)
+    rv, rf = oracle.RefRegularTree(keys, vals, 64).find(q, nthreads=4)
+    assert np.array_equal(rf, ef) and np.array_equal(rv[rf == 1], ev[ef == 1])
+    hv = oracle.RefHarmoniaTree(keys, vals).search(q, nthreads=4)
+    assert np.array_equal(np.where(ef == 1, ev.view(np.int64), -1), hv)
 
 
 @needs_ref
@@ -234,6 +291,22 @@ def test_planted_child_rule_mistake_fails_golden():
     v, f = oracle.search_cfb(L, g["queries"])
     assert np.array_equal(f, g["rbpt_found"])  # the unmutated layout does not
     assert oracle.check_layout(bad) != 0
+
+
+def test_planted_index_step_mistake_caught():
+    """search_harmonia's index step compares index slots 4..7 only when slots 0..3 are all <= key
+    (getChildIdx_avx2_version2).  Dropping that second half (emulated by padding slots 4..7, so
+    the step never counts them) sends every query of the upper half of a node to group 4 at most;
+    the reference golden vectors must catch it, while the key-only scan (search_cfb) is unaffected."""
+    g = load_gold("search_rbpt64.npz")
+    sk = np.sort(g["keys"])
+    L = oracle.build_bfs(sk, W.splitmix64_mix(sk), 64, 8)
+    bad = oracle.Layout(L.inner.copy(), L.prefix, L.leaves, L.height, L.fanout, L.level_start)
+    bad.inner.reshape(-1, oracle.node_slots(64))[:, 4:8] = KEY_MAX
+    v, f = oracle.search_harmonia(bad, g["queries"], 2)
+    assert not np.array_equal(f, g["rbpt_found"])
+    v, f = oracle.search_cfb(bad, g["queries"])
+    assert np.array_equal(f, g["rbpt_found"]) and np.array_equal(v[f == 1], g["rbpt_vals"][f == 1])
 
 
 def test_planted_prefix_mistake_caught():
