@@ -147,6 +147,8 @@ SIGNATURES = [
     ("hm_tree_destroy", ctypes.c_int, [vp]),
     ("hm_search", ctypes.c_int, [vp, vp, ctypes.c_uint64, vp, vp, vp]),
     ("hm_search_ex", ctypes.c_int, [vp, vp, ctypes.c_uint64, vp, vp, ctypes.POINTER(HmSearchOpts), vp]),
+    ("hm_search_noalloc", ctypes.c_int, [vp, vp, ctypes.c_uint64, vp, vp, vp]),
+    ("hm_search_noalloc_max", ctypes.c_uint64, []),
     ("hm_baseline_search", ctypes.c_int, [vp, ctypes.c_int, ctypes.c_uint32, vp, ctypes.c_uint64, vp, vp, vp]),
     ("hm_range", ctypes.c_int, [vp, vp, vp, ctypes.c_uint64, vp, vp, vp, vp, ctypes.c_uint64, vp]),
     ("hm_range_between", ctypes.c_int, [vp, vp, vp, ctypes.c_uint64, ctypes.c_uint32, vp, vp, vp, vp]),

@@ -109,8 +109,12 @@ hipEvent_t prof_event() {
 }
 }  // namespace
 
+thread_local int g_prof_muted = 0;
+ProfMute::ProfMute() { ++g_prof_muted; }
+ProfMute::~ProfMute() { --g_prof_muted; }
+
 ProfScope::ProfScope(const char* stage, hipStream_t s) : slot(-1), stream(s) {
-  if (!g_prof_on) return;
+  if (!g_prof_on || g_prof_muted) return;
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;  // no events in graphs
   std::lock_guard<std::mutex> g(g_prof_mu);

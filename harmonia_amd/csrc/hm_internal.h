@@ -56,6 +56,12 @@ struct ProfScope {
   int slot;
   hipStream_t stream;
 };
+// While one is alive, ProfScope records nothing on this thread (the allocation-free entry point:
+// profiling events come from hipEventCreate).
+struct ProfMute {
+  ProfMute();
+  ~ProfMute();
+};
 
 // ---- scan (hm_scan.hip) ----------------------------------------------------------------
 // Exclusive scan of n u32 / u64 values; out may alias in.  tmp must hold scan_tmp_bytes(n).

@@ -107,6 +107,20 @@ int hm_search(hm_tree* t, const uint64_t* q, uint64_t n, uint64_t* out_vals, uin
 int hm_search_ex(hm_tree* t, const uint64_t* q, uint64_t n, uint64_t* out_vals, uint8_t* out_found,
                  const hm_search_opts* opts, void* stream);
 
+/* Allocation-free batched search.  No counterpart in the reference (its drivers cudaMalloc and free
+ * every buffer per call, H/src/full-multi-prefix-sort-8thread.cu:338-342,478-491): this is for callers
+ * that cannot allocate device memory -- code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc.  Same results and pipeline as hm_search with the tree's defaults (results in
+ * input order), but the sort/un-sort scratch comes from a fixed 44 MiB array in the library's device
+ * code (one per GPU, held until the process ends), so the call makes no hipMalloc, hipFree or event
+ * creation.  Device buffers only, at most hm_search_noalloc_max() (2^20) queries per call, the tree's
+ * device current; larger batches or host pointers fail with HM_EINVAL.  All calls on one GPU share
+ * the array: calls that overlap in time race, so serialize them (one stream, or wait for the previous
+ * call).  Async on stream. */
+int hm_search_noalloc(hm_tree* t, const uint64_t* q, uint64_t n, uint64_t* out_vals, uint8_t* out_found,
+                      void* stream);
+uint64_t hm_search_noalloc_max(void);
+
 /* Comparator baselines for layout ablations (not a serving path): the reference's pointer-child
  * trees over the same keys and leaves, so only the inner layout differs.  The next node is a
  * dependent read of an explicit child index, child[node * F + slot], instead of prefix[node] + slot.

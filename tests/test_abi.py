@@ -16,7 +16,7 @@ HEADER = os.path.join(ROOT, "include", "harmonia.h")
 def declared_functions():
     src = open(HEADER).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"^\s*int\s+(hm_\w+)\s*\(", src, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|uint64_t)\s+(hm_\w+)\s*\(", src, flags=re.M)))
 
 
 def test_library_is_built():
