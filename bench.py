@@ -70,35 +70,166 @@ def stage_avg(lib, stage):
     return (ms.value / cnt.value if cnt.value else None), int(cnt.value)
 
 
-def cpu_baseline(tree, q_host, budget_s=12.0):
-    """The oracle's restatement of the reference CPU search (CFB::search under an OpenMP
-    parallel-for, H/regular_bpt/cfb.cpp:4-41, H/regular_bpt/main.cpp:92-116) on the same tree
-    layout, over a bounded sample of the same query stream."""
-    import oracle
+def cpu_model():
+    """CPU model name as lscpu prints it (falls back to /proc/cpuinfo)."""
+    import subprocess
 
-    e = tree.export()
-    s = e["stats"]
-    lay = oracle.Layout(e["inner"], e["prefix"], e["leaves"], s.height, s.fanout,
-                        np.array(s.level_start[:], dtype=np.uint32))
-    cores = len(os.sched_getaffinity(0))
-    sample = q_host[: 1 << 22]
-    oracle.search_cfb(lay, sample[: 1 << 16], cores)  # warm-up
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return None
+
+
+def _median_time(fn, runs=5):
+    """1 warm-up + `runs` timed calls; (median seconds, last result)."""
+    r = fn()
     times = []
-    t_all = time.perf_counter()
-    while time.perf_counter() - t_all < budget_s and len(times) < 5:
+    for _ in range(runs):
         t0 = time.perf_counter()
-        v, f = oracle.search_cfb(lay, sample, cores)
+        r = fn()
         times.append(time.perf_counter() - t0)
-    med = float(np.median(times))
-    return {"value": round(len(sample) / med / 1e6, 3), "unit": "Mqueries/s", "cores": cores, "kind": "port",
-            "sample": f"{len(sample)} queries (first 2^22 of the same unsorted stream) on the same 2^26-key "
-                      f"layout, median of {len(times)} runs, OpenMP static over queries"}
+    return float(np.median(times)), r
+
+
+def cpu_leg(args):
+    """The CPU reference path, run in a child process (OMP_PROC_BIND=spread set by the parent before
+    the OpenMP runtime starts, so binding never touches the GPU-driving process).  Every result is
+    compared with the sorted-array oracle before its time is reported.
+
+    * the headline workload: the oracle's build of the same 2^26-key fanout-64 layout (bit-identical to
+      the device build) and all 2^24 queries of the same unsorted stream, searched by the oracle's
+      restatements of Harmonia's BPlusTree::search under search_cpu_test's OpenMP loop
+      (H/src/main.cpp:772-812, H/src/bpt.cpp:673-680, H/src/avx.cpp:2-54) and of CFB::search under
+      search_cpu_multithread's (H/regular_bpt/main.cpp:92-116, H/regular_bpt/cfb.cpp:4-41);
+    * the CPU configuration (2^20 keys, fanout 64, 2^20 queries, 90% hits): the same two restatements
+      plus, where oracle/_ref is built, the reference's own regular_bpt find and Harmonia
+      BPlusTree::search compiled from its sources (trees built by its insert loop)."""
+    import oracle
+    from harmonia_amd import workloads as W
+
+    cores = len(os.sched_getaffinity(0))
+    res = {"unit": "Mqueries/s", "cores": cores, "kind": "port", "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+           "omp_proc_bind": os.environ.get("OMP_PROC_BIND"), "timing": "1 warm-up + 5 timed runs, median"}
+
+    def run_ports(lay, q, ev, ef):
+        out = {}
+        for name, fn in (("harmonia_search_port", lambda: oracle.search_harmonia(lay, q, cores)),
+                         ("regular_find_port", lambda: oracle.search_cfb(lay, q, cores))):
+            t, (v, f) = _median_time(fn)
+            if not (np.array_equal(f, ef) and np.array_equal(v, ev)):
+                raise SystemExit(f"cpu leg: {name} results differ from the oracle")
+            out[name] = round(len(q) / t / 1e6, 3)
+        return out
+
+    # headline workload
+    seed = W.config_seed(1)
+    keys, vals = W.gen_keys(1 << args.keys_log2, seed)
+    sk = np.sort(keys)
+    del keys, vals
+    sv = W.splitmix64_mix(sk)
+    q = W.point_queries(sk, 1 << args.queries_log2, seed + 7, order="unsorted")
+    lay = oracle.build_bfs(sk, sv, 64, 8)
+    ev, ef = oracle.search_sorted(sk, sv, q)
+    del sk, sv
+    paths = run_ports(lay, q, ev, ef)
+    del lay, q, ev, ef
+    res["paths"] = paths
+    res["value"] = max(paths.values())
+    res["sample"] = (f"all {1 << args.queries_log2} queries of the same unsorted stream on the same "
+                     f"2^{args.keys_log2}-key fanout-64 layout (built by the oracle from the same seed); value = "
+                     "the faster of the two restated reference CPU searches, OpenMP static over queries")
+    # CPU configuration: 2^20 keys, 2^20 queries
+    seed0 = W.config_seed(0)
+    keys, vals = W.gen_keys(1 << 20, seed0)
+    sk = np.sort(keys)
+    sv = W.splitmix64_mix(sk)
+    q = W.point_queries(sk, 1 << 20, seed0 + 7)
+    ev, ef = oracle.search_sorted(sk, sv, q)
+    c1 = {"keys": 1 << 20, "queries": 1 << 20, "fanout": 64, "hit_frac": 0.9}
+    c1.update(run_ports(oracle.build_bfs(sk, sv, 64, 8), q, ev, ef))
+    if oracle.ref_available():
+        rt = oracle.RefRegularTree(keys, vals, 64)
+        t, (v, f) = _median_time(lambda: rt.find(q, nthreads=cores))
+        if not (np.array_equal(f, ef) and np.array_equal(v, ev)):
+            raise SystemExit("cpu leg: reference regular_bpt find differs from the oracle")
+        c1["reference_regular_find"] = round(len(q) / t / 1e6, 3)
+        ht = oracle.RefHarmoniaTree(keys, vals)
+        t, hv = _median_time(lambda: ht.search(q, nthreads=cores))
+        hit = ef.astype(bool)
+        if not ((hv[~hit] == -1).all() and np.array_equal(hv[hit].view(np.uint64), ev[hit])):
+            raise SystemExit("cpu leg: reference Harmonia search differs from the oracle")
+        c1["reference_harmonia_search"] = round(len(q) / t / 1e6, 3)
+        c1["reference_kind"] = "reference (its CPU sources compiled in oracle/_ref, trees built by its insert loop)"
+    res["cpu_config"] = c1
+    print(json.dumps(res), flush=True)
+
+
+def cpu_baseline(args):
+    """Run cpu_leg in a child process with OMP_PROC_BIND=spread and one OpenMP thread per host core."""
+    import subprocess
+
+    env = dict(os.environ)
+    env["OMP_PROC_BIND"] = "spread"
+    env["OMP_NUM_THREADS"] = str(len(os.sched_getaffinity(0)))
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-leg", "--keys-log2", str(args.keys_log2),
+           "--queries-log2", str(args.queries_log2)]
+    r = subprocess.run(cmd, env=env, capture_output=True, text=True, timeout=900)
+    if r.returncode != 0:
+        log(r.stderr[-2000:])
+        raise SystemExit(f"cpu leg failed with status {r.returncode}")
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def gpu_cpu_config(dev, stream):
+    """The CPU configuration (2^20 keys, fanout 64, 2^20 unsorted queries) on the GPU: one hm_search
+    per step over device-resident queries, results in input order, checked against the oracle."""
+    import torch
+
+    import oracle
+    from harmonia_amd import BPlusTree
+    from harmonia_amd import workloads as W
+
+    seed0 = W.config_seed(0)
+    keys, vals = W.gen_keys(1 << 20, seed0)
+    sk = np.sort(keys)
+    q = W.point_queries(sk, 1 << 20, seed0 + 7)
+    tree = BPlusTree.from_keys(keys, vals, fanout=64)
+    dq = torch.from_numpy(q.view(np.int64)).to(dev)
+    ov = torch.empty(len(q), dtype=torch.int64, device=dev)
+    of = torch.empty(len(q), dtype=torch.uint8, device=dev)
+    fn = lambda: tree.search_batch(dq, out_vals=ov, out_found=of, stream=stream)
+    ms, _ = time_steps(fn, 20, 5, stream)
+    fn()
+    stream.synchronize()
+    ev, ef = oracle.search_sorted(sk, W.splitmix64_mix(sk), q)
+    ok = bool(np.array_equal(of.cpu().numpy(), ef) and np.array_equal(ov.cpu().numpy().view(np.uint64), ev))
+    tree.close()
+    if not ok:
+        log("CPU-CONFIG GPU RESULT CHECK FAILED")
+        sys.exit(1)
+    return {"gpu_hm_search_Mq_per_s": round(len(q) / ms / 1e3, 1), "gpu_ms_per_call": round(ms, 4),
+            "gpu_results_ok": ok}
 
 
 def batched_uneven(dev, n_batches=10000, keys_log2=25, seed=4):
     """Uneven batches (sizes log-uniform in [32, 2^20], per-batch distribution uniform / Zipf(0.99) /
     sorted runs of 4096) on an autotuned 2^25-key tree: one hm_search per batch (the plain path) against
-    the hipGraph-captured Batcher that packs batches into one replay."""
+    the hipGraph-captured batcher that packs batches into one replay.  Both loops run in the C++ driver
+    (harmonia_amd/libhm_batchbench.so); then every batch's results from a separate pass through the same
+    batcher are checked against the closed form (found exactly for keys of the set, value =
+    splitmix64_mix(key))."""
+    import ctypes
+
     import torch
 
     from harmonia_amd import Batcher, BPlusTree
@@ -108,9 +239,11 @@ def batched_uneven(dev, n_batches=10000, keys_log2=25, seed=4):
     keys, vals = W.gen_keys(1 << keys_log2, cseed)
     sk = np.sort(keys)
     pool_n = 1 << 24
+    n_miss = pool_n - int(round(0.9 * pool_n))
     pools = [W.point_queries(sk, pool_n, cseed + 1),
              W.zipf_queries(sk, pool_n, cseed + 2),
              W.point_queries(sk, pool_n, cseed + 3, order="runs4096")]
+    miss = [W.gen_misses(len(sk), n_miss, cseed + 1), None, W.gen_misses(len(sk), n_miss, cseed + 3)]
     # a sample as large as the batcher's buffer, so the tuner sees the batch size a flush runs at
     sample = np.concatenate([p[: (1 << 24) // 3] for p in pools])
     W.rng(cseed, 7).shuffle(sample)
@@ -118,62 +251,65 @@ def batched_uneven(dev, n_batches=10000, keys_log2=25, seed=4):
     tree, tune = BPlusTree.from_keys_tuned(keys, vals, sample, quick=True)
     tune_s = time.perf_counter() - t0
     log("autotune:", json.dumps(tune), f"{tune_s:.1f} s")
+    del sample, keys, vals
     dpools = [torch.from_numpy(p.view(np.int64)).to(dev) for p in pools]
-    del pools, keys, vals
     g = W.rng(cseed, 8)
-    sizes = W.log_uniform_sizes(n_batches, 32, 1 << 20, cseed)
+    sizes = W.log_uniform_sizes(n_batches, 32, 1 << 20, cseed).astype(np.uint64)
     dist = g.integers(0, 3, n_batches)
-    starts = [int(g.integers(0, pool_n - int(m) + 1)) for m in sizes]
+    starts = np.array([int(g.integers(0, pool_n - int(m) + 1)) for m in sizes], dtype=np.uint64)
+    base = np.array([p.data_ptr() for p in dpools], dtype=np.uint64)
+    ptrs = base[dist] + starts * np.uint64(8)
     total = int(sizes.sum())
+    cap = 1 << 24
+    stream = torch.cuda.Stream()
     ov = torch.empty(1 << 20, dtype=torch.int64, device=dev)
     of = torch.empty(1 << 20, dtype=torch.uint8, device=dev)
-    stream = torch.cuda.Stream()
-    check_idx = set(int(i) for i in g.integers(0, n_batches, 8))
-    plain_out = {}
-    # plain: one search call (its own launches) per batch
+    lib = ctypes.CDLL(os.path.join(ROOT, "harmonia_amd", "libhm_batchbench.so"))
+    lib.hm_batch_bench_run.restype = ctypes.c_int
+    lib.hm_batch_bench_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
+                                       ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(ctypes.c_uint64)]
+    plain_s, batched_s, flushes = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_uint64(0)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(n_batches):
-        m, d, s0 = int(sizes[i]), int(dist[i]), starts[i]
-        tree.search_batch(dpools[d][s0:s0 + m], out_vals=ov[:m], out_found=of[:m], stream=stream)
-        if i in check_idx:
-            stream.synchronize()
-            plain_out[i] = (ov[:m].cpu().numpy().copy(), of[:m].cpu().numpy().copy())
-    stream.synchronize()
-    plain_s = time.perf_counter() - t0
-    # batched: pack into the Batcher, one graph replay per full buffer
-    cap = 1 << 24
+    rc = lib.hm_batch_bench_run(tree._handle(), ptrs.ctypes.data, sizes.ctypes.data, n_batches, cap,
+                                stream.cuda_stream, ov.data_ptr(), of.data_ptr(), ctypes.byref(plain_s),
+                                ctypes.byref(batched_s), ctypes.byref(flushes))
+    if rc:
+        from harmonia_amd._lib import last_error
+        raise SystemExit(f"hm_batch_bench_run failed: {rc} {last_error()}")
+    # every batch checked, on a separate pass through a batcher of the same configuration
+    exp = []
+    for p, m in zip(pools, miss):
+        f = np.ones(len(p), dtype=bool) if m is None else ~np.isin(p, m)
+        exp.append((f.astype(np.uint8), np.where(f, W.splitmix64_mix(p), np.uint64(0))))
     b = Batcher(tree, cap, stream=stream)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    pending = []
-    fill = 0
-    mismatches = 0
-    for i in range(n_batches + 1):
-        if i == n_batches or fill + int(sizes[i]) > cap:
-            b.flush()
-            for j, off in pending:
-                if j in check_idx:
-                    v, f = b.copy_out(off, int(sizes[j]))
-                    pv, pf = plain_out[j]
-                    mismatches += int(not (np.array_equal(v.view(np.int64), pv) and np.array_equal(f, pf)))
-            pending = []
-            fill = 0
-            if i == n_batches:
-                break
-        m, d, s0 = int(sizes[i]), int(dist[i]), starts[i]
-        pending.append((i, b.add(dpools[d][s0:s0 + m])))
-        fill += m
-    b.sync()
-    batched_s = time.perf_counter() - t0
+    offs = np.zeros(n_batches, dtype=np.uint64)
+    i = checked = mismatches = 0
+    while i < n_batches:
+        k = b.add_many(ptrs[i:], sizes[i:], offsets=offs[i:])
+        b.flush()
+        b.sync()
+        fill = int(offs[i + k - 1] + sizes[i + k - 1])
+        v_all, f_all = b.copy_out(0, fill)
+        for j in range(i, i + k):
+            m, d, s0, o = int(sizes[j]), int(dist[j]), int(starts[j]), int(offs[j])
+            ok = (np.array_equal(f_all[o:o + m], exp[d][0][s0:s0 + m]) and
+                  np.array_equal(v_all[o:o + m], exp[d][1][s0:s0 + m]))
+            mismatches += 0 if ok else 1
+            checked += 1
+        i += k
     b.close()
     res = {"workload": f"{n_batches} batches, sizes log-uniform 32..2^20 (total {total} queries), per-batch "
-                       f"uniform / Zipf(0.99) / runs of 4096, 2^{keys_log2}-key tree",
-           "plain_qps": round(total / plain_s, 1), "batched_qps": round(total / batched_s, 1),
-           "speedup": round(plain_s / batched_s, 2), "plain_s": round(plain_s, 4), "batched_s": round(batched_s, 4),
-           "checked_batches": len(check_idx), "mismatches": mismatches, "autotune": tune,
-           "autotune_s": round(tune_s, 1)}
+                       f"uniform / Zipf(0.99) / runs of 4096, 2^{keys_log2}-key tree; both loops in C++",
+           "plain_qps": round(total / plain_s.value, 1), "batched_qps": round(total / batched_s.value, 1),
+           "speedup": round(plain_s.value / batched_s.value, 2), "plain_s": round(plain_s.value, 4),
+           "batched_s": round(batched_s.value, 4), "flushes": int(flushes.value),
+           "checked_batches": checked, "mismatches": mismatches, "autotune": tune, "autotune_s": round(tune_s, 1)}
     tree.close()
+    if mismatches:
+        log("UNEVEN-BATCH RESULT CHECK FAILED", mismatches)
+        sys.exit(1)
     return res
 
 
@@ -460,7 +596,9 @@ def main():
     ap.add_argument("--keys-log2", type=int, default=26)
     ap.add_argument("--queries-log2", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--variants", default="all", help="'all' or 'headline'")
+    ap.add_argument("--cpu-leg", action="store_true", help=argparse.SUPPRESS)  # the child process of cpu_baseline
+    ap.add_argument("--variants", default="all",
+                    help="'all', 'headline', or a comma-separated list of variant names (the headline always runs)")
     ap.add_argument("--no-batched", action="store_true", help="skip the uneven-batches comparison")
     ap.add_argument("--no-served", action="store_true", help="skip the served-load runs")
     ap.add_argument("--no-ablation", action="store_true", help="skip the layout / PSA / NTG ablation")
@@ -472,6 +610,9 @@ def main():
                     help="only run a plain 1-GiB streaming kernel (read 1 GiB, write 1 GiB), the reference point "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
     args = ap.parse_args()
+    if args.cpu_leg:
+        cpu_leg(args)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the tree is replicated per GPU "
               "and queries are sharded, so N GPUs is N independent copies of this run", file=sys.stderr)
@@ -525,16 +666,23 @@ def main():
 
     # (order, PSA bits, output mode): mode 1 = results in input order (the headline), 3 = PSA order plus
     # positions (the reference's own contract)
-    variants = [("unsorted", 16, 1), ("unsorted", 0, 1), ("runs4096", 16, 1), ("runs4096", 0, 1),
-                ("unsorted", 16, 3)]
+    # sorted kernel (PSA batches): 0 = the tree's choice (one descent per query), 2 = the leaf window
+    variants = [("unsorted", 16, 1, 0), ("unsorted", 16, 1, 2), ("unsorted", 0, 1, 0), ("runs4096", 16, 1, 0),
+                ("runs4096", 0, 1, 0), ("unsorted", 16, 3, 0)]
+    def vkey(order, psa, mode, skern):
+        return f"{order}_psa{psa}" + ("_psa_order" if mode == 3 else "") + ("_leaf_window_kernel" if skern == 2 else "")
+
     if args.variants == "headline":
-        variants = variants[:1]
+        variants = variants[:2]
+    elif args.variants != "all":  # a comma-separated list of variant names
+        want = set(args.variants.split(","))
+        variants = [v for v in variants if vkey(*v) in want or v == variants[0]]
     out_pos = torch.empty(n_q, dtype=torch.int32, device=dev)
     results = {}
-    for order, psa, mode in variants:
-        fn = lambda o=order, p=psa, m=mode: tree.search_batch(
+    for order, psa, mode, skern in variants:
+        fn = lambda o=order, p=psa, m=mode, k=skern: tree.search_batch(
             dq[o], out_vals=out_v, out_found=out_f, stream=stream, psa_bits=p, output_mode=m if p else 0,
-            out_pos=out_pos if m == 3 else None)
+            out_pos=out_pos if m == 3 else None, sorted_kernel=k)
         ms, wall_ms = time_steps(fn, args.steps, args.warmup, stream)
         # kernel-level timing of the same call, HIP events on the launch stream
         lib.hm_profile_reset()
@@ -552,7 +700,7 @@ def main():
         hist_ms, scat_ms, scan_ms, unsort_ms = (per_step(x) for x in ("psa_hist", "psa_scatter", "scan", "unsort"))
         lib.hm_profile_enable(0)
         lib.hm_profile_reset()
-        key = f"{order}_psa{psa}" + ("_psa_order" if mode == 3 else "")
+        key = vkey(order, psa, mode, skern)
         results[key] = {"ms_per_step": round(ms, 4), "Mq_per_s": round(n_q / ms / 1e3, 1),
                         "search_kernel_ms": round(search_ms, 4) if search_ms else None,
                         "search_kernel_Mq_per_s": round(n_q / search_ms / 1e3, 1) if search_ms else None,
@@ -589,18 +737,23 @@ def main():
                 "kernel": "search_shared_kernel<64,OUT_DIRECT> (PSA-sorted queries, 2^24 per launch)",
                 "bytes_per_query": BYTES_PER_QUERY}
     e2e_achieved = BYTES_PER_QUERY_PSA * n_q / (head["ms_per_step"] * 1e-3) / 1e9
+    # the whole step (sort + search + un-sort, results in input order) against the 192-B/query model
+    roofline_e2e = {"bound": "hbm", "achieved": round(e2e_achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                    "frac": round(e2e_achieved / PEAK_HBM_GBS, 4), "bytes_per_query": BYTES_PER_QUERY_PSA,
+                    "what": "whole hm_search step: PSA sort + search + un-sort to input order"}
 
     abl = None
     if not args.no_ablation:
         log("ablation ...")
         abl = ablation(tree, dq["unsorted"], out_v, out_f, stream, max(3, args.steps // 2), 2)
 
+    tree.close()
     cpu = None
     if not args.no_cpu:
         log("cpu baseline ...")
-        cpu = cpu_baseline(tree, q_unsorted)
+        cpu = cpu_baseline(args)
+        cpu["cpu_config"].update(gpu_cpu_config(dev, stream))
         log("cpu", json.dumps(cpu))
-    tree.close()
     del dq, out_v, out_f
     torch.cuda.empty_cache()
 
@@ -635,7 +788,7 @@ def main():
                    "queries": n_q,
                    "fanout": 64, "leaf_keys": 8, "order": "unsorted", "psa_bits": 16, "hit_frac": 0.9},
         "roofline": roofline,
-        "e2e_roofline_frac_psa192B": round(e2e_achieved / PEAK_HBM_GBS, 4),
+        "roofline_e2e": roofline_e2e,
         "cpu_baseline": cpu,
         "variants": results,
         "build_ms": round(build_ms, 1),

@@ -23,6 +23,10 @@ HM_OUT_PSA_ORDER = 3
 HM_BASELINE_POINTER_CHILD = 1
 HM_BASELINE_POINTER_LINEAR = 2
 HM_BASELINE_PREFIX_PLAIN = 3
+HM_BATCH_DEVICE = 1
+HM_SORTED_AUTO = 0
+HM_SORTED_QUERY = 1
+HM_SORTED_WINDOW = 2
 
 u8p = ctypes.POINTER(ctypes.c_uint8)
 u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -61,6 +65,7 @@ class HmStats(ctypes.Structure):
         ("bytes_device", ctypes.c_uint64),
         ("epoch", ctypes.c_uint64),
         ("relayouts", ctypes.c_uint64),
+        ("respreads", ctypes.c_uint64),
     ]
 
 
@@ -73,6 +78,8 @@ class HmSearchOpts(ctypes.Structure):
         ("output_mode", ctypes.c_uint32),
         ("queries_per_group", ctypes.c_uint32),
         ("out_pos", vp),
+        ("sorted_kernel", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 
@@ -156,6 +163,7 @@ SIGNATURES = [
     ("hm_epoch_publish", ctypes.c_int, [vp, vp, ctypes.POINTER(ctypes.c_int)]),
     ("hm_batcher_create", ctypes.c_int, [vp, ctypes.c_uint64, ctypes.POINTER(HmSearchOpts), vp, ctypes.POINTER(vp)]),
     ("hm_batcher_add", ctypes.c_int, [vp, vp, ctypes.c_uint64, u64p]),
+    ("hm_batcher_add_many", ctypes.c_int, [vp, vp, vp, ctypes.c_uint64, ctypes.c_uint32, vp, u64p]),
     ("hm_batcher_flush", ctypes.c_int, [vp]),
     ("hm_batcher_results", ctypes.c_int, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp)]),
     ("hm_batcher_copy_out", ctypes.c_int, [vp, ctypes.c_uint64, ctypes.c_uint64, vp, vp]),

@@ -202,6 +202,11 @@ int resolve_search(const hm_tree* t, uint64_t n, const hm_search_opts* o, Search
     return set_err(HM_EINVAL, "output_mode %d not in 0..3", mode);
   if (mode == HM_OUT_PSA_ORDER && (!psa || !o || !o->out_pos))
     return set_err(HM_EINVAL, "PSA-order output needs PSA on and out_pos");
+  const uint32_t sk = o ? o->sorted_kernel : (uint32_t)HM_SORTED_AUTO;
+  if (sk > HM_SORTED_WINDOW) return set_err(HM_EINVAL, "sorted_kernel %u not in 0..2", sk);
+  // leaf window: only when asked for (measured slower than one descent per query on the 2^26-key tree)
+  sl->window = psa && mode != HM_OUT_SCATTER && sk == HM_SORTED_WINDOW;
+  sl->window_blocks_per_cu = (o && o->blocks_per_cu && sk == HM_SORTED_WINDOW) ? (int)o->blocks_per_cu : 2;
   *psa_bits = psa ? bits : 0;
   *out_mode = psa ? mode : 0;
   return HM_OK;
@@ -342,6 +347,7 @@ int build_inner_into(hm_tree* t, const Leaf* leaves, uint64_t n_leaves, uint64_t
     m_child = m;
   }
   HM_TRY(exclusive_scan_u32(nchild, prefix, lv.n_inner, 1u, scan_tmp, s));
+  HM_TRY(fold_prefix(inner, prefix, lv.n_inner, t->F, s));
   return HM_OK;
 }
 
@@ -810,13 +816,14 @@ struct UpdBufs {
   uint32_t *keep, *keep_scan;
   void* stmp;
   uint32_t *leaf, *slot;
+  uint32_t* nlist;  // nodes to re-spread: (node, first batch entry) pairs
   uint8_t* found;
-  uint32_t* summary;  // [0] overflow, [1] new keys, [2] unique count, [3] validation flags
+  uint32_t* summary;  // [0] overflow, [1] new keys, [2] unique count, [3] validation flags, [4] nodes listed
 };
 
 static size_t upd_need(size_t n) {
   const size_t k8 = align256(n * 8), k4 = align256((n + 1) * 4);
-  return 6 * k8 + align256(radix_tmp_bytes(n)) + 2 * k4 + align256(scan_tmp_bytes(n + 1)) + 2 * k4 + align256(n) +
+  return 6 * k8 + align256(radix_tmp_bytes(n)) + 2 * k4 + align256(scan_tmp_bytes(n + 1)) + 4 * k4 + align256(n) +
          256;
 }
 
@@ -837,6 +844,7 @@ static UpdBufs carve_upd(void* base, size_t n) {
   b.stmp = cur, cur += align256(scan_tmp_bytes(n + 1));
   b.leaf = (uint32_t*)cur, cur += k4;
   b.slot = (uint32_t*)cur, cur += k4;
+  b.nlist = (uint32_t*)cur, cur += 2 * k4;
   b.found = (uint8_t*)cur;
   return b;
 }
@@ -986,7 +994,7 @@ int publish_prepare(hm_tree* t, hipStream_t s, PublishPlan* plan) {
   if (t->stage_valid) HM_TRY(hipStreamWaitEvent(s, t->stage_ev, 0));       // staging on other streams
   const UpdBufs b = carve_upd(t->upd_buf, n);
   uint32_t* dm = b.summary + 2;
-  HM_TRY(hipMemsetAsync(b.summary, 0, 16, s));
+  HM_TRY(hipMemsetAsync(b.summary, 0, 32, s));
   // sort by key (stable: a later write of a key follows the earlier one), keep the last write
   if (n <= kSmallUpdate) {
     HM_TRY(small_sort_dedupe(t->pend_k, t->pend_v, (uint32_t)n, b.uk, b.uv, dm, s));
@@ -1009,13 +1017,19 @@ int publish_prepare(hm_tree* t, hipStream_t s, PublishPlan* plan) {
   }
   const TreeView tv = t->view();
   HM_TRY(launch_locate(tv, b.uk, n, b.leaf, b.slot, b.found, s, dm));
-  HM_TRY(plan_runs(t->d_leaves, b.leaf, b.found, n, dm, b.summary, s));
-  HM_TRY(copy_u32s(t->upd_host, b.summary, 4, s));
+  HM_TRY(plan_runs(t->d_leaves, (uint32_t)t->n_leaves, t->F, b.leaf, b.found, n, dm, b.summary, b.nlist, s));
+  HM_TRY(copy_u32s(t->upd_host, b.summary, 5, s));
   HM_TRY(hipStreamSynchronize(s));
   plan->n_new = t->upd_host[1];
   plan->m = t->upd_host[2];
-  plan->relayout = (t->upd_host[0] & 1u) != 0;
-  if (!plan->relayout) return HM_OK;
+  // an overflowing leaf whose last-level node has room: that node is re-spread in place at commit
+  // (bounded cost); only when some node is full do all leaves get re-laid out
+  plan->relayout = (t->upd_host[0] & 2u) != 0;
+  if (!plan->relayout) {
+    plan->respread = (t->upd_host[0] & 1u) ? t->upd_host[4] : 0u;
+    if (plan->respread) HM_TRY(mark_nodes(b.nlist, plan->respread, t->F, b.leaf, b.found, dm, s));
+    return HM_OK;
+  }
 
   // ---- re-layout into the standby buffers ----
   ProfScope prof("relayout", s);
@@ -1203,6 +1217,12 @@ int publish_commit(hm_tree* t, hipStream_t s, const PublishPlan& plan, int* rebu
     int rc = wait_for_readers(t, s);  // leaves change in place
     if (rc) return rc;
     HM_TRY(apply_runs(t->d_leaves, b.leaf, b.slot, b.found, b.uk, b.uv, plan.n, b.summary + 2, s));
+    if (plan.respread) {
+      uint64_t* last = t->d_inner + (size_t)t->level_start[t->height - 1] * t->NS;
+      HM_TRY(node_respread(t->d_leaves, last, (uint32_t)t->n_leaves, t->F, b.nlist, plan.respread, b.leaf, b.slot,
+                           b.found, b.uk, b.uv, b.summary + 2, s));
+      t->respreads += plan.respread;
+    }
   } else if (plan.relayout) {
     std::swap(t->d_leaves, t->sb_leaves);
     std::swap(t->leaf_cap, t->sb_leaf_cap);
@@ -1438,6 +1458,7 @@ int hm_tree_stats(const hm_tree* t, hm_stats* st) {
   st->bytes_device = t->device_bytes();
   st->epoch = t->epoch;
   st->relayouts = t->relayouts;
+  st->respreads = t->respreads;
   return HM_OK;
 }
 
@@ -1447,7 +1468,11 @@ int hm_tree_export(const hm_tree* t, uint64_t* inner, uint32_t* prefix, uint64_t
   std::lock_guard<std::mutex> g(mt->mu);
   HM_TRY(hipStreamSynchronize(t->own));
   HM_TRY(hipDeviceSynchronize());
-  if (inner) HM_TRY(hipMemcpy(inner, t->d_inner, (size_t)t->n_inner * t->NS * 8, hipMemcpyDeviceToHost));
+  if (inner) {
+    HM_TRY(hipMemcpy(inner, t->d_inner, (size_t)t->n_inner * t->NS * 8, hipMemcpyDeviceToHost));
+    const int is = idx_slots(t->F);  // the device keeps prefix[node] in the last index slot
+    for (uint32_t j = 0; j < t->n_inner; ++j) inner[(size_t)j * t->NS + is - 1] = KEY_MAX;
+  }
   if (prefix) HM_TRY(hipMemcpy(prefix, t->d_prefix, (size_t)t->n_inner * 4, hipMemcpyDeviceToHost));
   if (leaves) HM_TRY(hipMemcpy(leaves, t->d_leaves, t->n_leaves * sizeof(Leaf), hipMemcpyDeviceToHost));
   return HM_OK;

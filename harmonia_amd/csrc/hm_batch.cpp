@@ -99,7 +99,7 @@ void free_batcher(hm_batcher* b) {
 // Capture the pipeline for `psa` (true: tree's PSA path at full capacity) into g.
 int capture(hm_batcher* b, bool psa, Graph& g) {
   hm_tree* t = b->t;
-  hm_search_opts o = b->have_opts ? b->opts : hm_search_opts{0, -1, -1, 0, 0, 0, nullptr};
+  hm_search_opts o = b->have_opts ? b->opts : hm_search_opts{0, -1, -1, 0, 0, 0, nullptr, 0, 0};
   if (!psa) o.psa_bits = 0;
   // force the PSA decision for the captured capacity regardless of the threshold
   if (psa && o.psa_bits <= 0) o.psa_bits = (int32_t)(t->p.psa_bits ? t->p.psa_bits : 16);
@@ -198,6 +198,30 @@ int hm_batcher_add(hm_batcher* b, const uint64_t* q, uint64_t n, uint64_t* out_o
   }
   if (out_offset) *out_offset = b->fill;
   b->fill += n;
+  return HM_OK;
+}
+
+int hm_batcher_add_many(hm_batcher* b, const uint64_t* const* qs, const uint64_t* ns, uint64_t count, uint32_t flags,
+                        uint64_t* out_offsets, uint64_t* n_added) {
+  if (!b || !n_added) return set_err(HM_EINVAL, "NULL argument");
+  *n_added = 0;
+  if (count && (!qs || !ns)) return set_err(HM_EINVAL, "NULL batch arrays");
+  if (flags & ~HM_BATCH_DEVICE) return set_err(HM_EINVAL, "unknown flags 0x%x", flags);
+  for (uint64_t i = 0; i < count; ++i) {
+    const uint64_t n = ns[i];
+    if (b->fill + n > b->cap) break;
+    if (n && !qs[i]) return set_err(HM_EINVAL, "NULL buffer for batch %llu", (unsigned long long)i);
+    if ((flags & HM_BATCH_DEVICE) && n) {
+      for (uint64_t o = 0; o < n; o += kGatherPiece)
+        b->items.push_back(GatherItem{qs[i] + o, b->fill + o, std::min<uint64_t>(kGatherPiece, n - o)});
+      if (out_offsets) out_offsets[i] = b->fill;
+      b->fill += n;
+    } else {
+      int rc = hm_batcher_add(b, qs[i], n, out_offsets ? out_offsets + i : nullptr);
+      if (rc) return rc;
+    }
+    *n_added = i + 1;
+  }
   return HM_OK;
 }
 

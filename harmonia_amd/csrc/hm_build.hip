@@ -138,6 +138,16 @@ __global__ void build_level_kernel(const uint64_t* __restrict__ child_min, size_
   }
 }
 
+// prefix[node] into the node's last index slot (canonically KEY_MAX), one thread per node.
+template <int F>
+__global__ void fold_prefix_kernel(uint64_t* __restrict__ inner, const uint32_t* __restrict__ prefix, size_t n) {
+  constexpr int IS = idx_slots(F);
+  constexpr int NS = node_slots(F);
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride)
+    inner[j * NS + IS - 1] = (uint64_t)prefix[j];
+}
+
 int grid_for(size_t work, int threads) {
   size_t g = (work + threads - 1) / threads;
   if (g < 1) g = 1;
@@ -198,6 +208,19 @@ hipError_t build_level(const uint64_t* child_min, size_t m_child, int F, uint64_
       break;
     default:
       return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t fold_prefix(uint64_t* inner, const uint32_t* prefix, size_t n_inner, int F, hipStream_t s) {
+  if (n_inner == 0) return hipSuccess;
+  ProfScope prof("build", s);
+  const int g = grid_for(n_inner, 256);
+  switch (F) {
+    case 32: hipLaunchKernelGGL(fold_prefix_kernel<32>, dim3(g), dim3(256), 0, s, inner, prefix, n_inner); break;
+    case 64: hipLaunchKernelGGL(fold_prefix_kernel<64>, dim3(g), dim3(256), 0, s, inner, prefix, n_inner); break;
+    case 128: hipLaunchKernelGGL(fold_prefix_kernel<128>, dim3(g), dim3(256), 0, s, inner, prefix, n_inner); break;
+    default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }

@@ -72,97 +72,118 @@ __device__ __forceinline__ bool group_any_eq(const uint64_t (&k)[W], uint64_t t,
   return e;
 }
 
+// The low 32 bits of v from the last lane of each G-lane group, in every lane of the group
+// (ds_swizzle in bitmask mode: lane i reads lane (i & and_mask) | or_mask of its 32-lane half).
+template <int G>
+__device__ __forceinline__ uint32_t group_last_u32(uint32_t v) {
+  if constexpr (G == 1) {
+    return v;
+  } else {
+    static_assert(G <= 32 && (G & (G - 1)) == 0, "groups of 2..32 lanes");
+    constexpr int kPattern = (0x1F & ~(G - 1)) | ((G - 1) << 5);
+    return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, kPattern);
+  }
+}
+
+// Index step of a node (Harmonia's inner_index, H/src/bpt.h:94-120): the number of index slots
+// <= t, i.e. which 8-key group holds the child slot.  The last index slot holds the node's prefix
+// entry (the BFS index of its first child): it is returned in `base` and counts as KEY_MAX.
+// F <= 64 has one index granule, F = 128 two.
+template <int F, int G>
+__device__ __forceinline__ int index_step(const uint64_t* nd, uint64_t t, int gl, int gbase, uint32_t& base) {
+  constexpr int W = 8 / G;
+  constexpr int NG = F / 8 > 8 ? 2 : 1;
+  uint64_t k[NG][W];
+#pragma unroll
+  for (int g = 0; g < NG; ++g) load_keys<W>(nd + 8 * g + gl * W, k[g]);
+  base = group_last_u32<G>((uint32_t)k[NG - 1][W - 1]);
+  if (gl == G - 1) k[NG - 1][W - 1] = KEY_MAX;
+  int j = 0;
+#pragma unroll
+  for (int g = 0; g < NG; ++g) j += group_count_le<G, W>(k[g], t, gbase);
+  return j;
+}
+
 // upper_bound rank of t inside one inner node: the child slot (H/regular_bpt/cfb.cpp:15-19,
 // H/regular_bpt/regular_bpt.cpp:361-365).  Two granule reads: the index granule picks the group
-// of 8 keys (Harmonia's inner_index step, H/src/bpt.h:94-120), the group gives the slot.
+// of 8 keys, the group gives the slot.
 template <int F, int G>
 __device__ __forceinline__ uint32_t node_rank(const uint64_t* nd, uint64_t t, int gl, int gbase) {
   constexpr int W = 8 / G;
   constexpr int IS = idx_slots(F);
+  uint32_t base;
+  const int j = index_step<F, G>(nd, t, gl, gbase, base);
   uint64_t k[W];
-  int j;
-  if constexpr (F / 8 <= 8) {
-    load_keys<W>(nd + gl * W, k);
-    j = group_count_le<G, W>(k, t, gbase);
-  } else {
-    uint64_t k2[W];
-    load_keys<W>(nd + gl * W, k);
-    load_keys<W>(nd + 8 + gl * W, k2);
-    j = group_count_le<G, W>(k, t, gbase) + group_count_le<G, W>(k2, t, gbase);
-  }
   load_keys<W>(nd + IS + 8 * j + gl * W, k);
   return (uint32_t)(8 * j + group_count_le<G, W>(k, t, gbase));
 }
 
-// Descend from the root to a leaf id.  Levels [0, lds_levels) are read from LDS (nodes at
-// `snodes`, prefix entries at `spre`), deeper levels from global memory.
+// The next node of the descent: child = prefix[node] + slot, the prefix entry read with the index
+// granule.
+template <int F, int G>
+__device__ __forceinline__ uint32_t node_next(const uint64_t* nd, uint64_t t, int gl, int gbase) {
+  constexpr int W = 8 / G;
+  constexpr int IS = idx_slots(F);
+  uint32_t base;
+  const int j = index_step<F, G>(nd, t, gl, gbase, base);
+  uint64_t k[W];
+  load_keys<W>(nd + IS + 8 * j + gl * W, k);
+  return base + (uint32_t)(8 * j + group_count_le<G, W>(k, t, gbase));
+}
+
+// Descend from the root to a leaf id.  Levels [0, lds_levels) are read from the copy of the top
+// nodes at `snodes` (LDS), deeper levels from global memory.
 // t must be < KEY_MAX (callers clamp), so every node holds a separator > t.
 template <int F, int G>
-__device__ __forceinline__ uint32_t descend(const TreeView& tv, int lds_levels, const uint64_t* snodes,
-                                            const uint32_t* spre, uint64_t t, int gl, int gbase) {
+__device__ __forceinline__ uint32_t descend(const TreeView& tv, int lds_levels, const uint64_t* snodes, uint64_t t,
+                                            int gl, int gbase) {
   constexpr int NS = node_slots(F);
   uint32_t node = 0;
   int l = 0;
-  for (; l < lds_levels; ++l) node = spre[node] + node_rank<F, G>(snodes + (size_t)node * NS, t, gl, gbase);
-  for (; l < tv.height; ++l) {
-    const uint32_t base = tv.prefix[node];
-    node = base + node_rank<F, G>(tv.inner + (size_t)node * NS, t, gl, gbase);
-  }
+  for (; l < lds_levels; ++l) node = node_next<F, G>(snodes + (size_t)node * NS, t, gl, gbase);
+  for (; l < tv.height; ++l) node = node_next<F, G>(tv.inner + (size_t)node * NS, t, gl, gbase);
   return node - tv.n_inner;
 }
 
-// Q independent descents interleaved per group, so each lane keeps Q loads in flight per level
-// (memory-level parallelism: random L2/MALL/HBM accesses are latency bound).
+// One level for Q independent queries of a group, every load of a step issued before the first is
+// used (memory-level parallelism: random L2/MALL/HBM accesses are latency bound).
 template <int F, int G, int Q>
-__device__ __forceinline__ void descend_q(const TreeView& tv, int lds_levels, const uint64_t* snodes,
-                                          const uint32_t* spre, const uint64_t (&t)[Q], uint32_t (&node)[Q], int gl,
-                                          int gbase) {
+__device__ __forceinline__ void level_q(const uint64_t* nodes, const uint64_t (&t)[Q], uint32_t (&node)[Q], int gl,
+                                        int gbase) {
   constexpr int W = 8 / G;
   constexpr int IS = idx_slots(F);
   constexpr int NS = node_slots(F);
+  constexpr int NG = F / 8 > 8 ? 2 : 1;
+  uint64_t k[Q][NG][W];
+  uint32_t base[Q];
+  int j[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q)
+#pragma unroll
+    for (int g = 0; g < NG; ++g) load_keys<W>(nodes + (size_t)node[q] * NS + 8 * g + gl * W, k[q][g]);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    base[q] = group_last_u32<G>((uint32_t)k[q][NG - 1][W - 1]);
+    if (gl == G - 1) k[q][NG - 1][W - 1] = KEY_MAX;
+    j[q] = 0;
+#pragma unroll
+    for (int g = 0; g < NG; ++g) j[q] += group_count_le<G, W>(k[q][g], t[q], gbase);
+  }
+#pragma unroll
+  for (int q = 0; q < Q; ++q) load_keys<W>(nodes + (size_t)node[q] * NS + IS + 8 * j[q] + gl * W, k[q][0]);
+#pragma unroll
+  for (int q = 0; q < Q; ++q) node[q] = base[q] + 8 * j[q] + group_count_le<G, W>(k[q][0], t[q], gbase);
+}
+
+// Q independent descents interleaved per group.
+template <int F, int G, int Q>
+__device__ __forceinline__ void descend_q(const TreeView& tv, int lds_levels, const uint64_t* snodes,
+                                          const uint64_t (&t)[Q], uint32_t (&node)[Q], int gl, int gbase) {
 #pragma unroll
   for (int q = 0; q < Q; ++q) node[q] = 0;
   int l = 0;
-  for (; l < lds_levels; ++l) {
-    uint64_t k[Q][W];
-    int j[Q];
-#pragma unroll
-    for (int q = 0; q < Q; ++q) load_keys<W>(snodes + (size_t)node[q] * NS + gl * W, k[q]);
-#pragma unroll
-    for (int q = 0; q < Q; ++q) j[q] = group_count_le<G, W>(k[q], t[q], gbase);
-    if constexpr (F / 8 > 8) {
-#pragma unroll
-      for (int q = 0; q < Q; ++q) load_keys<W>(snodes + (size_t)node[q] * NS + 8 + gl * W, k[q]);
-#pragma unroll
-      for (int q = 0; q < Q; ++q) j[q] += group_count_le<G, W>(k[q], t[q], gbase);
-    }
-#pragma unroll
-    for (int q = 0; q < Q; ++q) load_keys<W>(snodes + (size_t)node[q] * NS + IS + 8 * j[q] + gl * W, k[q]);
-#pragma unroll
-    for (int q = 0; q < Q; ++q) node[q] = spre[node[q]] + 8 * j[q] + group_count_le<G, W>(k[q], t[q], gbase);
-  }
-  for (; l < tv.height; ++l) {
-    uint64_t k[Q][W];
-    uint64_t k2[Q][W];
-    uint32_t base[Q];
-    int j[Q];
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      const uint64_t* nd = tv.inner + (size_t)node[q] * NS;
-      base[q] = tv.prefix[node[q]];
-      load_keys<W>(nd + gl * W, k[q]);
-      if constexpr (F / 8 > 8) load_keys<W>(nd + 8 + gl * W, k2[q]);
-    }
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      j[q] = group_count_le<G, W>(k[q], t[q], gbase);
-      if constexpr (F / 8 > 8) j[q] += group_count_le<G, W>(k2[q], t[q], gbase);
-    }
-#pragma unroll
-    for (int q = 0; q < Q; ++q) load_keys<W>(tv.inner + (size_t)node[q] * NS + IS + 8 * j[q] + gl * W, k[q]);
-#pragma unroll
-    for (int q = 0; q < Q; ++q) node[q] = base[q] + 8 * j[q] + group_count_le<G, W>(k[q], t[q], gbase);
-  }
+  for (; l < lds_levels; ++l) level_q<F, G, Q>(snodes, t, node, gl, gbase);
+  for (; l < tv.height; ++l) level_q<F, G, Q>(tv.inner, t, node, gl, gbase);
 #pragma unroll
   for (int q = 0; q < Q; ++q) node[q] -= tv.n_inner;
 }
@@ -170,20 +191,19 @@ __device__ __forceinline__ void descend_q(const TreeView& tv, int lds_levels, co
 
 // One global level for PSA-sorted queries with G lanes per query.  At the upper levels the 64/G
 // queries of a wave's query slot almost always share their node, so the wave loads the node of its
-// first lane once (one coalesced 16-B piece per lane, the prefix entry as a uniform load) into the
-// slot's LDS buffer and the groups in that node rank it from there; a group in another node reads
-// its own directly.  Per query slot that is one vector load per lane instead of the index and key
-// granules of every group.  (At the last inner level the slot straddles two or three nodes: staging
-// two of them there was measured and lost to the divergent ranks, so callers keep it direct.)
+// first lane once (one coalesced 16-B piece per lane, prefix entry included) into the slot's LDS
+// buffer and the groups in that node rank it from there; a group in another node reads its own
+// directly.  Per query slot that is one vector load per lane instead of the index and key granules
+// of every group.  (At the last inner level the slot straddles two or three nodes: staging two of
+// them there was measured and lost to the divergent ranks, so callers keep it direct.)
 template <int F, int G, int Q>
-__device__ __forceinline__ void shared_node_level(const uint64_t* __restrict__ inner,
-                                                  const uint32_t* __restrict__ prefix, uint64_t* wslot,
+__device__ __forceinline__ void shared_node_level(const uint64_t* __restrict__ inner, uint64_t* wslot,
                                                   const uint64_t (&t)[Q], uint32_t (&node)[Q], int gl, int gbase,
                                                   int lane) {
   constexpr int NS = node_slots(F);
   constexpr int CH = NS / 2;          // 16-byte pieces per node
   constexpr int PL = (CH + 63) / 64;  // pieces per lane
-  uint32_t n0[Q], base0[Q];
+  uint32_t n0[Q];
   ulonglong2 piece[Q][PL];
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
@@ -194,7 +214,6 @@ __device__ __forceinline__ void shared_node_level(const uint64_t* __restrict__ i
       const int idx = lane + 64 * k;
       piece[q][k] = src[idx < CH ? idx : CH - 1];  // clamped, not predicated: keeps piece in registers
     }
-    base0[q] = prefix[n0[q]];
   }
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
@@ -206,9 +225,9 @@ __device__ __forceinline__ void shared_node_level(const uint64_t* __restrict__ i
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
     if (node[q] == n0[q])  // uniform inside a group (its lanes share the query)
-      node[q] = base0[q] + node_rank<F, G>(wslot + q * NS, t[q], gl, gbase);
+      node[q] = node_next<F, G>(wslot + q * NS, t[q], gl, gbase);
     else
-      node[q] = prefix[node[q]] + node_rank<F, G>(inner + (size_t)node[q] * NS, t[q], gl, gbase);
+      node[q] = node_next<F, G>(inner + (size_t)node[q] * NS, t[q], gl, gbase);
   }
   wave_lds_sync();  // the next level overwrites the buffers
 }
@@ -217,38 +236,37 @@ __device__ __forceinline__ void shared_node_level(const uint64_t* __restrict__ i
 // (wave buffers of Q * node_slots(F) words at `wslot`), the last level reads per group.
 template <int F, int G, int Q>
 __device__ __forceinline__ void descend_shared(const TreeView& tv, int lds_levels, const uint64_t* snodes,
-                                               const uint32_t* spre, uint64_t* wslot, const uint64_t (&t)[Q],
-                                               uint32_t (&node)[Q], int gl, int gbase, int lane) {
+                                               uint64_t* wslot, const uint64_t (&t)[Q], uint32_t (&node)[Q], int gl,
+                                               int gbase, int lane) {
   constexpr int NS = node_slots(F);
 #pragma unroll
   for (int q = 0; q < Q; ++q) node[q] = 0;
   int l = 0;
   for (; l < lds_levels; ++l) {
 #pragma unroll
-    for (int q = 0; q < Q; ++q) node[q] = spre[node[q]] + node_rank<F, G>(snodes + (size_t)node[q] * NS, t[q], gl, gbase);
+    for (int q = 0; q < Q; ++q) node[q] = node_next<F, G>(snodes + (size_t)node[q] * NS, t[q], gl, gbase);
   }
-  for (; l < tv.height - 1; ++l) shared_node_level<F, G, Q>(tv.inner, tv.prefix, wslot, t, node, gl, gbase, lane);
+  for (; l < tv.height - 1; ++l) shared_node_level<F, G, Q>(tv.inner, wslot, t, node, gl, gbase, lane);
   for (; l < tv.height; ++l) {
 #pragma unroll
-    for (int q = 0; q < Q; ++q)
-      node[q] = tv.prefix[node[q]] + node_rank<F, G>(tv.inner + (size_t)node[q] * NS, t[q], gl, gbase);
+    for (int q = 0; q < Q; ++q) node[q] = node_next<F, G>(tv.inner + (size_t)node[q] * NS, t[q], gl, gbase);
   }
 #pragma unroll
   for (int q = 0; q < Q; ++q) node[q] -= tv.n_inner;
 }
 
-// upper_bound rank of t in a node held in LDS (one lane per query).
+// The next node for a node held in LDS (one lane per query).
 template <int F>
-__device__ __forceinline__ uint32_t node_rank_lds(const uint64_t* nd, uint64_t t) {
+__device__ __forceinline__ uint32_t node_next_lds(const uint64_t* nd, uint64_t t) {
   constexpr int IS = idx_slots(F);
   int j = 0;
 #pragma unroll
-  for (int i = 0; i < IS; ++i) j += nd[i] <= t ? 1 : 0;
+  for (int i = 0; i < IS - 1; ++i) j += nd[i] <= t ? 1 : 0;
   const uint64_t* g = nd + IS + 8 * j;
   int c = 0;
 #pragma unroll
   for (int i = 0; i < 8; ++i) c += g[i] <= t ? 1 : 0;
-  return (uint32_t)(8 * j + c);
+  return (uint32_t)nd[IS - 1] + (uint32_t)(8 * j + c);
 }
 
 // One global level of the descent for PSA-sorted queries, one lane per query, Q queries per lane.
@@ -257,8 +275,8 @@ __device__ __forceinline__ uint32_t node_rank_lds(const uint64_t* nd, uint64_t t
 // LDS slot and the lanes that need it rank from LDS.  Up to K distinct nodes per query slot are
 // served that way; lanes left over (unsorted input) read their node directly.
 template <int F, int Q, int K>
-__device__ __forceinline__ void sorted_level(const uint64_t* __restrict__ inner, const uint32_t* __restrict__ prefix,
-                                             uint64_t* wslot, const uint64_t (&t)[Q], uint32_t (&node)[Q], int lane) {
+__device__ __forceinline__ void sorted_level(const uint64_t* __restrict__ inner, uint64_t* wslot,
+                                             const uint64_t (&t)[Q], uint32_t (&node)[Q], int lane) {
   constexpr int NS = node_slots(F);
   constexpr int CH = NS / 2;          // 16-byte pieces per node
   constexpr int PL = (CH + 63) / 64;  // pieces per lane
@@ -280,7 +298,6 @@ __device__ __forceinline__ void sorted_level(const uint64_t* __restrict__ inner,
     // every slot is loaded (a finished slot reloads node n0 = 0, harmless) so all loads of the
     // step are issued before the first is waited for and stay in registers
     ulonglong2 piece[Q][PL];
-    uint32_t base[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const ulonglong2* src = reinterpret_cast<const ulonglong2*>(inner + (size_t)n0[q] * NS);
@@ -289,7 +306,6 @@ __device__ __forceinline__ void sorted_level(const uint64_t* __restrict__ inner,
         const int idx = lane + 64 * k;
         piece[q][k] = src[idx < CH ? idx : CH - 1];
       }
-      base[q] = prefix[n0[q]];
     }
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
@@ -301,7 +317,7 @@ __device__ __forceinline__ void sorted_level(const uint64_t* __restrict__ inner,
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       if (((todo >> q) & 1u) && node[q] == n0[q]) {
-        res[q] = base[q] + node_rank_lds<F>(wslot + q * NS, t[q]);
+        res[q] = node_next_lds<F>(wslot + q * NS, t[q]);
         todo &= ~(1u << q);
       }
     }
@@ -309,22 +325,19 @@ __device__ __forceinline__ void sorted_level(const uint64_t* __restrict__ inner,
   }
 #pragma unroll
   for (int q = 0; q < Q; ++q)
-    if ((todo >> q) & 1u) res[q] = prefix[node[q]] + node_rank<F, 1>(inner + (size_t)node[q] * NS, t[q], 0, lane);
+    if ((todo >> q) & 1u) res[q] = node_next<F, 1>(inner + (size_t)node[q] * NS, t[q], 0, lane);
 #pragma unroll
   for (int q = 0; q < Q; ++q) node[q] = res[q];
 }
 
-// Stage the top `lds_levels` levels (nodes, then their prefix entries) into dynamic LDS.
+// Stage the top `lds_levels` levels (their nodes, prefix entries included) into dynamic LDS.
 template <int F>
-__device__ __forceinline__ void stage_top_levels(const TreeView& tv, int lds_levels, uint64_t* smem,
-                                                 uint32_t*& spre) {
+__device__ __forceinline__ void stage_top_levels(const TreeView& tv, int lds_levels, uint64_t* smem) {
   constexpr int NS = node_slots(F);
   const uint32_t nl = lds_levels > 0 ? tv.level_start[lds_levels] : 0u;
   const uint32_t words = nl * NS;
   for (uint32_t i = threadIdx.x; i < words / 2; i += blockDim.x)
     reinterpret_cast<ulonglong2*>(smem)[i] = reinterpret_cast<const ulonglong2*>(tv.inner)[i];
-  spre = reinterpret_cast<uint32_t*>(smem + words);
-  for (uint32_t i = threadIdx.x; i < nl; i += blockDim.x) spre[i] = tv.prefix[i];
   __syncthreads();
 }
 

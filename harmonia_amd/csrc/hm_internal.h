@@ -8,6 +8,10 @@
 //             (= 1 + exclusive scan of child counts, H/regular_bpt/traversal.cpp:40,58).
 //             BFS indices >= n_inner name leaves: leaf = index - n_inner.
 //   leaves  : 128-byte records {key[8], val[8]}, keys ascending then KEY_MAX padding.
+// On the device the last index slot, idx[IS-1] (KEY_MAX in the layout above: it would be
+// key[F-1], or an unused slot), holds prefix[node] instead, so a descent step reads the next
+// node's base with the index granule rather than as a separate load; rank computations treat that
+// slot as KEY_MAX, and hm_tree_export restores KEY_MAX.
 // Child choice is upper_bound (number of separators <= target), leaf match is exact equality
 // (H/regular_bpt/prefix-sort.cu:147-173, 202-228; H/regular_bpt/regular_bpt.cpp:346-393).
 #pragma once
@@ -111,6 +115,8 @@ hipError_t leaf_min_keys(const Leaf* leaves, size_t n_leaves, uint64_t* child_mi
 // One inner level: m nodes over m_child children whose minimum keys are child_min.
 hipError_t build_level(const uint64_t* child_min, size_t m_child, int F, uint64_t* nodes, uint64_t* node_min,
                        uint32_t* nchild, size_t m, hipStream_t s);
+// inner[node][IS-1] = prefix[node] for every node (after the prefix scan of a build or rebuild).
+hipError_t fold_prefix(uint64_t* inner, const uint32_t* prefix, size_t n_inner, int F, hipStream_t s);
 
 // ---- search (hm_search.hip) ------------------------------------------------------------
 struct SearchLaunch {
@@ -119,7 +125,11 @@ struct SearchLaunch {
   int blocks_per_cu = 8;
   int queries_per_group = 2;  // independent descents interleaved per group
   bool sorted = false;        // queries are PSA-sorted: waves share node reads (group 1, or group 2 x 4)
+  bool window = false;        // with sorted, OUT_DIRECT: the leaf-window kernel (dense batches)
+  int window_blocks_per_cu = 2;
 };
+// Leaves a wave's LDS ring holds in the leaf-window search (64 B of keys each).
+constexpr int kWindowLeaves = 256;
 // Where a search writes result i of its (possibly PSA-sorted) query array:
 enum SearchOut {
   OUT_DIRECT = 0,   // out_vals[i], out_found[i]
@@ -176,10 +186,22 @@ hipError_t merge_sort_pairs(const uint64_t* pk, const uint64_t* pv, uint32_t n, 
                             uint32_t* i0, uint32_t* i1, uint64_t* vout, hipStream_t s, const uint64_t** sorted_keys);
 // *dm = keep_scan[n - 1] + keep[n - 1] (unique count after compact_pairs).
 hipError_t unique_count(const uint32_t* keep, const uint32_t* keep_scan, size_t n, uint32_t* dm, hipStream_t s);
-// Over the m located unique keys (sorted, so each leaf's keys form one run):
-// summary[0] |= 1 if some leaf would exceed 8 keys, summary[1] += number of new keys.  n sizes the grid.
-hipError_t plan_runs(const Leaf* leaves, const uint32_t* leaf, const uint8_t* found, size_t n, const uint32_t* dm,
-                     uint32_t* summary, hipStream_t s);
+// Over the m located unique keys (sorted, so each leaf's keys form one run and each last-level
+// node's keys one range): summary[1] += number of new keys; summary[0] |= 1 if some leaf would exceed
+// 8 keys.  Such a node goes on the re-spread list (nlist: pairs of node index within the last level and
+// first batch entry; summary[4] = their count) when its leaves can hold all its keys, else
+// summary[0] |= 2 (a full re-layout is needed).  n sizes the grid; nlist holds 2 n entries.
+hipError_t plan_runs(const Leaf* leaves, uint32_t n_leaves, int F, const uint32_t* leaf, const uint8_t* found, size_t n,
+                     const uint32_t* dm, uint32_t* summary, uint32_t* nlist, hipStream_t s);
+// found[j] |= 2 for the batch entries of the listed nodes (the in-place merge then skips them).
+hipError_t mark_nodes(const uint32_t* nlist, uint32_t n_nodes, int F, const uint32_t* leaf, uint8_t* found,
+                      const uint32_t* dm, hipStream_t s);
+// Re-spread each listed node in place: its keys, the batch's values and new keys merged, spread evenly
+// over its own leaves, separators and index slots rewritten.  last_level = the first node of the last
+// inner level.
+hipError_t node_respread(Leaf* leaves, uint64_t* last_level, uint32_t n_leaves, int F, const uint32_t* nlist,
+                         uint32_t n_nodes, const uint32_t* leaf, const uint32_t* slot, const uint8_t* found,
+                         const uint64_t* uk, const uint64_t* uv, const uint32_t* dm, hipStream_t s);
 // No leaf overflows: per run, overwrite the values of existing keys and merge the new keys in place.
 hipError_t apply_runs(Leaf* leaves, const uint32_t* leaf, const uint32_t* slot, const uint8_t* found,
                       const uint64_t* uk, const uint64_t* uv, size_t n, const uint32_t* dm, hipStream_t s);

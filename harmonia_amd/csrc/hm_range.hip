@@ -46,7 +46,7 @@ __global__ __launch_bounds__(kThreads) void range_kernel(TreeView tv, const uint
     const uint32_t L = valid ? (len ? len[ri] : cap_const) : 0u;
     const uint64_t off = valid ? (offsets ? offsets[ri] : (uint64_t)ri * offset_stride) : 0ull;
     const uint64_t sd = s == KEY_MAX ? KEY_MAX - 1 : s;
-    uint32_t leaf = descend<F, G>(tv, 0, nullptr, nullptr, sd, gl, gbase);
+    uint32_t leaf = descend<F, G>(tv, 0, nullptr, sd, gl, gbase);
     uint32_t got = 0;
     bool done = !valid || L == 0 || s > h || s == KEY_MAX;
     while (!done) {  // uniform inside the group

@@ -14,10 +14,10 @@ namespace {
 
 constexpr int kThreads = 256;
 
+// node_slots(F) is even, so the wave buffers after the staged nodes stay 16-byte aligned.
 size_t sorted_lds_bytes(const TreeView& tv, int levels, int Q) {
   const size_t nl = levels > 0 ? tv.level_start[levels] : 0;
-  const size_t top_words = (nl * node_slots(tv.fanout) + (nl + 1) / 2 + 1) & ~(size_t)1;
-  return (top_words + (size_t)(kThreads / 64) * Q * node_slots(tv.fanout)) * sizeof(uint64_t);
+  return (nl + (size_t)(kThreads / 64) * Q) * node_slots(tv.fanout) * sizeof(uint64_t);
 }
 
 // SHARED: the queries are PSA-sorted and the upper global levels go through shared_node_level
@@ -28,8 +28,7 @@ __device__ __forceinline__ void search_body(const TreeView& tv, int lds_levels, 
                                             uint8_t* __restrict__ out_found, uint32_t n,
                                             const uint32_t* __restrict__ dn) {
   extern __shared__ __align__(16) uint64_t smem[];
-  uint32_t* spre;
-  stage_top_levels<F>(tv, lds_levels, smem, spre);
+  stage_top_levels<F>(tv, lds_levels, smem);
 
   constexpr int W = 8 / G;
   // G == 1: one lane per query reads its 8 leaf keys, then only the matching value (same 128-B
@@ -52,8 +51,7 @@ __device__ __forceinline__ void search_body(const TreeView& tv, int lds_levels, 
   if constexpr (SHARED) {
     constexpr int NS = node_slots(F);
     const uint32_t nl = lds_levels > 0 ? tv.level_start[lds_levels] : 0u;
-    const uint32_t top_words = (nl * NS + (nl + 1) / 2 + 1) & ~1u;  // 16-byte aligned
-    wslot = smem + top_words + (threadIdx.x >> 6) * (Q * NS);
+    wslot = smem + (nl + (threadIdx.x >> 6) * Q) * NS;
   }
 
   for (uint32_t qb = begin; qb < end; qb += tile) {  // uniform across the block
@@ -66,9 +64,9 @@ __device__ __forceinline__ void search_body(const TreeView& tv, int lds_levels, 
       td[i] = t[i] == KEY_MAX ? KEY_MAX - 1 : t[i];
     }
     if constexpr (SHARED)
-      descend_shared<F, G, Q>(tv, lds_levels, smem, spre, wslot, td, leaf, gl, gbase, threadIdx.x & 63);
+      descend_shared<F, G, Q>(tv, lds_levels, smem, wslot, td, leaf, gl, gbase, threadIdx.x & 63);
     else
-      descend_q<F, G, Q>(tv, lds_levels, smem, spre, td, leaf, gl, gbase);
+      descend_q<F, G, Q>(tv, lds_levels, smem, td, leaf, gl, gbase);
     uint64_t k[Q][W], v[Q][W];
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
@@ -136,12 +134,10 @@ __global__ __launch_bounds__(kThreads) void search_sorted_kernel(TreeView tv, in
                                                                  uint8_t* __restrict__ out_found, uint32_t n,
                                                                  const uint32_t* __restrict__ dn) {
   extern __shared__ __align__(16) uint64_t smem[];
-  uint32_t* spre;
-  stage_top_levels<F>(tv, lds_levels, smem, spre);
+  stage_top_levels<F>(tv, lds_levels, smem);
   constexpr int NS = node_slots(F);
   const uint32_t nl = lds_levels > 0 ? tv.level_start[lds_levels] : 0u;
-  const uint32_t top_words = (nl * NS + (nl + 1) / 2 + 1) & ~1u;  // 16-byte aligned
-  uint64_t* wslot = smem + top_words + (threadIdx.x >> 6) * (Q * NS);
+  uint64_t* wslot = smem + (nl + (threadIdx.x >> 6) * Q) * NS;
   const int lane = threadIdx.x & 63;
   constexpr uint32_t tile = kThreads * Q;
   if (dn) n = *dn;
@@ -151,7 +147,6 @@ __global__ __launch_bounds__(kThreads) void search_sorted_kernel(TreeView tv, in
   const uint32_t begin = vb * chunk;
   const uint32_t end = begin + chunk < n ? begin + chunk : n;
   const uint64_t* __restrict__ inner = tv.inner;
-  const uint32_t* __restrict__ prefix = tv.prefix;
 
   for (uint32_t qb = begin; qb < end; qb += tile) {  // uniform across the block
     uint64_t t[Q], td[Q];
@@ -165,9 +160,9 @@ __global__ __launch_bounds__(kThreads) void search_sorted_kernel(TreeView tv, in
     }
     for (int l = 0; l < lds_levels; ++l) {
 #pragma unroll
-      for (int i = 0; i < Q; ++i) node[i] = spre[node[i]] + node_rank_lds<F>(smem + (size_t)node[i] * NS, td[i]);
+      for (int i = 0; i < Q; ++i) node[i] = node_next_lds<F>(smem + (size_t)node[i] * NS, td[i]);
     }
-    for (int l = lds_levels; l < tv.height; ++l) sorted_level<F, Q, 4>(inner, prefix, wslot, td, node, lane);
+    for (int l = lds_levels; l < tv.height; ++l) sorted_level<F, Q, 4>(inner, wslot, td, node, lane);
     uint64_t k[Q][8];
 #pragma unroll
     for (int i = 0; i < Q; ++i) load_keys<8>(tv.leaves[node[i] - tv.n_inner].key, k[i]);
@@ -192,6 +187,187 @@ __global__ __launch_bounds__(kThreads) void search_sorted_kernel(TreeView tv, in
   }
 }
 
+// ---- leaf-window search (dense PSA-sorted batches) -------------------------------------------
+//
+// After the PSA sort, a chunk of consecutive queries lies in a narrow key range, and every query's
+// leaf lies between the leaves of the chunk's smallest and largest key (the leaf of t is
+// nondecreasing in t).  When that range holds at most R leaves, the wave streams the range's leaf
+// keys into a wave-private LDS ring (consecutive 16-B pieces: each leaf read once, whole 64-B key
+// halves), ranks every query against the ring (binary search over the leaf minima, then the leaf's
+// 8 keys), and reads from HBM only the value of a matching key.  Each wave walks one contiguous part
+// of the batch, so consecutive chunks share most of their range and the ring keeps it: per chunk
+// only the leaves past the previous range are loaded.  A chunk whose range exceeds R leaves (a
+// sparse part of the batch) is searched query by query instead.
+//
+// The leaf of t is the last leaf j of the range with min(j) <= t: a leaf's minimum key is its
+// separator in the parent (the bulk build and re-layouts set separators to leaf minima, and an
+// insert only adds keys >= the separator), so this is the descent's choice, and no leaf is empty.
+constexpr int kWinThreads = 256;
+constexpr int kWinWaves = kWinThreads / 64;
+constexpr int kWinQ = 4;                    // queries per lane per chunk
+constexpr uint32_t kWinChunk = 64 * kWinQ;  // queries per chunk
+
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint64_t w = __shfl_xor(v, o);
+    v = w < v ? w : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint64_t w = __shfl_xor(v, o);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Keys of leaves [a, b] (b - a < R) into the ring: leaf x at ring[(x % R) * 8], 4 pieces of 16 B.
+template <int R>
+__device__ __forceinline__ void ring_load(const Leaf* __restrict__ leaves, uint64_t* ring, uint32_t a, uint32_t b,
+                                          int lane) {
+  const uint32_t pieces = (b - a + 1) * 4;
+  constexpr int kBatch = 8;  // loads in flight per lane before their stores
+  for (uint32_t p0 = 0; p0 < pieces; p0 += 64 * kBatch) {  // wave-uniform
+    ulonglong2 v[kBatch];
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k) {
+      const uint32_t p = p0 + lane + 64 * k;
+      const uint32_t x = a + (p < pieces ? p : pieces - 1) / 4;
+      v[k] = reinterpret_cast<const ulonglong2*>(leaves[x].key)[p & 3];
+    }
+#pragma unroll
+    for (int k = 0; k < kBatch; ++k) {
+      const uint32_t p = p0 + lane + 64 * k;
+      if (p < pieces) reinterpret_cast<ulonglong2*>(ring + ((a + p / 4) % R) * 8)[p & 3] = v[k];
+    }
+  }
+}
+
+template <int F, int R>
+__global__ __launch_bounds__(kWinThreads) void search_window_kernel(TreeView tv, const uint64_t* __restrict__ q,
+                                                                    uint64_t* __restrict__ out_vals,
+                                                                    uint8_t* __restrict__ out_found, uint32_t n,
+                                                                    const uint32_t* __restrict__ dn) {
+  static_assert((R & (R - 1)) == 0, "ring size is a power of two");
+  constexpr int kSteps = __builtin_ctz(R);  // binary-search steps over at most R leaves
+  __shared__ __align__(16) uint64_t rings[kWinWaves][R * 8];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint64_t* ring = rings[wv];
+  if (dn) n = *dn;
+  // one contiguous part of the batch per wave, whole chunks
+  const uint32_t nwaves = gridDim.x * kWinWaves;
+  const uint32_t vw = xcd_block(blockIdx.x, gridDim.x) * kWinWaves + wv;
+  uint32_t per = (n + nwaves - 1) / nwaves;
+  per = (per + kWinChunk - 1) / kWinChunk * kWinChunk;
+  const uint64_t b64 = (uint64_t)vw * per;
+  const uint32_t begin = b64 < n ? (uint32_t)b64 : n;
+  const uint32_t end = begin + per < n ? begin + per : n;
+  uint32_t wlo = 1, whi = 0;  // leaves held by the ring (empty)
+  for (uint32_t cb = begin; cb < end; cb += kWinChunk) {  // wave-uniform
+    uint64_t t[kWinQ], td[kWinQ];
+#pragma unroll
+    for (int i = 0; i < kWinQ; ++i) {
+      const uint32_t qi = cb + lane + 64 * i;
+      t[i] = q[qi < end ? qi : end - 1];  // lanes past the end repeat the last query
+      td[i] = t[i] == KEY_MAX ? KEY_MAX - 1 : t[i];
+    }
+    uint64_t mn = td[0], mx = td[0];
+#pragma unroll
+    for (int i = 1; i < kWinQ; ++i) {
+      mn = td[i] < mn ? td[i] : mn;
+      mx = td[i] > mx ? td[i] : mx;
+    }
+    mn = uniform_u64(wave_min_u64(mn));
+    mx = uniform_u64(wave_max_u64(mx));
+    // leaves of the smallest and largest key: 8-lane groups, even groups mn, odd groups mx
+    const int gl = lane & 7, gbase = lane & ~7;
+    const uint32_t lf = descend<F, 8>(tv, 0, nullptr, ((lane >> 3) & 1) ? mx : mn, gl, gbase);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)lf, 0);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)lf, 8);
+    uint32_t leaf[kWinQ];
+    int lt[kWinQ];
+    bool eq[kWinQ];
+    if (hi - lo < (uint32_t)R) {
+      // bring the ring to [lo, hi]: what it lacks below and above the leaves it holds
+      if (wlo > whi || lo > whi || hi < wlo) {
+        ring_load<R>(tv.leaves, ring, lo, hi, lane);
+      } else {
+        if (lo < wlo) ring_load<R>(tv.leaves, ring, lo, wlo - 1, lane);
+        if (hi > whi) ring_load<R>(tv.leaves, ring, whi + 1, hi, lane);
+      }
+      wlo = lo;
+      whi = hi;
+      wave_lds_sync();
+#pragma unroll
+      for (int i = 0; i < kWinQ; ++i) {
+        uint32_t a = lo, b = hi;  // the leaf of td[i] is in [a, b]
+#pragma unroll
+        for (int st = 0; st < kSteps; ++st) {
+          const uint32_t mid = (a + b + 1) >> 1;
+          const bool up = ring[(mid % R) * 8] <= td[i];
+          if (a < b) {
+            if (up)
+              a = mid;
+            else
+              b = mid - 1;
+          }
+        }
+        leaf[i] = a;
+        uint64_t k[8];
+        load_keys<8>(ring + (a % R) * 8, k);
+        lt[i] = group_count_lt<1, 8>(k, t[i], 0);
+        eq[i] = group_any_eq<1, 8>(k, t[i], 0) && t[i] != KEY_MAX;
+      }
+    } else {
+      // sparse chunk: one descent per query
+      descend_q<F, 1, kWinQ>(tv, 0, nullptr, td, leaf, 0, lane);
+#pragma unroll
+      for (int i = 0; i < kWinQ; ++i) {
+        uint64_t k[8];
+        load_keys<8>(tv.leaves[leaf[i]].key, k);
+        lt[i] = group_count_lt<1, 8>(k, t[i], 0);
+        eq[i] = group_any_eq<1, 8>(k, t[i], 0) && t[i] != KEY_MAX;
+      }
+    }
+    uint64_t val[kWinQ];
+#pragma unroll
+    for (int i = 0; i < kWinQ; ++i) val[i] = eq[i] ? tv.leaves[leaf[i]].val[lt[i]] : 0ull;
+#pragma unroll
+    for (int i = 0; i < kWinQ; ++i) {
+      const uint32_t qi = cb + lane + 64 * i;
+      if (qi < end) {
+        out_vals[qi] = val[i];
+        out_found[qi] = eq[i] ? 1 : 0;
+      }
+    }
+    wave_lds_sync();  // the next chunk's ring loads overwrite what this one read
+  }
+}
+
+template <int F>
+hipError_t launch_window(const TreeView& tv, const SearchLaunch& sl, const uint64_t* q, uint64_t* out_vals,
+                         uint8_t* out_found, size_t n, hipStream_t s, const uint32_t* dn) {
+  const LaunchCfg& lc = launch_cfg();
+  const size_t per_block = (size_t)kWinThreads * kWinQ;
+  const size_t max_blocks = (size_t)lc.num_cus * (size_t)(sl.blocks_per_cu > 0 ? sl.blocks_per_cu : 2);
+  size_t nb = (n + per_block - 1) / per_block;
+  nb = nb < max_blocks ? nb : max_blocks;
+  nb = (nb + 7) / 8 * 8;
+  hipLaunchKernelGGL((search_window_kernel<F, kWindowLeaves>), dim3((unsigned)nb), dim3(kWinThreads), 0, s, tv, q,
+                     out_vals, out_found, (uint32_t)n, dn);
+  return hipGetLastError();
+}
+
 // Locate for the update path: leaf id, lower_bound slot and found flag per key (input order).
 template <int F, int G>
 __global__ __launch_bounds__(kThreads) void locate_kernel(TreeView tv, const uint64_t* __restrict__ keys, uint32_t n,
@@ -211,7 +387,7 @@ __global__ __launch_bounds__(kThreads) void locate_kernel(TreeView tv, const uin
     const bool valid = qi < n;
     const uint64_t t = valid ? keys[qi] : 0ull;
     const uint64_t td = t == KEY_MAX ? KEY_MAX - 1 : t;
-    const uint32_t leaf = descend<F, G>(tv, 0, nullptr, nullptr, td, gl, gbase);
+    const uint32_t leaf = descend<F, G>(tv, 0, nullptr, td, gl, gbase);
     uint64_t k[W];
     load_keys<W>(tv.leaves[leaf].key + gl * W, k);
     const int lt = group_count_lt<G, W>(k, t, gbase);
@@ -302,6 +478,11 @@ template <int F>
 hipError_t launch_f(const TreeView& tv, const SearchLaunch& sl, const uint64_t* q, const uint32_t* pos,
                     uint64_t* out_vals, uint8_t* out_found, size_t n, SearchOut mode, hipStream_t s,
                      const uint32_t* dn) {
+  if (sl.sorted && sl.window && mode == OUT_DIRECT) {
+    SearchLaunch w = sl;
+    w.blocks_per_cu = sl.window_blocks_per_cu;
+    return launch_window<F>(tv, w, q, out_vals, out_found, n, s, dn);
+  }
   if (sl.sorted && sl.group == 1) {
     switch (sl.queries_per_group) {
       case 1: return launch_sorted_q<F, 1>(tv, sl, q, pos, out_vals, out_found, n, mode, s, dn);
@@ -324,7 +505,7 @@ hipError_t launch_f(const TreeView& tv, const SearchLaunch& sl, const uint64_t* 
 size_t lds_bytes_for(const TreeView& tv, int levels) {
   if (levels <= 0) return 0;
   const size_t nl = tv.level_start[levels];
-  return nl * (size_t)node_slots(tv.fanout) * sizeof(uint64_t) + nl * sizeof(uint32_t);
+  return nl * (size_t)node_slots(tv.fanout) * sizeof(uint64_t);
 }
 
 hipError_t launch_search(const TreeView& tv, const SearchLaunch& sl, const uint64_t* q, const uint32_t* pos,

@@ -6,8 +6,10 @@
 // stream of requests against the serving loop: 85% point-lookup requests (batch size log-uniform in
 // [1, 65536]; 90% hits), 10% range requests (one range, length uniform in [16, 256]), 5% insert
 // batches (1..4096 new keys, log-uniform).  Every result is checked: hits are found with
-// value = splitmix64_mix(key), misses (keys never inserted) are absent, ranges return the right
-// number of ascending keys with matching values.  Prints one JSON object per run.
+// value = splitmix64_mix(key), misses (keys never inserted) are absent, and each range returns exactly
+// the first `length` keys >= its start of the key set at the epoch it reports (the initial keys, read
+// back from the built tree's leaves, plus the inserts published up to that epoch), with their values.
+// Prints one JSON object per run.
 //
 // Usage: hm_serve_bench --keys-log2 27 --requests 20000 --rate 5000 [--plain] [--closed]
 //                       [--inflight 4] [--budget-mb 4096] [--seed 1] [--leaf-fill 5]
@@ -119,6 +121,22 @@ extern "C" int hm_serve_bench_main(int argc, char** argv, char* out_json, uint64
   CHECK(hm_tree_build(keys.data(), vals.data(), n, &p, &t));
   const double build_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - tb).count();
   std::fprintf(stderr, "built in %.2f s\n", build_s);
+  // the initial key set in order, for the range checks: the leaf chain of the built tree, whose key set
+  // is checked against `keys` (same count, every key present by the lookups below, none extra)
+  std::vector<uint64_t> sorted;
+  {
+    hm_stats st0;
+    CHECK(hm_tree_stats(t, &st0));
+    std::vector<uint64_t> lv(st0.n_leaves * 16);
+    CHECK(hm_tree_export(t, nullptr, nullptr, lv.data()));
+    sorted.reserve(n);
+    for (uint64_t i = 0; i < st0.n_leaves; ++i)
+      for (int s2 = 0; s2 < 8 && lv[i * 16 + s2] != ~0ull; ++s2) sorted.push_back(lv[i * 16 + s2]);
+    if (sorted.size() != n || !std::is_sorted(sorted.begin(), sorted.end())) {
+      std::fprintf(stderr, "leaf chain of the built tree is not the key set in order\n");
+      return 1;
+    }
+  }
 
   // request stream
   std::mt19937_64 g(seed * 7919 + 17);
@@ -170,6 +188,7 @@ extern "C" int hm_serve_bench_main(int argc, char** argv, char* out_json, uint64
   hm_server* sv = nullptr;
   CHECK(hm_server_create(t, &sp, &sv));
   // warm-up outside the measurement: every request kind once (first launches load the kernels)
+  const uint64_t warm_key = perm63(base + n + M + (1ull << 41) + 1, salt);
   {
     std::vector<uint64_t> wv(4096), wk(64), wv2(64);
     std::vector<uint8_t> wf(4096);
@@ -179,7 +198,7 @@ extern "C" int hm_serve_bench_main(int argc, char** argv, char* out_json, uint64
     CHECK(hm_server_wait(sv, tk, nullptr, nullptr));
     CHECK(hm_server_range(sv, keys.data(), &wlen, 1, wk.data(), wv2.data(), &wc, 64, &tk));
     CHECK(hm_server_wait(sv, tk, nullptr, nullptr));
-    const uint64_t uk = perm63(base + n + M + (1ull << 41) + 1, salt), uv = value_of(uk);
+    const uint64_t uk = warm_key, uv = value_of(uk);
     CHECK(hm_server_update(sv, &uk, &uv, 1, &tk));
     CHECK(hm_server_wait(sv, tk, nullptr, nullptr));
     CHECK(hm_server_reset_stats(sv));
@@ -204,12 +223,22 @@ extern "C" int hm_serve_bench_main(int argc, char** argv, char* out_json, uint64
   CHECK(hm_server_drain(sv));
   hm_server_stats st;
   CHECK(hm_server_get_stats(sv, &st));
-  // verify every result
+  // verify every result; first the epoch of every request, and the inserted keys by key with the
+  // epoch that published them
   uint64_t bad = 0;
-  for (auto& r : reqs) {
-    int status = 0;
-    CHECK(hm_server_wait(sv, r.ticket, &status, nullptr));
-    if (status) {
+  std::vector<uint64_t> epoch_of(reqs.size());
+  std::vector<int> status_of(reqs.size());
+  std::vector<std::pair<uint64_t, uint64_t>> ins{{warm_key, 0}};  // (key, publishing epoch); the warm-up
+                                                                 // insert precedes every request
+  for (size_t i = 0; i < reqs.size(); ++i) {
+    CHECK(hm_server_wait(sv, reqs[i].ticket, &status_of[i], &epoch_of[i]));
+    if (reqs[i].type == 2 && !status_of[i])
+      for (uint64_t k : reqs[i].keys) ins.emplace_back(k, epoch_of[i]);
+  }
+  std::sort(ins.begin(), ins.end());
+  for (size_t ri = 0; ri < reqs.size(); ++ri) {
+    auto& r = reqs[ri];
+    if (status_of[ri]) {
       ++bad;
       continue;
     }
@@ -220,12 +249,21 @@ extern "C" int hm_serve_bench_main(int argc, char** argv, char* out_json, uint64
           break;
         }
     } else if (r.type == 1) {
-      const uint32_t c = r.out_c[0];
-      bool ok = c <= r.lens[0];
-      for (uint32_t j = 0; ok && j < c; ++j) {
-        ok = r.out_k[j] >= r.keys[0] && r.out_v[j] == value_of(r.out_k[j]) && (j == 0 || r.out_k[j - 1] < r.out_k[j]);
+      // the first lens[0] keys >= start of the initial keys and the inserts visible at this epoch
+      const uint64_t start = r.keys[0], ep = epoch_of[ri];
+      auto a = std::lower_bound(sorted.begin(), sorted.end(), start);
+      auto b = std::lower_bound(ins.begin(), ins.end(), std::make_pair(start, (uint64_t)0));
+      uint32_t c = 0;
+      bool ok = true;
+      while (c < r.lens[0] && ok) {
+        while (b != ins.end() && b->second > ep) ++b;  // published after this read's epoch
+        const bool from_init = a != sorted.end() && (b == ins.end() || *a < b->first);
+        if (!from_init && b == ins.end()) break;  // end of the key set
+        const uint64_t k = from_init ? *a++ : (b++)->first;
+        ok = c < r.out_c[0] && r.out_k[c] == k && r.out_v[c] == value_of(k);  // inserts carry value_of too
+        ++c;
       }
-      if (!ok) ++bad;
+      if (!ok || r.out_c[0] != c) ++bad;
     }
   }
   CHECK(hm_server_destroy(sv));

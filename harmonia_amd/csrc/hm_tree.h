@@ -74,6 +74,7 @@ struct hm_tree {
   bool na_valid = false;
   uint64_t epoch = 0;
   uint64_t relayouts = 0;
+  uint64_t respreads = 0;  // last-level nodes re-spread in place by update epochs
   uint64_t* pend_k = nullptr;
   uint64_t* pend_v = nullptr;
   size_t pend_n = 0, pend_cap = 0;
@@ -131,6 +132,7 @@ struct PublishPlan {
   size_t m = 0;          // unique keys among them
   uint64_t n_new = 0;    // keys not in the tree before
   bool relayout = false; // the standby layout holds the new epoch
+  uint32_t respread = 0;  // last-level nodes re-spread in place (their list in the update scratch)
   uint64_t new_leaves = 0;
   Levels lv;
 };

@@ -449,24 +449,49 @@ __device__ __forceinline__ bool run_head(const uint32_t* __restrict__ leaf, uint
   return i == 0 || leaf[i] != leaf[i - 1];
 }
 
-__global__ void plan_runs_kernel(const Leaf* __restrict__ leaves, const uint32_t* __restrict__ leaf,
-                                 const uint8_t* __restrict__ found, size_t n_max, const uint32_t* __restrict__ dm,
-                                 uint32_t* __restrict__ summary) {
-  __shared__ uint32_t block_new, block_over;
-  if (threadIdx.x == 0) block_new = block_over = 0;
+// Plan an epoch, one thread per last-level node the batch touches (its first batch entry: the batch
+// is sorted, so a node's entries are contiguous).  Last-level node j holds leaves
+// [j F, min((j + 1) F, n_leaves)): the bulk build and re-layouts fill every node but the last, and
+// an in-node re-spread keeps each node's leaf count.  summary[1] += new keys; if a leaf of the node
+// would overflow, summary[0] |= 1 and the node either goes on the re-spread list (nlist: node, first
+// batch entry; count in summary[4]) when its leaves can hold all its keys, or summary[0] |= 2 (the
+// epoch needs a full re-layout).
+__global__ void plan_runs_kernel(const Leaf* __restrict__ leaves, uint32_t n_leaves, int F,
+                                 const uint32_t* __restrict__ leaf, const uint8_t* __restrict__ found, size_t n_max,
+                                 const uint32_t* __restrict__ dm, uint32_t* __restrict__ summary,
+                                 uint32_t* __restrict__ nlist) {
+  __shared__ uint32_t block_new;
+  if (threadIdx.x == 0) block_new = 0;
   __syncthreads();
   const uint32_t m = dm ? *dm : (uint32_t)n_max;
   const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t uF = (uint32_t)F;
   uint32_t my_new = 0;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
     my_new += found[i] ? 0u : 1u;
-    if (run_head(leaf, i)) {
-      // count this leaf's new keys, stopping once it cannot fit
-      const uint32_t L = leaf[i];
+    const uint32_t node = leaf[i] / uF;
+    if (i > 0 && leaf[i - 1] / uF == node) continue;  // not the node's first entry
+    bool over = false;
+    uint32_t node_ins = 0;
+    for (uint32_t j = i; j < m && leaf[j] / uF == node;) {
+      const uint32_t L = leaf[j];
       const uint32_t room = (uint32_t)(LEAF_KEYS - leaf_count(leaves[L]));
       uint32_t ins = 0;
-      for (uint32_t j = i; j < m && leaf[j] == L && ins <= room; ++j) ins += found[j] ? 0u : 1u;
-      if (ins > room) block_over = 1u;  // a benign race: every writer stores 1
+      for (; j < m && leaf[j] == L; ++j) ins += found[j] ? 0u : 1u;
+      over |= ins > room;
+      node_ins += ins;
+    }
+    if (!over) continue;
+    atomicOr(&summary[0], 1u);
+    const uint32_t x0 = node * uF, x1 = x0 + uF < n_leaves ? x0 + uF : n_leaves;
+    uint32_t total = node_ins;
+    for (uint32_t x = x0; x < x1; ++x) total += (uint32_t)leaf_count(leaves[x]);
+    if (total <= (x1 - x0) * (uint32_t)LEAF_KEYS) {
+      const uint32_t k = atomicAdd(&summary[4], 1u);
+      nlist[2 * k] = node;
+      nlist[2 * k + 1] = i;
+    } else {
+      atomicOr(&summary[0], 2u);
     }
   }
   // new-key count: wave sum, then one LDS atomic per wave and one global atomic per block (a
@@ -476,7 +501,131 @@ __global__ void plan_runs_kernel(const Leaf* __restrict__ leaves, const uint32_t
   if ((threadIdx.x & 63) == 0 && my_new) atomicAdd(&block_new, my_new);
   __syncthreads();
   if (threadIdx.x == 0 && block_new) atomicAdd(&summary[1], block_new);
-  if (threadIdx.x == 0 && block_over) atomicOr(&summary[0], 1u);
+}
+
+// Mark the batch entries of the listed nodes (found |= 2) so the in-place merge leaves them to the
+// node re-spread.  One block per listed node.
+__global__ void mark_nodes_kernel(const uint32_t* __restrict__ nlist, uint32_t n_nodes, int F,
+                                  const uint32_t* __restrict__ leaf, uint8_t* __restrict__ found,
+                                  const uint32_t* __restrict__ dm) {
+  const uint32_t m = *dm;
+  for (uint32_t k = blockIdx.x; k < n_nodes; k += gridDim.x) {
+    const uint32_t node = nlist[2 * k], i0 = nlist[2 * k + 1];
+    for (uint32_t j = i0 + threadIdx.x;; j += blockDim.x) {
+      if (j >= m || leaf[j] / (uint32_t)F != node) break;
+      found[j] |= 2u;
+    }
+  }
+}
+
+// Re-spread one last-level node in place (one 64-lane block per listed node): its leaves' keys with
+// the batch's values applied and its new keys merged in, spread evenly over the same leaves (leaf s
+// gets merged keys [s K / c, (s + 1) K / c), at least one and at most 8 each), then the node's
+// separators rewritten as the leaves' minima (key[s] = min of leaf s + 1) and its index granule
+// (idx[g] = key[8 g + 7]; the last index slot, the node's prefix entry, is kept).  The node's
+// smallest key does not change (new keys are >= the separator that leads to the node, which is that
+// key), so the levels above stay valid.  Bounded cost: at most F leaves and one node per overflow.
+template <int F>
+__global__ __launch_bounds__(64) void node_respread_kernel(Leaf* __restrict__ leaves, uint64_t* __restrict__ nodes,
+                                                           uint32_t n_leaves, const uint32_t* __restrict__ nlist,
+                                                           uint32_t n_nodes, const uint32_t* __restrict__ leaf,
+                                                           const uint32_t* __restrict__ slot,
+                                                           const uint8_t* __restrict__ found,
+                                                           const uint64_t* __restrict__ uk,
+                                                           const uint64_t* __restrict__ uv,
+                                                           const uint32_t* __restrict__ dm) {
+  constexpr int CAP = LEAF_KEYS * F;
+  constexpr int IS = idx_slots(F);
+  constexpr int NS = node_slots(F);
+  __shared__ uint64_t ek[CAP], ev[CAP];  // the node's keys in order (values updated), then merged
+  __shared__ uint64_t nk[CAP], nv[CAP];  // its new keys in order
+  __shared__ uint64_t mk[CAP], mv[CAP];
+  __shared__ uint32_t off[F + 1];
+  const int lane = threadIdx.x;
+  const uint32_t m = *dm;
+  for (uint32_t k = blockIdx.x; k < n_nodes; k += gridDim.x) {
+    const uint32_t node = nlist[2 * k], i0 = nlist[2 * k + 1];
+    const uint32_t x0 = node * (uint32_t)F;
+    const uint32_t c = (x0 + F < n_leaves ? x0 + F : n_leaves) - x0;
+    // leaf offsets in the node's key list: counts, then a wave scan (c <= F <= 128: two per lane)
+    uint32_t c0 = 0, c1 = 0;
+    if ((uint32_t)lane < c) c0 = (uint32_t)leaf_count(leaves[x0 + lane]);
+    if ((uint32_t)lane + 64 < c) c1 = (uint32_t)leaf_count(leaves[x0 + 64 + lane]);
+    const uint32_t s0 = wave_inclusive_scan(c0);
+    const uint32_t t0 = (uint32_t)__shfl((int)s0, 63);
+    const uint32_t s1 = wave_inclusive_scan(c1) + t0;
+    if ((uint32_t)lane < c) off[lane] = s0 - c0;
+    if ((uint32_t)lane + 64 < c) off[64 + lane] = s1 - c1;
+    const uint32_t ke = (uint32_t)__shfl((int)s1, 63);  // existing keys
+    if (lane == 0) off[c] = ke;
+    __syncthreads();
+    for (uint32_t u = lane; u < c * LEAF_KEYS; u += 64) {
+      const uint32_t s = u / LEAF_KEYS, q = u % LEAF_KEYS;
+      if (q < off[s + 1] - off[s]) {
+        ek[off[s] + q] = leaves[x0 + s].key[q];
+        ev[off[s] + q] = leaves[x0 + s].val[q];
+      }
+    }
+    __syncthreads();
+    // the batch entries of this node: values of existing keys, and the new keys compacted in order
+    uint32_t kn = 0;
+    for (uint32_t j0 = i0;; j0 += 64) {  // uniform: the node's entries are contiguous from i0
+      const uint32_t j = j0 + lane;
+      const bool in = j < m && leaf[j] / (uint32_t)F == node;
+      const uint8_t f = in ? (found[j] & 1u) : 0u;
+      if (in && f) ev[off[leaf[j] - x0] + slot[j]] = uv[j];
+      const uint32_t isnew = in && !f ? 1u : 0u;
+      const uint32_t inc = wave_inclusive_scan(isnew);
+      if (isnew) {
+        nk[kn + inc - 1] = uk[j];
+        nv[kn + inc - 1] = uv[j];
+      }
+      kn += (uint32_t)__shfl((int)inc, 63);
+      if (!__any(in) || j0 + 64 >= m) break;
+    }
+    __syncthreads();
+    // merge: an element's place = its index + how many of the other list are smaller (keys distinct)
+    const uint32_t K = ke + kn;
+    for (uint32_t a = lane; a < ke; a += 64) {
+      uint32_t lo = 0, hi = kn;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (nk[mid] < ek[a]) lo = mid + 1; else hi = mid;
+      }
+      mk[a + lo] = ek[a];
+      mv[a + lo] = ev[a];
+    }
+    for (uint32_t b = lane; b < kn; b += 64) {
+      uint32_t lo = 0, hi = ke;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (ek[mid] < nk[b]) lo = mid + 1; else hi = mid;
+      }
+      mk[b + lo] = nk[b];
+      mv[b + lo] = nv[b];
+    }
+    __syncthreads();
+    // leaves: leaf s = merged [s K / c, (s + 1) K / c), padded
+    for (uint32_t u = lane; u < c * 2 * LEAF_KEYS; u += 64) {
+      const uint32_t s = u / (2 * LEAF_KEYS), q = u % (2 * LEAF_KEYS);
+      const uint32_t lo = s * K / c, hi = (s + 1) * K / c;
+      const uint32_t p = lo + (q % LEAF_KEYS);
+      Leaf* L = leaves + x0 + s;
+      if (q < LEAF_KEYS)
+        L->key[q] = p < hi ? mk[p] : KEY_MAX;
+      else
+        L->val[q - LEAF_KEYS] = p < hi ? mv[p] : 0ull;
+    }
+    // the node: separators, then the index slots below the last
+    uint64_t* nd = nodes + (size_t)node * NS;
+    for (uint32_t s = lane; s < (uint32_t)F; s += 64)
+      nd[IS + s] = s + 1 < c ? mk[(s + 1) * K / c] : KEY_MAX;
+    for (uint32_t g = lane; g + 1 < (uint32_t)IS; g += 64) {
+      const uint32_t ks = 8 * g + 7;
+      nd[g] = (g < (uint32_t)(F / 8) && ks + 1 < c) ? mk[(ks + 1) * K / c] : KEY_MAX;
+    }
+    __syncthreads();  // the LDS lists are reused by the next node
+  }
 }
 
 __global__ void apply_runs_kernel(Leaf* __restrict__ leaves, const uint32_t* __restrict__ leaf,
@@ -486,7 +635,7 @@ __global__ void apply_runs_kernel(Leaf* __restrict__ leaves, const uint32_t* __r
   const uint32_t m = dm ? *dm : (uint32_t)n_max;
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
-    if (!run_head(leaf, i)) continue;
+    if (!run_head(leaf, i) || (found[i] & 2u)) continue;  // marked: the node re-spread handles it
     const uint32_t L = leaf[i];
     uint32_t ins = 0;
     for (uint32_t j = i; j < m && leaf[j] == L; ++j) {
@@ -547,11 +696,46 @@ hipError_t unique_count(const uint32_t* keep, const uint32_t* keep_scan, size_t 
   return hipGetLastError();
 }
 
-hipError_t plan_runs(const Leaf* leaves, const uint32_t* leaf, const uint8_t* found, size_t n, const uint32_t* dm,
-                     uint32_t* summary, hipStream_t s) {
+hipError_t plan_runs(const Leaf* leaves, uint32_t n_leaves, int F, const uint32_t* leaf, const uint8_t* found, size_t n,
+                     const uint32_t* dm, uint32_t* summary, uint32_t* nlist, hipStream_t s) {
   if (!n) return hipSuccess;
   ProfScope prof("update", s);
-  hipLaunchKernelGGL(plan_runs_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, leaves, leaf, found, n, dm, summary);
+  hipLaunchKernelGGL(plan_runs_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, leaves, n_leaves, F, leaf, found, n,
+                     dm, summary, nlist);
+  return hipGetLastError();
+}
+
+hipError_t mark_nodes(const uint32_t* nlist, uint32_t n_nodes, int F, const uint32_t* leaf, uint8_t* found,
+                      const uint32_t* dm, hipStream_t s) {
+  if (!n_nodes) return hipSuccess;
+  ProfScope prof("update", s);
+  const unsigned g = n_nodes < 4096 ? n_nodes : 4096;
+  hipLaunchKernelGGL(mark_nodes_kernel, dim3(g), dim3(64), 0, s, nlist, n_nodes, F, leaf, found, dm);
+  return hipGetLastError();
+}
+
+hipError_t node_respread(Leaf* leaves, uint64_t* last_level, uint32_t n_leaves, int F, const uint32_t* nlist,
+                         uint32_t n_nodes, const uint32_t* leaf, const uint32_t* slot, const uint8_t* found,
+                         const uint64_t* uk, const uint64_t* uv, const uint32_t* dm, hipStream_t s) {
+  if (!n_nodes) return hipSuccess;
+  ProfScope prof("update", s);
+  const unsigned g = n_nodes < 4096 ? n_nodes : 4096;
+  switch (F) {
+    case 32:
+      hipLaunchKernelGGL(node_respread_kernel<32>, dim3(g), dim3(64), 0, s, leaves, last_level, n_leaves, nlist,
+                         n_nodes, leaf, slot, found, uk, uv, dm);
+      break;
+    case 64:
+      hipLaunchKernelGGL(node_respread_kernel<64>, dim3(g), dim3(64), 0, s, leaves, last_level, n_leaves, nlist,
+                         n_nodes, leaf, slot, found, uk, uv, dm);
+      break;
+    case 128:
+      hipLaunchKernelGGL(node_respread_kernel<128>, dim3(g), dim3(64), 0, s, leaves, last_level, n_leaves, nlist,
+                         n_nodes, leaf, slot, found, uk, uv, dm);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

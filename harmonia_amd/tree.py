@@ -141,11 +141,12 @@ class BPlusTree:
     # ---- point search --------------------------------------------------------------------
     def search_batch(self, q, out_vals=None, out_found=None, stream=None, group_size=0, lds_levels=-1,
                      psa_bits=-1, blocks_per_cu=0, queries_per_group=0, output_mode=0, sorted_output=False,
-                     out_pos=None):
+                     out_pos=None, sorted_kernel=0):
         """Batched search; returns (values, found) in input order (or PSA order with out_pos).
 
         output_mode: 0 tree default, 1 input order via the replayed sort passes, 2 input order via direct
-        scatter, 3 (or sorted_output=True) PSA order with the permutation in out_pos."""
+        scatter, 3 (or sorted_output=True) PSA order with the permutation in out_pos.
+        sorted_kernel (PSA batches): 0 automatic, 1 one descent per query, 2 leaf window."""
         if sorted_output:
             output_mode = _lib.HM_OUT_PSA_ORDER
         q = _u64(q)
@@ -155,7 +156,7 @@ class BPlusTree:
         if out_found is None:
             out_found = np.empty(n, dtype=np.uint8)
         o = HmSearchOpts(group_size, lds_levels, psa_bits, blocks_per_cu, output_mode, queries_per_group,
-                         _ptr(out_pos))
+                         _ptr(out_pos), sorted_kernel)
         check(self._lib.hm_search_ex(self._handle(), _ptr(q), n, _ptr(out_vals), _ptr(out_found),
                                      ctypes.byref(o), _stream(stream)))
         return out_vals, out_found
@@ -341,6 +342,20 @@ class Batcher:
         if hasattr(q, "data_ptr"):
             self._keep.append(q)
         return int(off.value)
+
+    def add_many(self, ptrs, sizes, device=True, offsets=None):
+        """Add batches given as raw addresses (uint64 array) and sizes until the buffer is full; returns
+        how many were added.  With device=True every address must be device memory, kept valid until
+        the next flush has run (sync())."""
+        ptrs = np.ascontiguousarray(ptrs, dtype=np.uint64)
+        sizes = np.ascontiguousarray(sizes, dtype=np.uint64)
+        if len(ptrs) != len(sizes):
+            raise ValueError("ptrs and sizes differ in length")
+        added = ctypes.c_uint64(0)
+        check(self._lib.hm_batcher_add_many(self._h, _ptr(ptrs), _ptr(sizes), len(ptrs),
+                                            _lib.HM_BATCH_DEVICE if device else 0, _ptr(offsets),
+                                            ctypes.byref(added)))
+        return int(added.value)
 
     def flush(self):
         check(self._lib.hm_batcher_flush(self._h))

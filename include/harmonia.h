@@ -66,7 +66,9 @@ typedef struct hm_stats {
   uint64_t bytes_leaves;
   uint64_t bytes_device;   /* everything the tree holds on the device */
   uint64_t epoch;          /* number of published update epochs */
-  uint64_t relayouts;      /* epochs that re-laid out every leaf (some leaf overflowed) */
+  uint64_t relayouts;      /* epochs that re-laid out every leaf (a last-level node was full) */
+  uint64_t respreads;      /* last-level nodes whose keys an epoch re-spread over their own leaves in
+                            * place (a leaf overflowed, the node had room) */
 } hm_stats;
 
 /* hm_search_opts.output_mode (with PSA on; without PSA results are always in input order) */
@@ -84,7 +86,16 @@ typedef struct hm_search_opts {
   uint32_t output_mode;    /* HM_OUT_* */
   uint32_t queries_per_group; /* independent queries each group keeps in flight: 1, 2, 4; 0 = default */
   uint32_t* out_pos;       /* with HM_OUT_PSA_ORDER: out_pos[i] = input index of result i */
+  uint32_t sorted_kernel;  /* search kernel for PSA-sorted batches: HM_SORTED_* */
+  uint32_t reserved;
 } hm_search_opts;
+
+/* hm_search_opts.sorted_kernel */
+#define HM_SORTED_AUTO 0    /* the tree's choice (one descent per query) */
+#define HM_SORTED_QUERY 1   /* one descent per query (waves share upper-level node reads) */
+#define HM_SORTED_WINDOW 2  /* leaf window: each wave streams the leaves its chunk of sorted queries
+                             * spans into LDS and ranks the chunk there (chunks spanning more
+                             * than 256 leaves fall back to one descent per query) */
 
 /* Fill *p with defaults: F=64, fill 8, LDS auto, group auto, PSA 16 bits, threshold 2^20. */
 int hm_params_init(hm_params* p);
@@ -179,6 +190,14 @@ int hm_batcher_create(hm_tree* t, uint64_t capacity, const hm_search_opts* opts,
  * the call returns; a device batch is only recorded and read by the next flush's work on the
  * batcher's stream, so keep it valid and unchanged until then (hm_batcher_sync). */
 int hm_batcher_add(hm_batcher* b, const uint64_t* q, uint64_t n, uint64_t* out_offset);
+/* Add batches qs[0..count) (ns[i] queries each) in order until the next one does not fit; *n_added =
+ * how many were added (flush, then add the rest), out_offsets[i] (may be NULL) = slot of batch i.
+ * flags HM_BATCH_DEVICE: every qs[i] is device memory (the per-batch pointer query is skipped).
+ * One call per flush in place of the reference driver's per-bucket host loop
+ * (H/regular_bpt/prefix-sort.cu:261-309). */
+#define HM_BATCH_DEVICE 1u
+int hm_batcher_add_many(hm_batcher* b, const uint64_t* const* qs, const uint64_t* ns, uint64_t count, uint32_t flags,
+                        uint64_t* out_offsets, uint64_t* n_added);
 /* Launch the pipeline over everything added since the last flush (asynchronous). */
 int hm_batcher_flush(hm_batcher* b);
 /* Device pointers of the result buffers (valid in stream order after a flush). */

@@ -82,7 +82,7 @@ class BPlusTree {
   std::optional<uint64_t> search(uint64_t key) {
     uint64_t v = 0;
     uint8_t f = 0;
-    hm_search_opts o{0, -1, 0, 0, 0, 0, nullptr};
+    hm_search_opts o{0, -1, 0, 0, 0, 0, nullptr, 0, 0};
     check(hm_search_ex(handle(), &key, 1, &v, &f, &o, nullptr));
     if (!f) return std::nullopt;
     return v;

@@ -85,10 +85,44 @@ def test_search_all_launch_shapes(gpu, F):
                 # kernel then meets more distinct nodes per wave than it shares and falls back
                 for psa in (0, 2, 16):
                     for mode in ((1, 2) if psa else (0,)):
-                        v, f = t.search_batch(q, group_size=g, queries_per_group=qpg, lds_levels=lds, psa_bits=psa,
-                                              output_mode=mode)
-                        assert np.array_equal(f, ef), (g, qpg, lds, psa, mode)
-                        assert np.array_equal(v, ev), (g, qpg, lds, psa, mode)
+                        for sk_ in ((1, 2) if psa else (0,)):  # one descent per query / leaf window
+                            v, f = t.search_batch(q, group_size=g, queries_per_group=qpg, lds_levels=lds,
+                                                  psa_bits=psa, output_mode=mode, sorted_kernel=sk_)
+                            assert np.array_equal(f, ef), (g, qpg, lds, psa, mode, sk_)
+                            assert np.array_equal(v, ev), (g, qpg, lds, psa, mode, sk_)
+
+
+@pytest.mark.parametrize("F", [32, 64, 128])
+def test_search_leaf_window(gpu, F):
+    """The leaf-window kernel on dense and sparse batches (chunks within the LDS ring and chunks that
+    fall back to one descent per query), skewed and edge-key queries, nearly unsorted input (PSA over
+    2 bits: ranges that move backwards between chunks), after in-place inserts and after a re-layout."""
+    t, keys, vals, sk, sv = make(300007, F, 6, seed=3 * F)
+    cur_k, cur_v = sk, sv
+    for step in range(3):
+        for m, bits in ((400000, 16), (200000, 24), (60000, 16), (3000, 16), (150000, 2), (100000, 32)):
+            base = W.point_queries(cur_k, m, 100 * step + m % 97)
+            z = W.zipf_queries(cur_k, m // 3, 7 + step)
+            q = np.concatenate([base, z, edge(cur_k)])
+            ev, ef = oracle.search_sorted(cur_k, cur_v, q)
+            for mode in (1, 3):
+                pos = np.empty(len(q), dtype=np.uint32) if mode == 3 else None
+                v, f = t.search_batch(q, psa_bits=bits, output_mode=mode, sorted_kernel=2, out_pos=pos)
+                if mode == 3:
+                    v2, f2 = np.empty_like(v), np.empty_like(f)
+                    v2[pos], f2[pos] = v, f
+                    v, f = v2, f2
+                assert np.array_equal(f, ef), (step, m, bits, mode)
+                assert np.array_equal(v, ev), (step, m, bits, mode)
+        # step 0 -> 1: in-place inserts (leaf minima below their old minimum are impossible: an insert
+        # lands in the leaf whose separator is <= the key); step 1 -> 2: enough inserts to re-lay out
+        n_new = 20000 if step == 0 else 400000
+        new = W.gen_misses(len(keys), n_new, 50 + step) ^ np.uint64(step << 40)
+        new = np.setdiff1d(new, cur_k)
+        nv = W.splitmix64_mix(new ^ np.uint64(9))
+        t.update(new, nv)
+        t.publish()
+        cur_k, cur_v = oracle.upsert_sorted(cur_k, cur_v, new, nv)
 
 
 @pytest.mark.parametrize("F", [32, 64])
@@ -295,6 +329,61 @@ def test_update_in_place_and_relayout(gpu, fill):
         full_check(t, cur_k, cur_v, 60 + step)
     if fill == 8:
         assert rebuilt_any  # full leaves force the re-layout path
+
+
+def keys_inside_leaf(cur_k, lo_i, hi_i, count, seed):
+    """`count` keys strictly between cur_k[lo_i] and cur_k[hi_i] and not in cur_k."""
+    g = W.rng(seed, 3)
+    a, b = int(cur_k[lo_i]), int(cur_k[hi_i])
+    out = np.unique(g.integers(a + 1, b, size=4 * count, dtype=np.uint64))
+    out = out[~np.isin(out, cur_k)]
+    assert len(out) >= count
+    return out[:count]
+
+
+@pytest.mark.parametrize("F", [32, 64, 128])
+def test_update_node_respread(gpu, F):
+    """Leaves that overflow inside a last-level node with room: that node's keys are re-spread over its
+    own leaves in place (no re-layout) -- first and last leaves of nodes, two overflowing leaves in one
+    node, the tree's last leaf -- with every result, the layout invariants (separator = minimum of its
+    right subtree) and the leaf-window search checked; a node without room forces the re-layout."""
+    fill = 6
+    t, keys, vals, sk, sv = make(50000, F, fill, seed=91 + F)
+    cur_k, cur_v = sk, sv
+    nl = int(t.stats().n_leaves)
+    g = W.rng(92, F)
+    for step, leaves in enumerate(([0, F - 1, F, 3 * F + 2, 3 * F + 5], [nl - 1, 1, 2 * F + 1])):
+        bk = []
+        for L in leaves:
+            lo = L * fill
+            hi = min(lo + fill - 1, len(sk) - 1)
+            if L == nl - 1:  # keys above every key land in the last leaf
+                bk.append(np.arange(1, 10, dtype=np.uint64) + cur_k[-1])
+            else:
+                bk.append(keys_inside_leaf(sk, lo, hi, 4, 100 * step + L))
+        new = np.concatenate(bk)
+        old = cur_k[g.integers(0, len(cur_k), 200)]
+        bkeys = np.concatenate([new, old])
+        bvals = g.integers(0, 1 << 64, len(bkeys), dtype=np.uint64)
+        before = t.stats()
+        rebuilt = t.insert_parallel(bkeys, bvals)
+        after = t.stats()
+        assert not rebuilt and after.relayouts == before.relayouts
+        assert after.respreads > before.respreads
+        cur_k, cur_v = oracle.upsert_sorted(cur_k, cur_v, bkeys, bvals)
+        full_check(t, cur_k, cur_v, 300 + step)
+        q = np.concatenate([cur_k, W.gen_misses(len(keys), 20000, 5 + step)])
+        ev, ef = oracle.search_sorted(cur_k, cur_v, q)
+        v, f = t.search_batch(q, psa_bits=16, sorted_kernel=2)
+        assert np.array_equal(f, ef) and np.array_equal(v, ev)
+    # more new keys than one node's leaves can hold: the epoch re-lays every leaf out
+    lo = 5 * F * fill
+    crowd = keys_inside_leaf(sk, lo, lo + F * fill - 1, 3 * F, 77)
+    before = t.stats()
+    assert t.insert_parallel(crowd, W.splitmix64_mix(crowd))
+    assert t.stats().relayouts == before.relayouts + 1
+    cur_k, cur_v = oracle.upsert_sorted(cur_k, cur_v, crowd, W.splitmix64_mix(crowd))
+    full_check(t, cur_k, cur_v, 400)
 
 
 @pytest.mark.parametrize("size", [1, 4095, 4096, 4097, 30000, 65537, (1 << 22) + 1])
