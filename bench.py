@@ -667,10 +667,11 @@ def main():
     # (order, PSA bits, output mode): mode 1 = results in input order (the headline), 3 = PSA order plus
     # positions (the reference's own contract)
     # sorted kernel (PSA batches): 0 = the tree's choice (one descent per query), 2 = the leaf window
-    variants = [("unsorted", 16, 1, 0), ("unsorted", 16, 1, 2), ("unsorted", 0, 1, 0), ("runs4096", 16, 1, 0),
-                ("runs4096", 0, 1, 0), ("unsorted", 16, 3, 0)]
+    variants = [("unsorted", 16, 1, 0), ("unsorted", 16, 1, 2), ("unsorted", 0, 1, 0),
+                ("runs4096", 16, 1, 0), ("runs4096", 0, 1, 0), ("unsorted", 16, 3, 0)]
     def vkey(order, psa, mode, skern):
-        return f"{order}_psa{psa}" + ("_psa_order" if mode == 3 else "") + ("_leaf_window_kernel" if skern == 2 else "")
+        return (f"{order}_psa{psa}" + ("_psa_order" if mode == 3 else "") +
+                {0: "", 1: "_per_query_kernel", 2: "_leaf_window_kernel"}[skern])
 
     if args.variants == "headline":
         variants = variants[:2]

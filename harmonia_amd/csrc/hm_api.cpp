@@ -190,8 +190,8 @@ int resolve_search(const hm_tree* t, uint64_t n, const hm_search_opts* o, Search
   const bool psa = bits > 0 && (n >= t->p.sort_threshold || (o && o->psa_bits > 0));
   const SearchCfg& base = psa ? t->cfg_psa : t->cfg_direct;
   sl->group = (o && o->group_size) ? (int)o->group_size : base.group;
-  if (sl->group != 1 && sl->group != 2 && sl->group != 4 && sl->group != 8)
-    return set_err(HM_EINVAL, "group_size %d not in {1,2,4,8}", sl->group);
+  if (sl->group < 1 || sl->group > 64 || (sl->group & (sl->group - 1)) || sl->group > t->F)
+    return set_err(HM_EINVAL, "group_size %d not a power of two in 1..min(64, fanout %d)", sl->group, t->F);
   sl->lds_levels = (o && o->lds_levels >= 0) ? choose_lds_levels(t, o->lds_levels) : t->lds_levels;
   sl->blocks_per_cu = (o && o->blocks_per_cu) ? (int)o->blocks_per_cu : base.blocks_per_cu;
   sl->queries_per_group = (o && o->queries_per_group) ? (int)o->queries_per_group : base.queries_per_group;
@@ -421,8 +421,8 @@ int check_params(const hm_params* p) {
     return set_err(HM_EINVAL, "fanout %u not in {32, 64, 128}", p->fanout);
   if (p->leaf_fill < 1 || p->leaf_fill > 8) return set_err(HM_EINVAL, "leaf_fill %u not in 1..8", p->leaf_fill);
   if (p->lds_levels < -1 || p->lds_levels > 3) return set_err(HM_EINVAL, "lds_levels %d not in -1..3", p->lds_levels);
-  if (p->group_size != 0 && p->group_size != 1 && p->group_size != 2 && p->group_size != 4 && p->group_size != 8)
-    return set_err(HM_EINVAL, "group_size %u not in {0,1,2,4,8}", p->group_size);
+  if (p->group_size > 64 || (p->group_size & (p->group_size - 1)) || p->group_size > p->fanout)
+    return set_err(HM_EINVAL, "group_size %u not 0 or a power of two in 1..min(64, fanout)", p->group_size);
   if (p->psa_bits > 32) return set_err(HM_EINVAL, "psa_bits %u > 32", p->psa_bits);
   return HM_OK;
 }

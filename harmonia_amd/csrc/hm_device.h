@@ -189,6 +189,48 @@ __device__ __forceinline__ void descend_q(const TreeView& tv, int lds_levels, co
 }
 
 
+// Wide narrowed groups, G = 16, 32 or 64 lanes: the reference's NTG shape, Thread_Per_Query =
+// DEFAULT_ORDER / Narrow_Rate lanes stepping over a node's keys (H/regular_bpt/prefix-sort.cu:30,148-167),
+// taken to one step: lane gl compares keys [gl * WN, gl * WN + WN) of the node's F key slots (WN = F / G),
+// so the whole node is ranked by one load per lane and the index granule is not read.  The prefix entry
+// (last index slot) is one more load, to the same address in every lane of the group.
+template <int F, int G>
+__device__ __forceinline__ uint32_t node_next_wide(const uint64_t* nd, uint64_t t, int gl, int gbase) {
+  constexpr int IS = idx_slots(F);
+  constexpr int WN = F / G;
+  static_assert(WN >= 1 && G >= 16, "wide groups: 16..F lanes");
+  uint64_t k[WN];
+  load_keys<WN>(nd + IS + gl * WN, k);
+  const uint32_t base = (uint32_t)nd[IS - 1];
+  return base + (uint32_t)group_count_le<G, WN>(k, t, gbase);
+}
+
+// Q descents of a wide group interleaved, every load of a level issued before the first is used.
+template <int F, int G, int Q>
+__device__ __forceinline__ void descend_wide_q(const TreeView& tv, int lds_levels, const uint64_t* snodes,
+                                               const uint64_t (&t)[Q], uint32_t (&node)[Q], int gl, int gbase) {
+  constexpr int IS = idx_slots(F);
+  constexpr int NS = node_slots(F);
+  constexpr int WN = F / G;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) node[q] = 0;
+  for (int l = 0; l < tv.height; ++l) {
+    const uint64_t* nodes = l < lds_levels ? snodes : tv.inner;
+    uint64_t k[Q][WN];
+    uint32_t base[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const uint64_t* nd = nodes + (size_t)node[q] * NS;
+      load_keys<WN>(nd + IS + gl * WN, k[q]);
+      base[q] = (uint32_t)nd[IS - 1];
+    }
+#pragma unroll
+    for (int q = 0; q < Q; ++q) node[q] = base[q] + (uint32_t)group_count_le<G, WN>(k[q], t[q], gbase);
+  }
+#pragma unroll
+  for (int q = 0; q < Q; ++q) node[q] -= tv.n_inner;
+}
+
 // One global level for PSA-sorted queries with G lanes per query.  At the upper levels the 64/G
 // queries of a wave's query slot almost always share their node, so the wave loads the node of its
 // first lane once (one coalesced 16-B piece per lane, prefix entry included) into the slot's LDS

@@ -113,6 +113,60 @@ __global__ __launch_bounds__(kThreads) void search_kernel(TreeView tv, int lds_l
   search_body<F, G, Q, OUT, false>(tv, lds_levels, q, pos, out_vals, out_found, n, dn);
 }
 
+// Wide groups (G = 16, 32, 64 lanes per query): whole-node ranks (descend_wide_q); at the leaf, lanes
+// 0..7 of the group hold its 8 keys and values, the rest compare padding.
+template <int F, int G, int Q, int OUT>
+__global__ __launch_bounds__(kThreads) void search_wide_kernel(TreeView tv, int lds_levels,
+                                                               const uint64_t* __restrict__ q,
+                                                               const uint32_t* __restrict__ pos,
+                                                               uint64_t* __restrict__ out_vals,
+                                                               uint8_t* __restrict__ out_found, uint32_t n,
+                                                               const uint32_t* __restrict__ dn) {
+  extern __shared__ __align__(16) uint64_t smem[];
+  stage_top_levels<F>(tv, lds_levels, smem);
+  const int gl = threadIdx.x % G;
+  const int gbase = (threadIdx.x & 63) - gl;
+  const uint32_t gid = threadIdx.x / G;
+  constexpr uint32_t gpb = kThreads / G;
+  constexpr uint32_t tile = gpb * Q;
+  if (dn) n = *dn;
+  uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  chunk = (chunk + tile - 1) / tile * tile;
+  const uint32_t vb = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t begin = vb * chunk;
+  const uint32_t end = begin + chunk < n ? begin + chunk : n;
+  for (uint32_t qb = begin; qb < end; qb += tile) {  // uniform across the block
+    uint64_t t[Q], td[Q];
+    uint32_t leaf[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const uint32_t qi = qb + i * gpb + gid;
+      t[i] = q[qi < end ? qi : end - 1];
+      td[i] = t[i] == KEY_MAX ? KEY_MAX - 1 : t[i];
+    }
+    descend_wide_q<F, G, Q>(tv, lds_levels, smem, td, leaf, gl, gbase);
+    uint64_t k[Q], v[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      k[i] = gl < LEAF_KEYS ? tv.leaves[leaf[i]].key[gl] : KEY_MAX;
+      v[i] = gl < LEAF_KEYS ? tv.leaves[leaf[i]].val[gl] : 0ull;
+    }
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      const uint32_t qi = qb + i * gpb + gid;
+      const uint64_t kk[1] = {k[i]};
+      const int lt = group_count_lt<G, 1>(kk, t[i], gbase);
+      const bool eq = group_any_eq<G, 1>(kk, t[i], gbase) && t[i] != KEY_MAX;
+      if (qi < end && gl == (eq ? lt : 0)) {
+        const uint64_t val = eq ? v[i] : 0ull;
+        const uint32_t o = OUT == OUT_DIRECT ? qi : pos[qi];
+        out_vals[o] = val;
+        out_found[o] = eq ? 1 : 0;
+      }
+    }
+  }
+}
+
 // PSA-sorted queries, two lanes per query, four queries per group (the shape it was measured faster
 // on; G = 4 and Q = 2 lost).  Five waves per SIMD as the plain kernel of that shape: without the
 // hint the shared levels cost four more VGPRs and a wave of occupancy.
@@ -462,6 +516,42 @@ hipError_t launch_sorted_q(const TreeView& tv, const SearchLaunch& sl, const uin
   return hipGetLastError();
 }
 
+template <int F, int G, int Q>
+hipError_t launch_wide_q(const TreeView& tv, const SearchLaunch& sl, const uint64_t* q, const uint32_t* pos,
+                         uint64_t* out_vals, uint8_t* out_found, size_t n, SearchOut mode, hipStream_t s,
+                         const uint32_t* dn) {
+  const LaunchCfg& lc = launch_cfg();
+  constexpr uint32_t gpb = kThreads / G * Q;
+  const size_t max_blocks = (size_t)lc.num_cus * (size_t)(sl.blocks_per_cu > 0 ? sl.blocks_per_cu : 8);
+  const size_t need = (n + gpb - 1) / gpb;
+  size_t nb = need < max_blocks ? need : max_blocks;
+  nb = (nb + 7) / 8 * 8;
+  const size_t lds = lds_bytes_for(tv, sl.lds_levels);
+  if (mode == OUT_DIRECT)
+    hipLaunchKernelGGL((search_wide_kernel<F, G, Q, OUT_DIRECT>), dim3((unsigned)nb), dim3(kThreads), lds, s, tv,
+                       sl.lds_levels, q, pos, out_vals, out_found, (uint32_t)n, dn);
+  else
+    hipLaunchKernelGGL((search_wide_kernel<F, G, Q, OUT_SCATTER>), dim3((unsigned)nb), dim3(kThreads), lds, s, tv,
+                       sl.lds_levels, q, pos, out_vals, out_found, (uint32_t)n, dn);
+  return hipGetLastError();
+}
+
+template <int F, int G>
+hipError_t launch_wide(const TreeView& tv, const SearchLaunch& sl, const uint64_t* q, const uint32_t* pos,
+                       uint64_t* out_vals, uint8_t* out_found, size_t n, SearchOut mode, hipStream_t s,
+                       const uint32_t* dn) {
+  if constexpr (G > F) {
+    return hipErrorInvalidValue;
+  } else {
+    switch (sl.queries_per_group) {
+      case 1: return launch_wide_q<F, G, 1>(tv, sl, q, pos, out_vals, out_found, n, mode, s, dn);
+      case 2: return launch_wide_q<F, G, 2>(tv, sl, q, pos, out_vals, out_found, n, mode, s, dn);
+      case 4: return launch_wide_q<F, G, 4>(tv, sl, q, pos, out_vals, out_found, n, mode, s, dn);
+      default: return hipErrorInvalidValue;
+    }
+  }
+}
+
 template <int F, int G>
 hipError_t launch_fg(const TreeView& tv, const SearchLaunch& sl, const uint64_t* q, const uint32_t* pos,
                      uint64_t* out_vals, uint8_t* out_found, size_t n, SearchOut mode, hipStream_t s,
@@ -496,6 +586,9 @@ hipError_t launch_f(const TreeView& tv, const SearchLaunch& sl, const uint64_t* 
     case 2: return launch_fg<F, 2>(tv, sl, q, pos, out_vals, out_found, n, mode, s, dn);
     case 4: return launch_fg<F, 4>(tv, sl, q, pos, out_vals, out_found, n, mode, s, dn);
     case 8: return launch_fg<F, 8>(tv, sl, q, pos, out_vals, out_found, n, mode, s, dn);
+    case 16: return launch_wide<F, 16>(tv, sl, q, pos, out_vals, out_found, n, mode, s, dn);
+    case 32: return launch_wide<F, 32>(tv, sl, q, pos, out_vals, out_found, n, mode, s, dn);
+    case 64: return launch_wide<F, 64>(tv, sl, q, pos, out_vals, out_found, n, mode, s, dn);
     default: return hipErrorInvalidValue;
   }
 }

@@ -76,7 +76,9 @@ int tune_tree(hm_tree* t, const uint64_t* sample, uint64_t n, uint32_t flags, hm
                         s));
   const int reps = (flags & HM_TUNE_QUICK) ? 2 : 4;
   const int maxl = std::min(3, t->height);
-  const int groups[] = {1, 2, 4, 8};
+  // 1..8 lanes rank through the index granule; 16, 32, 64 lanes rank the whole node in one step (the
+  // reference's narrowed groups, wider than its 32-lane warps allow); at most F lanes
+  const int groups[] = {1, 2, 4, 8, 16, 32, 64};
   const int qpgs[] = {1, 2, 4};
   const int bpcs[] = {4, 5, 8, 10};  // persistent grids: whole rounds of resident blocks matter
   uint32_t trials = 0;
@@ -86,6 +88,7 @@ int tune_tree(hm_tree* t, const uint64_t* sample, uint64_t n, uint32_t flags, hm
     for (int q : qpgs)
       for (int bpc : bpcs)
         for (int lds = 0; lds <= maxl; ++lds) {
+          if (g > t->F) continue;
           SearchLaunch sl;
           sl.group = g;
           sl.queries_per_group = q;
@@ -105,6 +108,7 @@ int tune_tree(hm_tree* t, const uint64_t* sample, uint64_t n, uint32_t flags, hm
       for (int g : groups)
         for (int q : qpgs)
           for (int bpc : bpcs) {
+            if (g > t->F) continue;
             SearchLaunch sl = bd.sl;
             sl.group = g;
             sl.queries_per_group = q;

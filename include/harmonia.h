@@ -43,7 +43,9 @@ typedef struct hm_params {
   uint32_t fanout;         /* inner-node fanout F: 32, 64 or 128 (H/CMakeLists.txt:55 DEFAULT_ORDER) */
   uint32_t leaf_fill;      /* keys per leaf at bulk build, 1..8 (slack = 8 - fill for inserts) */
   int32_t lds_levels;      /* top inner levels staged in LDS, 0..3; -1 = pick automatically */
-  uint32_t group_size;     /* lanes cooperating on one query: 1, 2, 4 or 8; 0 = automatic */
+  uint32_t group_size;     /* lanes cooperating on one query: 1, 2, 4, 8 (index granule, then one key
+                            * granule per node), 16, 32, 64 (the whole node in one step; at most F);
+                            * 0 = automatic */
   uint32_t psa_bits;       /* partial-sort bits for batched search (reference: 16); 0 = PSA off */
   uint32_t reserved0;
   uint64_t sort_threshold; /* PSA applies only to batches of at least this many queries */
@@ -79,7 +81,7 @@ typedef struct hm_stats {
 
 /* Per-call overrides for batched search (used by the autotuner); 0 / -1 = tree default. */
 typedef struct hm_search_opts {
-  uint32_t group_size;     /* lanes per query: 1, 2, 4, 8 */
+  uint32_t group_size;     /* lanes per query: 1, 2, 4, 8, 16, 32, 64 (at most the fanout) */
   int32_t lds_levels;      /* -1 = default */
   int32_t psa_bits;        /* -1 = default (PSA for batches >= sort_threshold), 0 = off, 1..32 = on */
   uint32_t blocks_per_cu;  /* 0 = default */
@@ -253,7 +255,7 @@ int hm_server_destroy(hm_server* s);
 /* Autotuner.  The reference fixes fanout / narrowing / PSA bits at compile time
  * (H/CMakeLists.txt:55-56, H/regular_bpt/prefix-sort.cu:288-293) and the paper sizes the thread
  * group with a static model (Harmonia.pdf p.140-141); here candidates are timed on the device with a
- * sample batch: lanes per query {1,2,4,8} x queries per group {1,2,4} x blocks per CU {4,8} x
+ * sample batch: lanes per query {1,2,...,64} x queries per group {1,2,4} x blocks per CU {4,5,8,10} x
  * LDS-resident levels for the unsorted path; PSA bits {12,16,20,24} x output mode x shape for the
  * sorted path; then the sort threshold (smallest power-of-two batch at which PSA wins).  The winners
  * become the tree's defaults. */

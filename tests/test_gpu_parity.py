@@ -78,7 +78,9 @@ def test_search_all_launch_shapes(gpu, F):
     L = oracle.build_bfs(sk, sv, F, 8)
     cv, cf = oracle.search_cfb(L, q)
     assert np.array_equal(cv, ev) and np.array_equal(cf, ef)
-    for g in (1, 2, 4, 8):
+    for g in (1, 2, 4, 8, 16, 32, 64):
+        if g > F:
+            continue
         for qpg in (1, 2, 4):
             for lds in (0, 1, 2):
                 # psa 2 leaves the batch nearly unsorted: with one lane per query the sorted-input
