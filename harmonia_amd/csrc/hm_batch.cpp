@@ -21,7 +21,10 @@ using namespace hm;
 
 namespace {
 
-constexpr uint64_t kGatherPiece = 1u << 16;  // queries per gather block
+// Queries per gather block: 8192 = 256 threads x 32, so a 2^24-query flush spreads over 2048 blocks
+// (eight per CU) instead of one 64K-query block per CU.
+constexpr uint64_t kGatherPiece = 1u << 13;
+constexpr int kGatherUnroll = 8;
 
 struct GatherItem {
   const uint64_t* src;
@@ -29,10 +32,24 @@ struct GatherItem {
   uint64_t n;    // <= kGatherPiece
 };
 
+// One block per item; each thread issues kGatherUnroll loads before their stores, so a wave keeps
+// eight 512-byte loads in flight.
 __global__ __launch_bounds__(256) void gather_kernel(const GatherItem* __restrict__ items, uint64_t* __restrict__ dst) {
   const GatherItem it = items[blockIdx.x];
   uint64_t* d = dst + it.dst;
-  for (uint64_t i = threadIdx.x; i < it.n; i += blockDim.x) d[i] = it.src[i];
+  for (uint64_t i0 = 0; i0 < it.n; i0 += 256 * kGatherUnroll) {
+    uint64_t v[kGatherUnroll];
+#pragma unroll
+    for (int r = 0; r < kGatherUnroll; ++r) {
+      const uint64_t i = i0 + (uint64_t)r * 256 + threadIdx.x;
+      v[r] = i < it.n ? it.src[i] : 0ull;
+    }
+#pragma unroll
+    for (int r = 0; r < kGatherUnroll; ++r) {
+      const uint64_t i = i0 + (uint64_t)r * 256 + threadIdx.x;
+      if (i < it.n) d[i] = v[r];
+    }
+  }
 }
 
 struct Graph {

@@ -84,11 +84,30 @@ int tune_tree(hm_tree* t, const uint64_t* sample, uint64_t n, uint32_t flags, hm
   uint32_t trials = 0;
   Best bd, bp;
   int rc = 0;
+  // Wide groups cost G/8 times the bytes of a level and are rarely competitive: before their grid, one
+  // probe (16 lanes, 4 queries, 8 blocks per CU) must come within 1.5x of the best narrow shape.
+  auto wide_probe = [&](int bits, int mode, const Best& best, bool* ok) {
+    *ok = false;
+    if (t->F < 16) return HM_OK;
+    SearchLaunch sl = best.sl;
+    sl.group = 16;
+    sl.queries_per_group = 4;
+    sl.blocks_per_cu = 8;
+    float ms = 0;
+    const int r = time_pipeline(t, sl, bits, mode, b, n, s, reps, &ms);
+    if (r) return r;
+    ++trials;
+    *ok = ms < 1.5f * best.ms;
+    return HM_OK;
+  };
+  bool wide = false;
   for (int g : groups)
     for (int q : qpgs)
       for (int bpc : bpcs)
         for (int lds = 0; lds <= maxl; ++lds) {
           if (g > t->F) continue;
+          if (g == 16 && q == qpgs[0] && bpc == bpcs[0] && lds == 0 && (rc = wide_probe(0, 0, bd, &wide))) return rc;
+          if (g >= 16 && !wide) continue;
           SearchLaunch sl;
           sl.group = g;
           sl.queries_per_group = q;
@@ -109,6 +128,8 @@ int tune_tree(hm_tree* t, const uint64_t* sample, uint64_t n, uint32_t flags, hm
         for (int q : qpgs)
           for (int bpc : bpcs) {
             if (g > t->F) continue;
+            if (g == 16 && q == qpgs[0] && bpc == bpcs[0] && (rc = wide_probe(bits, mode, bp, &wide))) return rc;
+            if (g >= 16 && !wide) continue;
             SearchLaunch sl = bd.sl;
             sl.group = g;
             sl.queries_per_group = q;

@@ -257,7 +257,8 @@ int hm_server_destroy(hm_server* s);
  * group with a static model (Harmonia.pdf p.140-141); here candidates are timed on the device with a
  * sample batch: lanes per query {1,2,...,64} x queries per group {1,2,4} x blocks per CU {4,5,8,10} x
  * LDS-resident levels for the unsorted path; PSA bits {12,16,20,24} x output mode x shape for the
- * sorted path; then the sort threshold (smallest power-of-two batch at which PSA wins).  The winners
+ * sorted path; then the sort threshold (the smallest batch on a sqrt(2) grid ending at the sample size
+ * at which PSA wins).  The winners
  * become the tree's defaults. */
 #define HM_TUNE_QUICK 1u /* fewer timing repetitions */
 typedef struct hm_tune_result {
