@@ -244,23 +244,24 @@ def batched_uneven(dev, n_batches=10000, keys_log2=25, seed=4):
              W.zipf_queries(sk, pool_n, cseed + 2),
              W.point_queries(sk, pool_n, cseed + 3, order="runs4096")]
     miss = [W.gen_misses(len(sk), n_miss, cseed + 1), None, W.gen_misses(len(sk), n_miss, cseed + 3)]
-    # a sample as large as the batcher's buffer, so the tuner sees the batch size a flush runs at
-    sample = np.concatenate([p[: (1 << 24) // 3] for p in pools])
-    W.rng(cseed, 7).shuffle(sample)
+    g = W.rng(cseed, 8)
+    sizes = W.log_uniform_sizes(n_batches, 32, 1 << 20, cseed).astype(np.uint64)
+    dist = g.integers(0, 3, n_batches)
+    starts = np.array([int(g.integers(0, pool_n - int(m) + 1)) for m in sizes], dtype=np.uint64)
+    cap = 1 << 24
+    # the tuner's sample is the first flush exactly as the batcher packs it (whole batches back to back,
+    # each with its own order and skew), so it sees the batch size and locality a flush runs at
+    fit = int(np.searchsorted(np.cumsum(sizes), cap, side="right"))
+    sample = np.concatenate([pools[dist[j]][int(starts[j]):int(starts[j] + sizes[j])] for j in range(fit)])
     t0 = time.perf_counter()
     tree, tune = BPlusTree.from_keys_tuned(keys, vals, sample, quick=True)
     tune_s = time.perf_counter() - t0
     log("autotune:", json.dumps(tune), f"{tune_s:.1f} s")
     del sample, keys, vals
     dpools = [torch.from_numpy(p.view(np.int64)).to(dev) for p in pools]
-    g = W.rng(cseed, 8)
-    sizes = W.log_uniform_sizes(n_batches, 32, 1 << 20, cseed).astype(np.uint64)
-    dist = g.integers(0, 3, n_batches)
-    starts = np.array([int(g.integers(0, pool_n - int(m) + 1)) for m in sizes], dtype=np.uint64)
     base = np.array([p.data_ptr() for p in dpools], dtype=np.uint64)
     ptrs = base[dist] + starts * np.uint64(8)
     total = int(sizes.sum())
-    cap = 1 << 24
     stream = torch.cuda.Stream()
     ov = torch.empty(1 << 20, dtype=torch.int64, device=dev)
     of = torch.empty(1 << 20, dtype=torch.uint8, device=dev)
@@ -310,6 +311,44 @@ def batched_uneven(dev, n_batches=10000, keys_log2=25, seed=4):
     if mismatches:
         log("UNEVEN-BATCH RESULT CHECK FAILED", mismatches)
         sys.exit(1)
+    return res
+
+
+def host_streaming(tree, q, reps=3):
+    """Queries and results in host memory (the reference's host-resident bucket loop,
+    H/regular_bpt/prefix-sort.cu:261-309): hm_search streams a batch above 2^22 queries through the
+    device in 2^22-query chunks on two streams, so the PCIe copies of one chunk overlap the other's
+    kernels.  Pinned (torch pin_memory) and pageable (numpy) caller buffers; wall time of the whole call,
+    median of `reps` after one warm-up; every result checked against the closed form."""
+    import torch
+
+    from harmonia_amd import workloads as W
+
+    n = len(q)
+    hq = torch.from_numpy(q.view(np.int64)).pin_memory()
+    hv = torch.empty(n, dtype=torch.int64).pin_memory()
+    hf = torch.empty(n, dtype=torch.uint8).pin_memory()
+    pv = np.empty(n, dtype=np.uint64)
+    pf = np.empty(n, dtype=np.uint8)
+    runs = {"pinned": (lambda: tree.search_batch(hq, out_vals=hv, out_found=hf),
+                       lambda: (hv.numpy().view(np.uint64), hf.numpy())),
+            "pageable": (lambda: tree.search_batch(q, out_vals=pv, out_found=pf), lambda: (pv, pf))}
+    res = {"workload": f"{n} queries and their results in host memory (2^26-key tree as above); chunks of "
+                       "2^22 queries on two streams; PCIe bytes = 8 B in + 9 B out per query"}
+    for name, (fn, get) in runs.items():
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        sec = float(np.median(ts))
+        v, f = get()
+        hit = f.astype(bool)
+        ok = (int(hit.sum()) == round(0.9 * n) and np.array_equal(v[hit], W.splitmix64_mix(q[hit]))
+              and not v[~hit].any())
+        res[name] = {"ms": round(sec * 1e3, 3), "Mq_per_s": round(n / sec / 1e6, 1),
+                     "pcie_GB_per_s": round(n * 17 / sec / 1e9, 2), "results_ok": bool(ok)}
     return res
 
 
@@ -604,6 +643,7 @@ def main():
     ap.add_argument("--no-ablation", action="store_true", help="skip the layout / PSA / NTG ablation")
     ap.add_argument("--no-ranges", action="store_true", help="skip the fanout-128 Zipf search and range scans")
     ap.add_argument("--no-updates", action="store_true", help="skip the batched-update epochs")
+    ap.add_argument("--no-host-stream", action="store_true", help="skip the host-memory streaming runs")
     ap.add_argument("--served-keys-log2", type=int, default=27)
     ap.add_argument("--served-requests", type=int, default=20000)
     ap.add_argument("--calibrate", action="store_true",
@@ -743,6 +783,15 @@ def main():
                     "frac": round(e2e_achieved / PEAK_HBM_GBS, 4), "bytes_per_query": BYTES_PER_QUERY_PSA,
                     "what": "whole hm_search step: PSA sort + search + un-sort to input order"}
 
+    hstream = None
+    if not args.no_host_stream:
+        log("host streaming ...")
+        hstream = host_streaming(tree, q_unsorted)
+        log("host_streaming", json.dumps(hstream))
+        if not (hstream["pinned"]["results_ok"] and hstream["pageable"]["results_ok"]):
+            log("HOST STREAMING RESULT CHECK FAILED")
+            sys.exit(1)
+
     abl = None
     if not args.no_ablation:
         log("ablation ...")
@@ -794,6 +843,7 @@ def main():
         "variants": results,
         "build_ms": round(build_ms, 1),
         "tree": {"height": st.height, "n_inner": st.n_inner, "n_leaves": st.n_leaves, "lds_levels": st.lds_levels},
+        "host_streaming": hstream,
         "ablation": abl,
         "skewed_and_ranges": skewed,
         "updates": updates,

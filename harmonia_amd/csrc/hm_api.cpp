@@ -10,6 +10,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -391,6 +392,8 @@ int rebuild_inner(hm_tree* t, hipStream_t s) {
 void free_tree(hm_tree* t) {
   if (!t) return;
   if (t->own) (void)hipStreamSynchronize(t->own);
+  for (hipStream_t h : t->hs)
+    if (h) (void)hipStreamSynchronize(h);
   for (auto& kv : t->ws) {
     if (kv.second.buf) (void)hipFree(kv.second.buf);
     if (kv.second.last_use) (void)hipEventDestroy(kv.second.last_use);
@@ -409,6 +412,11 @@ void free_tree(hm_tree* t) {
   if (t->sb_prefix) (void)hipFree(t->sb_prefix);
   if (t->rl_buf) (void)hipFree(t->rl_buf);
   if (t->bl_child) (void)hipFree(t->bl_child);
+  for (int k = 0; k < 2; ++k) {
+    if (t->hs_stage[k]) (void)hipHostFree(t->hs_stage[k]);
+    if (t->hs_ev[k]) (void)hipEventDestroy(t->hs_ev[k]);
+    if (t->hs[k]) (void)hipStreamDestroy(t->hs[k]);
+  }
   if (t->publish_ev) (void)hipEventDestroy(t->publish_ev);
   if (t->na_ev) (void)hipEventDestroy(t->na_ev);
   if (t->own) (void)hipStreamDestroy(t->own);
@@ -598,6 +606,123 @@ int hm_tree_destroy(hm_tree* t) {
 }  // extern "C"
 
 namespace hm {
+// Host-resident batches above one chunk: the reference streams fixed buckets through two CUDA streams,
+// copy in -> SortPairs -> search -> copy out per bucket (H/regular_bpt/prefix-sort.cu:261-309).  Here
+// chunk c runs on stream hs[c % 2] (copy in, the whole sort -> search -> un-sort pipeline, copy out), so
+// the PCIe copies of one chunk overlap the other chunk's kernels and copies in the other direction.
+// Pinned caller memory (hipHostMalloc, torch pin_memory) is copied by DMA in place; pageable memory is
+// staged through the tree's pinned buffers by the calling thread (several threads for large copies)
+// while the device works on the other chunk.
+constexpr uint64_t kStreamChunk = 1ull << 22;
+
+bool is_pinned_host(const void* p) {
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return a.type == hipMemoryTypeHost;
+}
+
+void par_copy(void* dst, const void* src, size_t bytes) {
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const size_t nt = std::min<size_t>(std::min(8u, hw), bytes >> 22);  // >= 4 MiB per thread
+  if (nt <= 1) {
+    std::memcpy(dst, src, bytes);
+    return;
+  }
+  const size_t per = (bytes + nt - 1) / nt;
+  std::vector<std::thread> th;
+  for (size_t i = 1; i < nt; ++i) {
+    const size_t o = i * per;
+    if (o >= bytes) break;
+    th.emplace_back([=] { std::memcpy((char*)dst + o, (const char*)src + o, std::min(per, bytes - o)); });
+  }
+  std::memcpy(dst, src, std::min(per, bytes));
+  for (auto& x : th) x.join();
+}
+
+int search_host_stream(hm_tree* t, const uint64_t* q, uint64_t n, uint64_t* out_vals, uint8_t* out_found,
+                       const hm_search_opts* o, hipStream_t s) {
+  const uint64_t C = kStreamChunk;
+  SearchLaunch sl;
+  int bits = 0, mode = 0;
+  int rc = resolve_search(t, C, o, &sl, &bits, &mode);
+  if (rc) return rc;
+  const bool pinned = is_pinned_host(q) && is_pinned_host(out_vals) && is_pinned_host(out_found);
+  for (int k = 0; k < 2; ++k) {
+    if (!t->hs[k]) HM_TRY(hipStreamCreateWithFlags(&t->hs[k], hipStreamNonBlocking));
+    if (!t->hs_ev[k]) HM_TRY(hipEventCreateWithFlags(&t->hs_ev[k], hipEventDisableTiming));
+  }
+  if (!pinned && t->hs_stage_n < C) {
+    for (int k = 0; k < 2; ++k) {
+      if (t->hs_stage[k]) HM_TRY(hipHostFree(t->hs_stage[k]));  // every earlier use was synchronized
+      t->hs_stage[k] = nullptr;
+    }
+    t->hs_stage_n = 0;
+    for (int k = 0; k < 2; ++k)
+      if ((rc = host_alloc_mapped(&t->hs_stage[k], C * 17))) return rc;
+    t->hs_stage_n = C;
+  }
+  // device buffers per stream: queries, values, found flags, then the pipeline's scratch
+  const size_t kq = align256(C * 8), kf = align256(C);
+  Workspace* w[2] = {nullptr, nullptr};
+  for (int k = 0; k < 2; ++k)
+    if ((rc = ensure_ws(t, t->hs[k], 2 * kq + kf + pipe_bytes(C, bits, mode), &w[k]))) return rc;
+  // after the caller's earlier work on s and the latest publish
+  Workspace* ws_s = nullptr;
+  if ((rc = ensure_ws(t, s, 0, &ws_s))) return rc;
+  HM_TRY(hipEventRecord(ws_s->last_use, s));
+  for (int k = 0; k < 2; ++k) {
+    HM_TRY(hipStreamWaitEvent(t->hs[k], ws_s->last_use, 0));
+    if (t->publish_valid) HM_TRY(hipStreamWaitEvent(t->hs[k], t->publish_ev, 0));
+  }
+  const TreeView tv = t->view();
+  const uint64_t chunks = (n + C - 1) / C;
+  auto drain = [&](uint64_t c) -> int {  // pageable: results of chunk c from staging to the caller
+    const int k = (int)(c & 1);
+    const uint64_t off = c * C, m = std::min(C, n - off);
+    HM_TRY(hipEventSynchronize(t->hs_ev[k]));
+    const char* st = (const char*)t->hs_stage[k];
+    par_copy(out_vals + off, st + C * 8, m * 8);
+    par_copy(out_found + off, st + C * 16, m);
+    return HM_OK;
+  };
+  for (uint64_t c = 0; c < chunks; ++c) {
+    const int k = (int)(c & 1);
+    const hipStream_t hk = t->hs[k];
+    const uint64_t off = c * C, m = std::min(C, n - off);
+    char* cur = (char*)w[k]->buf;
+    uint64_t* dq = (uint64_t*)cur;
+    uint64_t* dv = (uint64_t*)(cur + kq);
+    uint8_t* df = (uint8_t*)(cur + 2 * kq);
+    cur += 2 * kq + kf;
+    PipeBufs pb = carve_pipe(cur, C, bits, mode);
+    const void* src_q = q + off;
+    void* dst_v = out_vals + off;
+    void* dst_f = out_found + off;
+    if (!pinned) {
+      if (c >= 2 && (rc = drain(c - 2))) return rc;  // also frees staging slot k
+      char* st = (char*)t->hs_stage[k];
+      par_copy(st, q + off, m * 8);
+      src_q = st;
+      dst_v = st + C * 8;
+      dst_f = st + C * 16;
+    }
+    HM_TRY(hipMemcpyAsync(dq, src_q, m * 8, hipMemcpyHostToDevice, hk));
+    if ((rc = enqueue_pipeline(tv, sl, bits, mode, dq, m, nullptr, pb, dv, df, nullptr, hk))) return rc;
+    HM_TRY(hipMemcpyAsync(dst_v, dv, m * 8, hipMemcpyDeviceToHost, hk));
+    HM_TRY(hipMemcpyAsync(dst_f, df, m, hipMemcpyDeviceToHost, hk));
+    HM_TRY(hipEventRecord(t->hs_ev[k], hk));
+    HM_TRY(hipEventRecord(w[k]->last_use, hk));
+  }
+  if (!pinned)
+    for (uint64_t c = chunks > 2 ? chunks - 2 : 0; c < chunks; ++c)
+      if ((rc = drain(c))) return rc;
+  for (int k = 0; k < 2; ++k) HM_TRY(hipStreamSynchronize(t->hs[k]));
+  return HM_OK;
+}
+
 int search_impl(hm_tree* t, const uint64_t* q, uint64_t n, uint64_t* out_vals, uint8_t* out_found,
                 const hm_search_opts* o, hipStream_t s, bool mapped) {
   if (!t) return set_err(HM_EINVAL, "tree is NULL");
@@ -613,6 +738,7 @@ int search_impl(hm_tree* t, const uint64_t* q, uint64_t n, uint64_t* out_vals, u
   const bool psa_order = out_mode == HM_OUT_PSA_ORDER;
   const bool hq = !mapped && !is_device_ptr(q), hv = !mapped && !is_device_ptr(out_vals),
              hf = !mapped && !is_device_ptr(out_found);
+  if (hq && hv && hf && !psa_order && n > kStreamChunk) return search_host_stream(t, q, n, out_vals, out_found, o, s);
   const bool hp = psa_order && !mapped && !is_device_ptr(o->out_pos);
   const size_t kq = align256(n * 8), kp = align256(n * 4);
   size_t need = pipe_bytes(n, bits, out_mode);

@@ -7,8 +7,9 @@
 // the capacity; small flushes skip PSA (the tree's sort threshold), large ones sort first.
 //
 // Adding a batch costs no device call: a device batch is recorded (pointer, count, slot) and a host
-// batch is copied into pinned staging.  The flush packs everything with one H2D copy of the staged
-// host part and one gather kernel over the recorded device batches, then replays the graph.
+// batch is copied into pinned staging.  The flush copies the staged host part with one copy kernel (if
+// there is one) and replays the graph, whose first kernel gathers the recorded device batches and
+// stores the packed count that the pipeline's kernels read: one graph launch per flush.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -32,22 +33,38 @@ struct GatherItem {
   uint64_t n;    // <= kGatherPiece
 };
 
-// One block per item; each thread issues kGatherUnroll loads before their stores, so a wave keeps
-// eight 512-byte loads in flight.
-__global__ __launch_bounds__(256) void gather_kernel(const GatherItem* __restrict__ items, uint64_t* __restrict__ dst) {
-  const GatherItem it = items[blockIdx.x];
-  uint64_t* d = dst + it.dst;
-  for (uint64_t i0 = 0; i0 < it.n; i0 += 256 * kGatherUnroll) {
-    uint64_t v[kGatherUnroll];
+// The gather list of one flush, in mapped pinned memory written by the host before the launch: the
+// graph of list slot k captures the address of header k, the header points at the items.
+struct GatherHdr {
+  const GatherItem* items;
+  uint64_t n_items;
+  uint64_t total;  // packed query count of the flush
+};
+
+constexpr int kGatherBlocks = 2048;  // fixed grid (captured); blocks stride over the items
+
+// Items b, b + grid, ...: each thread issues kGatherUnroll loads before their stores, so a wave keeps
+// eight 512-byte loads in flight.  Block 0 stores the flush's count for the kernels after it.
+__global__ __launch_bounds__(256) void gather_kernel(const GatherHdr* __restrict__ hdr, uint64_t* __restrict__ dst,
+                                                     uint32_t* __restrict__ d_n) {
+  const uint64_t m = hdr->n_items;
+  const GatherItem* items = hdr->items;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *d_n = (uint32_t)hdr->total;
+  for (uint64_t j = blockIdx.x; j < m; j += gridDim.x) {
+    const GatherItem it = items[j];
+    uint64_t* d = dst + it.dst;
+    for (uint64_t i0 = 0; i0 < it.n; i0 += 256 * kGatherUnroll) {
+      uint64_t v[kGatherUnroll];
 #pragma unroll
-    for (int r = 0; r < kGatherUnroll; ++r) {
-      const uint64_t i = i0 + (uint64_t)r * 256 + threadIdx.x;
-      v[r] = i < it.n ? it.src[i] : 0ull;
-    }
+      for (int r = 0; r < kGatherUnroll; ++r) {
+        const uint64_t i = i0 + (uint64_t)r * 256 + threadIdx.x;
+        v[r] = i < it.n ? it.src[i] : 0ull;
+      }
 #pragma unroll
-    for (int r = 0; r < kGatherUnroll; ++r) {
-      const uint64_t i = i0 + (uint64_t)r * 256 + threadIdx.x;
-      if (i < it.n) d[i] = v[r];
+      for (int r = 0; r < kGatherUnroll; ++r) {
+        const uint64_t i = i0 + (uint64_t)r * 256 + threadIdx.x;
+        if (i < it.n) d[i] = v[r];
+      }
     }
   }
 }
@@ -79,7 +96,7 @@ struct hm_batcher {
   uint8_t* d_found = nullptr;
   uint32_t* d_n = nullptr;
   void* scratch = nullptr;
-  Graph g_psa, g_direct;
+  Graph g_psa[2], g_direct[2];  // per gather-list slot
   uint64_t fill = 0;
   // batches added since the last flush
   std::vector<GatherItem> items;     // device batches, split into pieces of at most kGatherPiece
@@ -88,7 +105,8 @@ struct hm_batcher {
   hipEvent_t stage_ev = nullptr;     // the last upload of h_stage (host adds wait for it)
   bool stage_busy = false;
   // gather lists: two mapped pinned buffers that the gather kernel reads in place, used alternately,
-  // each reused once its gather is done
+  // each reused once its flush is done; h_hdr[k] (mapped) describes list k
+  GatherHdr* h_hdr = nullptr;
   GatherItem* h_items[2] = {nullptr, nullptr};
   size_t items_cap[2] = {0, 0};
   hipEvent_t items_ev[2] = {nullptr, nullptr};
@@ -101,11 +119,13 @@ namespace {
 void free_batcher(hm_batcher* b) {
   if (!b) return;
   if (b->s) (void)hipStreamSynchronize(b->s);
-  b->g_psa.reset();
-  b->g_direct.reset();
+  for (int k = 0; k < 2; ++k) {
+    b->g_psa[k].reset();
+    b->g_direct[k].reset();
+  }
   for (void* p : {(void*)b->d_q, (void*)b->d_vals, (void*)b->d_found, (void*)b->d_n, b->scratch})
     if (p) (void)hipFree(p);
-  for (void* p : {(void*)b->h_stage, (void*)b->h_items[0], (void*)b->h_items[1]})
+  for (void* p : {(void*)b->h_stage, (void*)b->h_items[0], (void*)b->h_items[1], (void*)b->h_hdr})
     if (p) (void)hipHostFree(p);
   for (hipEvent_t e : {b->stage_ev, b->items_ev[0], b->items_ev[1]})
     if (e) (void)hipEventDestroy(e);
@@ -113,8 +133,8 @@ void free_batcher(hm_batcher* b) {
   delete b;
 }
 
-// Capture the pipeline for `psa` (true: tree's PSA path at full capacity) into g.
-int capture(hm_batcher* b, bool psa, Graph& g) {
+// Capture gather (list slot k) + the pipeline for `psa` (true: tree's PSA path at full capacity) into g.
+int capture(hm_batcher* b, bool psa, int k, Graph& g) {
   hm_tree* t = b->t;
   hm_search_opts o = b->have_opts ? b->opts : hm_search_opts{0, -1, -1, 0, 0, 0, nullptr, 0, 0};
   if (!psa) o.psa_bits = 0;
@@ -129,8 +149,11 @@ int capture(hm_batcher* b, bool psa, Graph& g) {
   PipeBufs pb = carve_pipe(cur, b->cap, bits, mode);
   g.reset();
   HM_TRY(hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
-  rc = enqueue_pipeline(t->view(), sl, bits, mode, b->d_q, b->cap, b->d_n, pb, b->d_vals, b->d_found, nullptr,
-                        b->s);
+  hipLaunchKernelGGL(gather_kernel, dim3(kGatherBlocks), dim3(256), 0, b->s, (const GatherHdr*)(b->h_hdr + k), b->d_q,
+                     b->d_n);
+  rc = hipGetLastError() == hipSuccess ? HM_OK : set_err(HM_EDEVICE, "gather launch failed");
+  if (!rc)
+    rc = enqueue_pipeline(t->view(), sl, bits, mode, b->d_q, b->cap, b->d_n, pb, b->d_vals, b->d_found, nullptr, b->s);
   hipGraph_t graph = nullptr;
   hipError_t e = hipStreamEndCapture(b->s, &graph);
   if (rc) {
@@ -182,6 +205,10 @@ int hm_batcher_create(hm_tree* t, uint64_t capacity, const hm_search_opts* opts,
       hipMalloc(&b->d_found, capacity) != hipSuccess || hipMalloc(&b->d_n, 256) != hipSuccess ||
       hipMalloc(&b->scratch, need) != hipSuccess)
     return fail(set_err(HM_ENOMEM, "batcher allocation failed"));
+  if (int rc = host_alloc_mapped((void**)&b->h_hdr, 2 * sizeof(GatherHdr))) return fail(rc);
+  for (int k = 0; k < 2; ++k)
+    if (hipEventCreateWithFlags(&b->items_ev[k], hipEventDisableTiming) != hipSuccess)
+      return fail(set_err(HM_EDEVICE, "hipEventCreate failed"));
   *out = b;
   return HM_OK;
 }
@@ -243,38 +270,31 @@ int hm_batcher_add_many(hm_batcher* b, const uint64_t* const* qs, const uint64_t
 }
 
 namespace {
-// Pack the added batches into d_q (stream order on b->s).
-int pack(hm_batcher* b) {
-  if (b->host_hi > b->host_lo) {  // host batches first: device batches inside the range are re-gathered
+// Copy the staged host batches into d_q (stream order on b->s) and fill gather-list slot k with the
+// device batches for the graph's gather kernel (after that slot's previous flush has finished).
+int pack(hm_batcher* b, int k) {
+  if (b->host_hi > b->host_lo) {  // host batches first: the gather in the graph writes only device batches
     // a copy kernel reads the mapped staging buffer: no DMA-engine copy in the flush
     HM_TRY(copy_words(b->d_q + b->host_lo, b->h_stage + b->host_lo, b->host_hi - b->host_lo, b->s));
     HM_TRY(hipEventRecord(b->stage_ev, b->s));
     b->stage_busy = true;
   }
-  const size_t m = b->items.size();
-  if (m) {
-    const int k = b->slot;
-    b->slot ^= 1;
-    if (b->items_busy[k]) {  // this list was used two flushes ago
-      HM_TRY(hipEventSynchronize(b->items_ev[k]));
-      b->items_busy[k] = false;
-    }
-    if (m > b->items_cap[k]) {
-      if (b->h_items[k]) HM_TRY(hipHostFree(b->h_items[k]));  // its last use has completed (event above)
-      b->h_items[k] = nullptr;
-      b->items_cap[k] = 0;
-      const size_t c = std::max<size_t>(2 * m, 1024);
-      int rc = host_alloc_mapped((void**)&b->h_items[k], c * sizeof(GatherItem));
-      if (rc) return rc;
-      if (!b->items_ev[k]) HM_TRY(hipEventCreateWithFlags(&b->items_ev[k], hipEventDisableTiming));
-      b->items_cap[k] = c;
-    }
-    std::memcpy(b->h_items[k], b->items.data(), m * sizeof(GatherItem));
-    hipLaunchKernelGGL(gather_kernel, dim3((unsigned)m), dim3(256), 0, b->s, (const GatherItem*)b->h_items[k], b->d_q);
-    HM_TRY(hipGetLastError());
-    HM_TRY(hipEventRecord(b->items_ev[k], b->s));
-    b->items_busy[k] = true;
+  if (b->items_busy[k]) {  // slot k was last used two flushes ago
+    HM_TRY(hipEventSynchronize(b->items_ev[k]));
+    b->items_busy[k] = false;
   }
+  const size_t m = b->items.size();
+  if (m > b->items_cap[k]) {
+    if (b->h_items[k]) HM_TRY(hipHostFree(b->h_items[k]));  // its last use has completed (event above)
+    b->h_items[k] = nullptr;
+    b->items_cap[k] = 0;
+    const size_t c = std::max<size_t>(2 * m, 1024);
+    int rc = host_alloc_mapped((void**)&b->h_items[k], c * sizeof(GatherItem));
+    if (rc) return rc;
+    b->items_cap[k] = c;
+  }
+  if (m) std::memcpy(b->h_items[k], b->items.data(), m * sizeof(GatherItem));
+  b->h_hdr[k] = GatherHdr{b->h_items[k], (uint64_t)m, b->fill};
   b->items.clear();
   b->host_lo = ~0ull;
   b->host_hi = 0;
@@ -289,22 +309,25 @@ int hm_batcher_flush(hm_batcher* b) {
   std::lock_guard<std::mutex> g(t->mu);
   bool psa = b->have_opts && b->opts.psa_bits >= 0 ? b->opts.psa_bits > 0
                                                    : (t->p.psa_bits > 0 && b->fill >= t->p.sort_threshold);
-  Graph& gr = psa ? b->g_psa : b->g_direct;
+  const int k = b->slot;
+  Graph& gr = psa ? b->g_psa[k] : b->g_direct[k];
   if (!gr.exec || gr.version != t->layout_version) {
-    int rc = capture(b, psa, gr);
+    int rc = capture(b, psa, k, gr);
     if (rc) return rc;
   }
   Workspace* w = nullptr;
   int rc = ensure_ws(t, b->s, 0, &w);  // registers the stream so epoch publishes wait for it
   if (rc) return rc;
   if (t->publish_valid) HM_TRY(hipStreamWaitEvent(b->s, t->publish_ev, 0));
-  rc = pack(b);
+  rc = pack(b, k);
   if (rc) return rc;
-  HM_TRY(launch_set_u32(b->d_n, (uint32_t)b->fill, b->s));
   {
     ProfScope prof("batcher_graph", b->s);
     HM_TRY(hipGraphLaunch(gr.exec, b->s));
   }
+  HM_TRY(hipEventRecord(b->items_ev[k], b->s));
+  b->items_busy[k] = true;
+  b->slot = k ^ 1;
   HM_TRY(hipEventRecord(w->last_use, b->s));
   b->fill = 0;
   return HM_OK;

@@ -66,6 +66,12 @@ struct hm_tree {
   std::mutex mu;
   std::unordered_map<hipStream_t, hm::Workspace> ws;
   hipStream_t own = nullptr;
+  // Host-resident batches larger than one chunk stream through two streams of the tree's own (default
+  // priority), one chunk in flight on each; pageable caller memory goes through pinned staging.
+  hipStream_t hs[2] = {nullptr, nullptr};
+  hipEvent_t hs_ev[2] = {nullptr, nullptr};   // the latest chunk on hs[k] completed
+  void* hs_stage[2] = {nullptr, nullptr};     // mapped pinned: chunk queries (8 B), values (8 B), found (1 B)
+  size_t hs_stage_n = 0;                      // queries per staging slot
   hipEvent_t publish_ev = nullptr;
   bool publish_valid = false;
   // the latest allocation-free search (hm_search_noalloc), so a publish waits for its reads; created

@@ -115,7 +115,12 @@ int hm_tree_destroy(hm_tree* t);
 /* Batched point search, results in input order: out_found[i] = 1 and out_vals[i] = value when
  * q[i] is present, else out_found[i] = 0 and out_vals[i] = 0.
  * Replaces multi_prefix_sort / full_multi_prefix_sort_8thread (H/regular_bpt/prefix-sort.cu:311,
- * H/src/full-multi-prefix-sort-8thread.cu:329) and BPlusTree::search (H/src/bpt.cpp:673-680). */
+ * H/src/full-multi-prefix-sort-8thread.cu:329) and BPlusTree::search (H/src/bpt.cpp:673-680).
+ * Buffers may be device or host memory.  With host buffers the call returns when the results are in
+ * them; when q, out_vals and out_found are all host memory and n > 2^22, the batch is streamed through
+ * the device in 2^22-query chunks on two streams of the tree's own, copies of one chunk overlapping the
+ * other's kernels (the reference's two-stream bucket loop, H/regular_bpt/prefix-sort.cu:261-309);
+ * pinned memory (hipHostMalloc) is copied in place, pageable memory through pinned staging. */
 int hm_search(hm_tree* t, const uint64_t* q, uint64_t n, uint64_t* out_vals, uint8_t* out_found, void* stream);
 int hm_search_ex(hm_tree* t, const uint64_t* q, uint64_t n, uint64_t* out_vals, uint8_t* out_found,
                  const hm_search_opts* opts, void* stream);
