@@ -8,8 +8,9 @@
 //
 // Adding a batch costs no device call: a device batch is recorded (pointer, count, slot) and a host
 // batch is copied into pinned staging.  The flush copies the staged host part with one copy kernel (if
-// there is one) and replays the graph, whose first kernel gathers the recorded device batches and
-// stores the packed count that the pipeline's kernels read: one graph launch per flush.
+// there is one) and replays the graph: on the direct path the search kernel reads every batch in place
+// from the segment list; on the PSA path a first kernel gathers the batches into the packed buffer and
+// stores the count that the sort and search read.  One graph launch per flush either way.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -22,29 +23,19 @@ using namespace hm;
 
 namespace {
 
-// Queries per gather block: 8192 = 256 threads x 32, so a 2^24-query flush spreads over 2048 blocks
-// (eight per CU) instead of one 64K-query block per CU.
-constexpr uint64_t kGatherPiece = 1u << 13;
+// The batches of one flush as a segment list (QuerySeg: source, slot, count), cut into pieces of
+// kSegPiece = 8192 queries (256 threads x 32), in mapped pinned memory written by the host before the
+// launch: the graph of list slot k captures the address of header k, the header points at the items.
+// Host batches are copied into the packed buffer first and listed as segments of that buffer.
+using GatherItem = QuerySeg;
+using GatherHdr = QuerySegList;
+constexpr uint64_t kGatherPiece = kSegPiece;
 constexpr int kGatherUnroll = 8;
-
-struct GatherItem {
-  const uint64_t* src;
-  uint64_t dst;  // slot in the batcher buffer
-  uint64_t n;    // <= kGatherPiece
-};
-
-// The gather list of one flush, in mapped pinned memory written by the host before the launch: the
-// graph of list slot k captures the address of header k, the header points at the items.
-struct GatherHdr {
-  const GatherItem* items;
-  uint64_t n_items;
-  uint64_t total;  // packed query count of the flush
-};
-
 constexpr int kGatherBlocks = 2048;  // fixed grid (captured); blocks stride over the items
 
-// Items b, b + grid, ...: each thread issues kGatherUnroll loads before their stores, so a wave keeps
-// eight 512-byte loads in flight.  Block 0 stores the flush's count for the kernels after it.
+// The PSA path's packing: items b, b + grid, ... copied into the packed buffer (each thread issues
+// kGatherUnroll loads before their stores, so a wave keeps eight 512-byte loads in flight).  Block 0
+// stores the flush's count for the kernels after it.  (The direct path reads the segments in place.)
 __global__ __launch_bounds__(256) void gather_kernel(const GatherHdr* __restrict__ hdr, uint64_t* __restrict__ dst,
                                                      uint32_t* __restrict__ d_n) {
   const uint64_t m = hdr->n_items;
@@ -108,6 +99,7 @@ struct hm_batcher {
   // each reused once its flush is done; h_hdr[k] (mapped) describes list k
   GatherHdr* h_hdr = nullptr;
   GatherItem* h_items[2] = {nullptr, nullptr};
+  uint32_t* h_gran[2] = {nullptr, nullptr};  // first segment of each kSegGranule slots (mapped)
   size_t items_cap[2] = {0, 0};
   hipEvent_t items_ev[2] = {nullptr, nullptr};
   bool items_busy[2] = {false, false};
@@ -125,7 +117,8 @@ void free_batcher(hm_batcher* b) {
   }
   for (void* p : {(void*)b->d_q, (void*)b->d_vals, (void*)b->d_found, (void*)b->d_n, b->scratch})
     if (p) (void)hipFree(p);
-  for (void* p : {(void*)b->h_stage, (void*)b->h_items[0], (void*)b->h_items[1], (void*)b->h_hdr})
+  for (void* p : {(void*)b->h_stage, (void*)b->h_items[0], (void*)b->h_items[1], (void*)b->h_hdr, (void*)b->h_gran[0],
+                  (void*)b->h_gran[1]})
     if (p) (void)hipHostFree(p);
   for (hipEvent_t e : {b->stage_ev, b->items_ev[0], b->items_ev[1]})
     if (e) (void)hipEventDestroy(e);
@@ -147,11 +140,16 @@ int capture(hm_batcher* b, bool psa, int k, Graph& g) {
   if (mode == HM_OUT_PSA_ORDER) return set_err(HM_EINVAL, "a batcher returns results in input order");
   char* cur = (char*)b->scratch;
   PipeBufs pb = carve_pipe(cur, b->cap, bits, mode);
+  // the direct path with 1..8 lanes per query reads the segments in place; PSA (and wide groups) gather
+  const bool in_place = bits == 0 && sl.group <= 8;
+  if (in_place) sl.segs = b->h_hdr + k;
   g.reset();
   HM_TRY(hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
-  hipLaunchKernelGGL(gather_kernel, dim3(kGatherBlocks), dim3(256), 0, b->s, (const GatherHdr*)(b->h_hdr + k), b->d_q,
-                     b->d_n);
-  rc = hipGetLastError() == hipSuccess ? HM_OK : set_err(HM_EDEVICE, "gather launch failed");
+  if (!in_place) {
+    hipLaunchKernelGGL(gather_kernel, dim3(kGatherBlocks), dim3(256), 0, b->s, (const GatherHdr*)(b->h_hdr + k), b->d_q,
+                       b->d_n);
+    rc = hipGetLastError() == hipSuccess ? HM_OK : set_err(HM_EDEVICE, "gather launch failed");
+  }
   if (!rc)
     rc = enqueue_pipeline(t->view(), sl, bits, mode, b->d_q, b->cap, b->d_n, pb, b->d_vals, b->d_found, nullptr, b->s);
   hipGraph_t graph = nullptr;
@@ -207,6 +205,9 @@ int hm_batcher_create(hm_tree* t, uint64_t capacity, const hm_search_opts* opts,
     return fail(set_err(HM_ENOMEM, "batcher allocation failed"));
   if (int rc = host_alloc_mapped((void**)&b->h_hdr, 2 * sizeof(GatherHdr))) return fail(rc);
   for (int k = 0; k < 2; ++k)
+    if (int rc = host_alloc_mapped((void**)&b->h_gran[k], ((capacity >> kSegGranuleLog2) + 2) * sizeof(uint32_t)))
+      return fail(rc);
+  for (int k = 0; k < 2; ++k)
     if (hipEventCreateWithFlags(&b->items_ev[k], hipEventDisableTiming) != hipSuccess)
       return fail(set_err(HM_EDEVICE, "hipEventCreate failed"));
   *out = b;
@@ -239,6 +240,9 @@ int hm_batcher_add(hm_batcher* b, const uint64_t* q, uint64_t n, uint64_t* out_o
     std::memcpy(b->h_stage + b->fill, q, n * 8);
     b->host_lo = std::min(b->host_lo, b->fill);
     b->host_hi = b->fill + n;
+    // listed as a segment of the packed buffer, which the flush's copy kernel fills first
+    for (uint64_t o = 0; o < n; o += kGatherPiece)
+      b->items.push_back(GatherItem{b->d_q + b->fill + o, b->fill + o, std::min<uint64_t>(kGatherPiece, n - o)});
   }
   if (out_offset) *out_offset = b->fill;
   b->fill += n;
@@ -294,7 +298,16 @@ int pack(hm_batcher* b, int k) {
     b->items_cap[k] = c;
   }
   if (m) std::memcpy(b->h_items[k], b->items.data(), m * sizeof(GatherItem));
-  b->h_hdr[k] = GatherHdr{b->h_items[k], (uint64_t)m, b->fill};
+  // segments are in slot order (batches are added at the fill position): index them per granule
+  uint32_t* gf = b->h_gran[k];
+  const uint64_t ng = (b->fill + (1ull << kSegGranuleLog2) - 1) >> kSegGranuleLog2;
+  size_t j = 0;
+  for (uint64_t gi = 0; gi < ng; ++gi) {
+    const uint64_t slot = gi << kSegGranuleLog2;
+    while (j + 1 < m && b->items[j].dst + b->items[j].n <= slot) ++j;
+    gf[gi] = (uint32_t)j;
+  }
+  b->h_hdr[k] = GatherHdr{b->h_items[k], (uint64_t)m, b->fill, gf};
   b->items.clear();
   b->host_lo = ~0ull;
   b->host_hi = 0;

@@ -119,6 +119,26 @@ hipError_t build_level(const uint64_t* child_min, size_t m_child, int F, uint64_
 hipError_t fold_prefix(uint64_t* inner, const uint32_t* prefix, size_t n_inner, int F, hipStream_t s);
 
 // ---- search (hm_search.hip) ------------------------------------------------------------
+// Query segments: segment j holds n queries at src (device memory) whose results go to slots
+// [dst, dst + n) of the output; the segments are sorted by dst and tile [0, total).  The list may live
+// in mapped pinned host memory (the batcher writes it there before each graph replay).
+// gran_first[g] = the first segment that reaches slot g * kSegGranule (a search block finds the
+// segments of its chunk with one lookup).
+struct QuerySeg {
+  const uint64_t* src;
+  uint64_t dst;
+  uint64_t n;
+};
+struct QuerySegList {
+  const QuerySeg* items;
+  uint64_t n_items;
+  uint64_t total;  // sum of n
+  const uint32_t* gran_first;
+};
+// Queries per segment: the batcher cuts batches into pieces of at most this many.
+constexpr uint64_t kSegPiece = 1u << 13;
+constexpr int kSegGranuleLog2 = 12;
+
 struct SearchLaunch {
   int group = 4;        // lanes per query
   int lds_levels = 2;   // levels staged in LDS
@@ -127,6 +147,9 @@ struct SearchLaunch {
   bool sorted = false;        // queries are PSA-sorted: waves share node reads (group 1, or group 2 x 4)
   bool window = false;        // with sorted, OUT_DIRECT: the leaf-window kernel (dense batches)
   int window_blocks_per_cu = 2;
+  // unsorted OUT_DIRECT searches with 1..8 lanes per query: read the queries from these segments
+  // (blocks take whole segments) instead of the packed array
+  const QuerySegList* segs = nullptr;
 };
 // Leaves a wave's LDS ring holds in the leaf-window search (64 B of keys each).
 constexpr int kWindowLeaves = 256;

@@ -20,16 +20,14 @@ size_t sorted_lds_bytes(const TreeView& tv, int levels, int Q) {
   return (nl + (size_t)(kThreads / 64) * Q) * node_slots(tv.fanout) * sizeof(uint64_t);
 }
 
-// SHARED: the queries are PSA-sorted and the upper global levels go through shared_node_level
-// (wave buffers after the staged top levels, sized by sorted_lds_bytes).
+// Queries [begin, end) of qs; result i goes to (ov, of)[i] (or to pos[i] with OUT_SCATTER).  SHARED: the
+// queries are PSA-sorted and the upper global levels go through shared_node_level (wave buffers at
+// wslot, after the staged top levels, sized by sorted_lds_bytes).
 template <int F, int G, int Q, int OUT, bool SHARED>
-__device__ __forceinline__ void search_body(const TreeView& tv, int lds_levels, const uint64_t* __restrict__ q,
-                                            const uint32_t* __restrict__ pos, uint64_t* __restrict__ out_vals,
-                                            uint8_t* __restrict__ out_found, uint32_t n,
-                                            const uint32_t* __restrict__ dn) {
-  extern __shared__ __align__(16) uint64_t smem[];
-  stage_top_levels<F>(tv, lds_levels, smem);
-
+__device__ __forceinline__ void search_run(const TreeView& tv, int lds_levels, const uint64_t* smem, uint64_t* wslot,
+                                           const uint64_t* __restrict__ qs, const uint32_t* __restrict__ pos,
+                                           uint64_t* __restrict__ ov, uint8_t* __restrict__ of, uint32_t begin,
+                                           uint32_t end) {
   constexpr int W = 8 / G;
   // G == 1: one lane per query reads its 8 leaf keys, then only the matching value (same 128-B
   // line, an L2 hit); G > 1: the group reads keys and values of the leaf line together.
@@ -39,28 +37,13 @@ __device__ __forceinline__ void search_body(const TreeView& tv, int lds_levels, 
   const uint32_t gid = threadIdx.x / G;
   constexpr uint32_t gpb = kThreads / G;
   constexpr uint32_t tile = gpb * Q;
-  // Persistent grid: each block owns one contiguous chunk of the batch; the count may live on the
-  // device (graph replays with data-dependent batch sizes).
-  if (dn) n = *dn;
-  uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
-  chunk = (chunk + tile - 1) / tile * tile;
-  const uint32_t vb = xcd_block(blockIdx.x, gridDim.x);
-  const uint32_t begin = vb * chunk;
-  const uint32_t end = begin + chunk < n ? begin + chunk : n;
-  uint64_t* wslot = nullptr;
-  if constexpr (SHARED) {
-    constexpr int NS = node_slots(F);
-    const uint32_t nl = lds_levels > 0 ? tv.level_start[lds_levels] : 0u;
-    wslot = smem + (nl + (threadIdx.x >> 6) * Q) * NS;
-  }
-
   for (uint32_t qb = begin; qb < end; qb += tile) {  // uniform across the block
     uint64_t t[Q], td[Q];
     uint32_t leaf[Q];
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
       const uint32_t qi = qb + i * gpb + gid;
-      t[i] = q[qi < end ? qi : end - 1];  // branch-free: lanes past the end repeat the last query
+      t[i] = qs[qi < end ? qi : end - 1];  // branch-free: lanes past the end repeat the last query
       td[i] = t[i] == KEY_MAX ? KEY_MAX - 1 : t[i];
     }
     if constexpr (SHARED)
@@ -91,16 +74,41 @@ __device__ __forceinline__ void search_body(const TreeView& tv, int lds_levels, 
           if (eq) val = tv.leaves[leaf[i]].val[lt];
         }
         if constexpr (OUT == OUT_DIRECT) {
-          out_vals[qi] = val;
-          out_found[qi] = eq ? 1 : 0;
+          ov[qi] = val;
+          of[qi] = eq ? 1 : 0;
         } else {
           const uint32_t o = pos[qi];
-          out_vals[o] = val;
-          out_found[o] = eq ? 1 : 0;
+          ov[o] = val;
+          of[o] = eq ? 1 : 0;
         }
       }
     }
   }
+}
+
+// Persistent grid: each block owns one contiguous chunk of the batch; the count may live on the
+// device (graph replays with data-dependent batch sizes).
+template <int F, int G, int Q, int OUT, bool SHARED>
+__device__ __forceinline__ void search_body(const TreeView& tv, int lds_levels, const uint64_t* __restrict__ q,
+                                            const uint32_t* __restrict__ pos, uint64_t* __restrict__ out_vals,
+                                            uint8_t* __restrict__ out_found, uint32_t n,
+                                            const uint32_t* __restrict__ dn) {
+  extern __shared__ __align__(16) uint64_t smem[];
+  stage_top_levels<F>(tv, lds_levels, smem);
+  constexpr uint32_t tile = kThreads / G * Q;
+  if (dn) n = *dn;
+  uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  chunk = (chunk + tile - 1) / tile * tile;
+  const uint32_t vb = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t begin = vb * chunk;
+  const uint32_t end = begin + chunk < n ? begin + chunk : n;
+  uint64_t* wslot = nullptr;
+  if constexpr (SHARED) {
+    constexpr int NS = node_slots(F);
+    const uint32_t nl = lds_levels > 0 ? tv.level_start[lds_levels] : 0u;
+    wslot = smem + (nl + (threadIdx.x >> 6) * Q) * NS;
+  }
+  search_run<F, G, Q, OUT, SHARED>(tv, lds_levels, smem, wslot, q, pos, out_vals, out_found, begin, end);
 }
 
 template <int F, int G, int Q, int OUT>
@@ -111,6 +119,38 @@ __global__ __launch_bounds__(kThreads) void search_kernel(TreeView tv, int lds_l
                                                           uint8_t* __restrict__ out_found, uint32_t n,
                                                           const uint32_t* __restrict__ dn) {
   search_body<F, G, Q, OUT, false>(tv, lds_levels, q, pos, out_vals, out_found, n, dn);
+}
+
+// search_kernel over a segment list (unsorted queries in several device arrays, results at each
+// segment's slots), so a packed batch needs no gather copy before the search.  Blocks split [0, total)
+// into chunks exactly as search_kernel does; a block finds the first segment of its chunk through
+// gran_first and walks the segments its chunk overlaps.
+template <int F, int G, int Q>
+__global__ __launch_bounds__(kThreads) void search_seg_kernel(TreeView tv, int lds_levels,
+                                                              const QuerySegList* __restrict__ segs,
+                                                              uint64_t* __restrict__ out_vals,
+                                                              uint8_t* __restrict__ out_found) {
+  extern __shared__ __align__(16) uint64_t smem[];
+  stage_top_levels<F>(tv, lds_levels, smem);
+  constexpr uint32_t tile = kThreads / G * Q;
+  const uint32_t n = (uint32_t)segs->total;
+  const uint64_t m = segs->n_items;
+  const QuerySeg* items = segs->items;
+  uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  chunk = (chunk + tile - 1) / tile * tile;
+  const uint32_t vb = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t begin = vb * chunk;
+  const uint32_t end = begin + chunk < n ? begin + chunk : n;
+  if (begin >= end) return;  // block-uniform
+  for (uint64_t j = segs->gran_first[begin >> kSegGranuleLog2]; j < m; ++j) {  // uniform across the block
+    const QuerySeg it = items[j];
+    if (it.dst >= end) break;
+    const uint64_t lo = it.dst > begin ? it.dst : begin;
+    const uint64_t hi = it.dst + it.n < end ? it.dst + it.n : end;
+    if (lo < hi)
+      search_run<F, G, Q, OUT_DIRECT, false>(tv, lds_levels, smem, nullptr, it.src + (lo - it.dst), nullptr,
+                                             out_vals + lo, out_found + lo, 0u, (uint32_t)(hi - lo));
+  }
 }
 
 // Wide groups (G = 16, 32, 64 lanes per query): whole-node ranks (descend_wide_q); at the leaf, lanes
@@ -481,6 +521,11 @@ hipError_t launch_fgq(const TreeView& tv, const SearchLaunch& sl, const uint64_t
     }
   }
   const size_t lds = lds_bytes_for(tv, sl.lds_levels);
+  if (sl.segs && !sl.sorted && mode == OUT_DIRECT) {
+    hipLaunchKernelGGL((search_seg_kernel<F, G, Q>), dim3((unsigned)nb), dim3(kThreads), lds, s, tv, sl.lds_levels,
+                       sl.segs, out_vals, out_found);
+    return hipGetLastError();
+  }
   switch (mode) {
     case OUT_DIRECT:
       hipLaunchKernelGGL((search_kernel<F, G, Q, OUT_DIRECT>), dim3((unsigned)nb), dim3(kThreads), lds, s, tv,
