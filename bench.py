@@ -390,7 +390,8 @@ def ablation(tree, dq, out_v, out_f, stream, steps, warmup):
     res = []
     for key, what, fn in steps_cfg:
         ms, _ = time_steps(fn, steps, warmup, stream)
-        out_f.fill_(7)
+        with torch.cuda.stream(stream):  # poison the flags on the stream the search runs on (ordered before it)
+            out_f.fill_(7)
         fn()
         stream.synchronize()
         same = bool(np.array_equal(out_f.cpu().numpy(), ref_f) and np.array_equal(out_v.cpu().numpy(), ref_v))
