@@ -589,8 +589,8 @@ def batched_updates(dev, lib):
 
 def served(args):
     """Run the served-load driver (hm_serve_bench_main in harmonia_amd/libhm_servebench.so, called
-    in-process): plain path closed-loop capacity, then the serving loop open-loop at Poisson rates of
-    1, 3, 5 and 10 times that."""
+    in-process): plain path closed-loop capacity, the plain path open-loop at 0.5 and 0.9 of it (its
+    latency), then the serving loop open-loop at Poisson rates of 1, 3, 5 and 10 times that capacity."""
     import ctypes
 
     path = os.path.join(ROOT, "harmonia_amd", "libhm_servebench.so")
@@ -616,10 +616,16 @@ def served(args):
 
     plain = run(["--plain", "--closed", "--requests", str(args.served_requests // 2)])
     log("plain", json.dumps(plain))
-    res = {"plain_closed_loop": plain, "served": []}
+    res = {"plain_closed_loop": plain, "plain_open_loop": [], "served": []}
     if "error" in plain:
         return res
     base_rps = plain["requests_per_s"]
+    # the plain path's own latency, open loop below its capacity (closed loop its p50 is queueing)
+    for mult in (0.5, 0.9):
+        r = run(["--plain", "--rate", str(base_rps * mult), "--requests", str(args.served_requests // 2)])
+        r["rate_vs_plain_capacity"] = mult
+        log("plain open loop", json.dumps(r))
+        res["plain_open_loop"].append(r)
     for mult in (1.0, 3.0, 5.0, 10.0):
         r = run(["--rate", str(base_rps * mult), "--requests", str(args.served_requests)])
         r["rate_vs_plain_capacity"] = mult
