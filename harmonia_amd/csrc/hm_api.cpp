@@ -642,8 +642,8 @@ void par_copy(void* dst, const void* src, size_t bytes) {
   for (auto& x : th) x.join();
 }
 
-int search_host_stream(hm_tree* t, const uint64_t* q, uint64_t n, uint64_t* out_vals, uint8_t* out_found,
-                       const hm_search_opts* o, hipStream_t s) {
+int search_host_stream_body(hm_tree* t, const uint64_t* q, uint64_t n, uint64_t* out_vals, uint8_t* out_found,
+                            const hm_search_opts* o, hipStream_t s) {
   const uint64_t C = kStreamChunk;
   SearchLaunch sl;
   int bits = 0, mode = 0;
@@ -721,6 +721,15 @@ int search_host_stream(hm_tree* t, const uint64_t* q, uint64_t n, uint64_t* out_
       if ((rc = drain(c))) return rc;
   for (int k = 0; k < 2; ++k) HM_TRY(hipStreamSynchronize(t->hs[k]));
   return HM_OK;
+}
+
+int search_host_stream(hm_tree* t, const uint64_t* q, uint64_t n, uint64_t* out_vals, uint8_t* out_found,
+                       const hm_search_opts* o, hipStream_t s) {
+  const int rc = search_host_stream_body(t, q, n, out_vals, out_found, o, s);
+  if (rc)  // a failed call leaves no copy in flight into the staging slots or the caller's buffers
+    for (hipStream_t h : t->hs)
+      if (h) (void)hipStreamSynchronize(h);
+  return rc;
 }
 
 int search_impl(hm_tree* t, const uint64_t* q, uint64_t n, uint64_t* out_vals, uint8_t* out_found,
