@@ -155,22 +155,22 @@ def cpu_leg(args):
     sv = W.splitmix64_mix(sk)
     q = W.point_queries(sk, 1 << 20, seed0 + 7)
     ev, ef = oracle.search_sorted(sk, sv, q)
-    c1 = {"keys": 1 << 20, "queries": 1 << 20, "fanout": 64, "hit_frac": 0.9}
-    c1.update(run_ports(oracle.build_bfs(sk, sv, 64, 8), q, ev, ef))
+    small = {"keys": 1 << 20, "queries": 1 << 20, "fanout": 64, "hit_frac": 0.9}
+    small.update(run_ports(oracle.build_bfs(sk, sv, 64, 8), q, ev, ef))
     if oracle.ref_available():
         rt = oracle.RefRegularTree(keys, vals, 64)
         t, (v, f) = _median_time(lambda: rt.find(q, nthreads=cores))
         if not (np.array_equal(f, ef) and np.array_equal(v, ev)):
             raise SystemExit("cpu leg: reference regular_bpt find differs from the oracle")
-        c1["reference_regular_find"] = round(len(q) / t / 1e6, 3)
+        small["reference_regular_find"] = round(len(q) / t / 1e6, 3)
         ht = oracle.RefHarmoniaTree(keys, vals)
         t, hv = _median_time(lambda: ht.search(q, nthreads=cores))
         hit = ef.astype(bool)
         if not ((hv[~hit] == -1).all() and np.array_equal(hv[hit].view(np.uint64), ev[hit])):
             raise SystemExit("cpu leg: reference Harmonia search differs from the oracle")
-        c1["reference_harmonia_search"] = round(len(q) / t / 1e6, 3)
-        c1["reference_kind"] = "reference (its CPU sources compiled in oracle/_ref, trees built by its insert loop)"
-    res["cpu_config"] = c1
+        small["reference_harmonia_search"] = round(len(q) / t / 1e6, 3)
+        small["reference_kind"] = "reference (its CPU sources compiled in oracle/_ref, trees built by its insert loop)"
+    res["cpu_config"] = small
     print(json.dumps(res), flush=True)
 
 

@@ -139,7 +139,7 @@ def test_search_matches_reference_code(order, F):
 
 
 @needs_ref
-def test_c1_cpu_reference_paths_agree():
+def test_cpu_config_2p20_reference_paths_agree():
     """The CPU configuration (2^20 keys, fanout 64, 2^20 queries: 90% hits, 10% misses, unsorted):
     the reference's own regular_bpt find and Harmonia BPlusTree::search under their OpenMP loops
     (search_cpu_multithread, search_cpu_test), the two restatements over the bulk-built layout and
