@@ -782,7 +782,7 @@ def main():
             traffic = None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                 "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
-                "kernel": "search_shared_kernel<64,OUT_DIRECT> (PSA-sorted queries, 2^24 per launch)",
+                "kernel": "search_span_kernel<64,OUT_DIRECT> (PSA-sorted queries, 2^24 per launch)",
                 "bytes_per_query": BYTES_PER_QUERY}
     e2e_achieved = BYTES_PER_QUERY_PSA * n_q / (head["ms_per_step"] * 1e-3) / 1e9
     # the whole step (sort + search + un-sort, results in input order) against the 192-B/query model

@@ -231,72 +231,6 @@ __device__ __forceinline__ void descend_wide_q(const TreeView& tv, int lds_level
   for (int q = 0; q < Q; ++q) node[q] -= tv.n_inner;
 }
 
-// One global level for PSA-sorted queries with G lanes per query.  At the upper levels the 64/G
-// queries of a wave's query slot almost always share their node, so the wave loads the node of its
-// first lane once (one coalesced 16-B piece per lane, prefix entry included) into the slot's LDS
-// buffer and the groups in that node rank it from there; a group in another node reads its own
-// directly.  Per query slot that is one vector load per lane instead of the index and key granules
-// of every group.  (At the last inner level the slot straddles two or three nodes: staging two of
-// them there was measured and lost to the divergent ranks, so callers keep it direct.)
-template <int F, int G, int Q>
-__device__ __forceinline__ void shared_node_level(const uint64_t* __restrict__ inner, uint64_t* wslot,
-                                                  const uint64_t (&t)[Q], uint32_t (&node)[Q], int gl, int gbase,
-                                                  int lane) {
-  constexpr int NS = node_slots(F);
-  constexpr int CH = NS / 2;          // 16-byte pieces per node
-  constexpr int PL = (CH + 63) / 64;  // pieces per lane
-  uint32_t n0[Q];
-  ulonglong2 piece[Q][PL];
-#pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    n0[q] = (uint32_t)__builtin_amdgcn_readfirstlane((int)node[q]);
-    const ulonglong2* src = reinterpret_cast<const ulonglong2*>(inner + (size_t)n0[q] * NS);
-#pragma unroll
-    for (int k = 0; k < PL; ++k) {
-      const int idx = lane + 64 * k;
-      piece[q][k] = src[idx < CH ? idx : CH - 1];  // clamped, not predicated: keeps piece in registers
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < Q; ++q) {
-#pragma unroll
-    for (int k = 0; k < PL; ++k)
-      if (lane + 64 * k < CH) reinterpret_cast<ulonglong2*>(wslot + q * NS)[lane + 64 * k] = piece[q][k];
-  }
-  wave_lds_sync();
-#pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    if (node[q] == n0[q])  // uniform inside a group (its lanes share the query)
-      node[q] = node_next<F, G>(wslot + q * NS, t[q], gl, gbase);
-    else
-      node[q] = node_next<F, G>(inner + (size_t)node[q] * NS, t[q], gl, gbase);
-  }
-  wave_lds_sync();  // the next level overwrites the buffers
-}
-
-// descend_q for PSA-sorted queries: the global levels above the last go through shared_node_level
-// (wave buffers of Q * node_slots(F) words at `wslot`), the last level reads per group.
-template <int F, int G, int Q>
-__device__ __forceinline__ void descend_shared(const TreeView& tv, int lds_levels, const uint64_t* snodes,
-                                               uint64_t* wslot, const uint64_t (&t)[Q], uint32_t (&node)[Q], int gl,
-                                               int gbase, int lane) {
-  constexpr int NS = node_slots(F);
-#pragma unroll
-  for (int q = 0; q < Q; ++q) node[q] = 0;
-  int l = 0;
-  for (; l < lds_levels; ++l) {
-#pragma unroll
-    for (int q = 0; q < Q; ++q) node[q] = node_next<F, G>(snodes + (size_t)node[q] * NS, t[q], gl, gbase);
-  }
-  for (; l < tv.height - 1; ++l) shared_node_level<F, G, Q>(tv.inner, wslot, t, node, gl, gbase, lane);
-  for (; l < tv.height; ++l) {
-#pragma unroll
-    for (int q = 0; q < Q; ++q) node[q] = node_next<F, G>(tv.inner + (size_t)node[q] * NS, t[q], gl, gbase);
-  }
-#pragma unroll
-  for (int q = 0; q < Q; ++q) node[q] -= tv.n_inner;
-}
-
 // The next node for a node held in LDS (one lane per query).
 template <int F>
 __device__ __forceinline__ uint32_t node_next_lds(const uint64_t* nd, uint64_t t) {

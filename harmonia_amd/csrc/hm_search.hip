@@ -20,18 +20,56 @@ size_t sorted_lds_bytes(const TreeView& tv, int levels, int Q) {
   return (nl + (size_t)(kThreads / 64) * Q) * node_slots(tv.fanout) * sizeof(uint64_t);
 }
 
-// Queries [begin, end) of qs; result i goes to (ov, of)[i] (or to pos[i] with OUT_SCATTER).  SHARED: the
-// queries are PSA-sorted and the upper global levels go through shared_node_level (wave buffers at
-// wslot, after the staged top levels, sized by sorted_lds_bytes).
-template <int F, int G, int Q, int OUT, bool SHARED>
-__device__ __forceinline__ void search_run(const TreeView& tv, int lds_levels, const uint64_t* smem, uint64_t* wslot,
-                                           const uint64_t* __restrict__ qs, const uint32_t* __restrict__ pos,
-                                           uint64_t* __restrict__ ov, uint8_t* __restrict__ of, uint32_t begin,
-                                           uint32_t end) {
+// Leaf step of a group's Q descents: exact match of t[i] in leaf[i]; the result of query qi[i] goes to
+// (ov, of)[qi[i]] (or to pos[qi[i]] with OUT_SCATTER), skipped for qi[i] >= end.
+template <int G, int Q, int OUT>
+__device__ __forceinline__ void leaf_step(const TreeView& tv, const uint64_t (&t)[Q], const uint32_t (&leaf)[Q],
+                                          const uint32_t (&qi)[Q], uint32_t end, const uint32_t* __restrict__ pos,
+                                          uint64_t* __restrict__ ov, uint8_t* __restrict__ of, int gl, int gbase) {
   constexpr int W = 8 / G;
   // G == 1: one lane per query reads its 8 leaf keys, then only the matching value (same 128-B
   // line, an L2 hit); G > 1: the group reads keys and values of the leaf line together.
   constexpr bool kValsWithKeys = G > 1;
+  uint64_t k[Q][W], v[Q][W];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const Leaf* lf = tv.leaves + leaf[i];
+    load_keys<W>(lf->key + gl * W, k[i]);
+    if constexpr (kValsWithKeys) load_keys<W>(lf->val + gl * W, v[i]);
+  }
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    const int lt = group_count_lt<G, W>(k[i], t[i], gbase);  // lower_bound slot
+    // exact match (H/regular_bpt/prefix-sort.cu:220-227); KEY_MAX is padding, never a stored key
+    const bool eq = group_any_eq<G, W>(k[i], t[i], gbase) && t[i] != KEY_MAX;
+    const int writer = eq ? lt / W : 0;
+    if (qi[i] < end && gl == writer) {
+      uint64_t val = 0ull;
+      if constexpr (kValsWithKeys) {
+#pragma unroll
+        for (int w = 0; w < W; ++w)
+          if (eq && lt % W == w) val = v[i][w];
+      } else {
+        if (eq) val = tv.leaves[leaf[i]].val[lt];
+      }
+      if constexpr (OUT == OUT_DIRECT) {
+        ov[qi[i]] = val;
+        of[qi[i]] = eq ? 1 : 0;
+      } else {
+        const uint32_t o = pos[qi[i]];
+        ov[o] = val;
+        of[o] = eq ? 1 : 0;
+      }
+    }
+  }
+}
+
+// Queries [begin, end) of qs; result i goes to (ov, of)[i] (or to pos[i] with OUT_SCATTER).
+template <int F, int G, int Q, int OUT>
+__device__ __forceinline__ void search_run(const TreeView& tv, int lds_levels, const uint64_t* smem,
+                                           const uint64_t* __restrict__ qs, const uint32_t* __restrict__ pos,
+                                           uint64_t* __restrict__ ov, uint8_t* __restrict__ of, uint32_t begin,
+                                           uint32_t end) {
   const int gl = threadIdx.x % G;
   const int gbase = (threadIdx.x & 63) - gl;
   const uint32_t gid = threadIdx.x / G;
@@ -39,56 +77,21 @@ __device__ __forceinline__ void search_run(const TreeView& tv, int lds_levels, c
   constexpr uint32_t tile = gpb * Q;
   for (uint32_t qb = begin; qb < end; qb += tile) {  // uniform across the block
     uint64_t t[Q], td[Q];
-    uint32_t leaf[Q];
+    uint32_t leaf[Q], qi[Q];
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-      const uint32_t qi = qb + i * gpb + gid;
-      t[i] = qs[qi < end ? qi : end - 1];  // branch-free: lanes past the end repeat the last query
+      qi[i] = qb + i * gpb + gid;
+      t[i] = qs[qi[i] < end ? qi[i] : end - 1];  // branch-free: lanes past the end repeat the last query
       td[i] = t[i] == KEY_MAX ? KEY_MAX - 1 : t[i];
     }
-    if constexpr (SHARED)
-      descend_shared<F, G, Q>(tv, lds_levels, smem, wslot, td, leaf, gl, gbase, threadIdx.x & 63);
-    else
-      descend_q<F, G, Q>(tv, lds_levels, smem, td, leaf, gl, gbase);
-    uint64_t k[Q][W], v[Q][W];
-#pragma unroll
-    for (int i = 0; i < Q; ++i) {
-      const Leaf* lf = tv.leaves + leaf[i];
-      load_keys<W>(lf->key + gl * W, k[i]);
-      if constexpr (kValsWithKeys) load_keys<W>(lf->val + gl * W, v[i]);
-    }
-#pragma unroll
-    for (int i = 0; i < Q; ++i) {
-      const uint32_t qi = qb + i * gpb + gid;
-      const int lt = group_count_lt<G, W>(k[i], t[i], gbase);  // lower_bound slot
-      // exact match (H/regular_bpt/prefix-sort.cu:220-227); KEY_MAX is padding, never a stored key
-      const bool eq = group_any_eq<G, W>(k[i], t[i], gbase) && t[i] != KEY_MAX;
-      const int writer = eq ? lt / W : 0;
-      if (qi < end && gl == writer) {
-        uint64_t val = 0ull;
-        if constexpr (kValsWithKeys) {
-#pragma unroll
-          for (int w = 0; w < W; ++w)
-            if (eq && lt % W == w) val = v[i][w];
-        } else {
-          if (eq) val = tv.leaves[leaf[i]].val[lt];
-        }
-        if constexpr (OUT == OUT_DIRECT) {
-          ov[qi] = val;
-          of[qi] = eq ? 1 : 0;
-        } else {
-          const uint32_t o = pos[qi];
-          ov[o] = val;
-          of[o] = eq ? 1 : 0;
-        }
-      }
-    }
+    descend_q<F, G, Q>(tv, lds_levels, smem, td, leaf, gl, gbase);
+    leaf_step<G, Q, OUT>(tv, t, leaf, qi, end, pos, ov, of, gl, gbase);
   }
 }
 
 // Persistent grid: each block owns one contiguous chunk of the batch; the count may live on the
 // device (graph replays with data-dependent batch sizes).
-template <int F, int G, int Q, int OUT, bool SHARED>
+template <int F, int G, int Q, int OUT>
 __device__ __forceinline__ void search_body(const TreeView& tv, int lds_levels, const uint64_t* __restrict__ q,
                                             const uint32_t* __restrict__ pos, uint64_t* __restrict__ out_vals,
                                             uint8_t* __restrict__ out_found, uint32_t n,
@@ -102,13 +105,7 @@ __device__ __forceinline__ void search_body(const TreeView& tv, int lds_levels, 
   const uint32_t vb = xcd_block(blockIdx.x, gridDim.x);
   const uint32_t begin = vb * chunk;
   const uint32_t end = begin + chunk < n ? begin + chunk : n;
-  uint64_t* wslot = nullptr;
-  if constexpr (SHARED) {
-    constexpr int NS = node_slots(F);
-    const uint32_t nl = lds_levels > 0 ? tv.level_start[lds_levels] : 0u;
-    wslot = smem + (nl + (threadIdx.x >> 6) * Q) * NS;
-  }
-  search_run<F, G, Q, OUT, SHARED>(tv, lds_levels, smem, wslot, q, pos, out_vals, out_found, begin, end);
+  search_run<F, G, Q, OUT>(tv, lds_levels, smem, q, pos, out_vals, out_found, begin, end);
 }
 
 template <int F, int G, int Q, int OUT>
@@ -118,7 +115,7 @@ __global__ __launch_bounds__(kThreads) void search_kernel(TreeView tv, int lds_l
                                                           uint64_t* __restrict__ out_vals,
                                                           uint8_t* __restrict__ out_found, uint32_t n,
                                                           const uint32_t* __restrict__ dn) {
-  search_body<F, G, Q, OUT, false>(tv, lds_levels, q, pos, out_vals, out_found, n, dn);
+  search_body<F, G, Q, OUT>(tv, lds_levels, q, pos, out_vals, out_found, n, dn);
 }
 
 // search_kernel over a segment list (unsorted queries in several device arrays, results at each
@@ -148,8 +145,8 @@ __global__ __launch_bounds__(kThreads) void search_seg_kernel(TreeView tv, int l
     const uint64_t lo = it.dst > begin ? it.dst : begin;
     const uint64_t hi = it.dst + it.n < end ? it.dst + it.n : end;
     if (lo < hi)
-      search_run<F, G, Q, OUT_DIRECT, false>(tv, lds_levels, smem, nullptr, it.src + (lo - it.dst), nullptr,
-                                             out_vals + lo, out_found + lo, 0u, (uint32_t)(hi - lo));
+      search_run<F, G, Q, OUT_DIRECT>(tv, lds_levels, smem, it.src + (lo - it.dst), nullptr, out_vals + lo,
+                                      out_found + lo, 0u, (uint32_t)(hi - lo));
   }
 }
 
@@ -207,14 +204,148 @@ __global__ __launch_bounds__(kThreads) void search_wide_kernel(TreeView tv, int 
   }
 }
 
-// PSA-sorted queries, two lanes per query, four queries per group (the shape it was measured faster
-// on; G = 4 and Q = 2 lost).  Five waves per SIMD as the plain kernel of that shape: without the
-// hint the shared levels cost four more VGPRs and a wave of occupancy.
+// ---- last-level span search (PSA-sorted batches) ----------------------------------------------
+//
+// Two lanes per query, four queries per group, and each wave takes 128 consecutive queries of the
+// sorted batch.  Those queries lie in a narrow key range, so at the upper global levels they share one
+// node (loaded once per wave into its LDS buffer), and at the last inner level they reach a short run
+// of consecutive nodes (BFS order keeps a level's nodes in key order): 128 queries sorted by their top
+// 16 bits out of 2^24 cover one or two 16-bit buckets, 3-5 nodes of 512 keys on a 2^26-key tree.  The
+// wave loads that whole run with one set of coalesced 16-B loads and every group ranks its node from
+// LDS, so the last level costs one global round trip instead of two dependent granule reads per
+// group.  A wave whose run exceeds the buffer (sparse or unsorted input) reads its nodes directly, and
+// a group outside the wave's shared upper-level node reads that level directly: correct for any input
+// order.  Five waves per SIMD (the hint keeps it at 96 VGPRs); with the two staged top levels of the
+// 2^26-key tree a block takes 30.5 KiB of LDS, so five blocks share a CU.
+constexpr int kSpanBytes = 2880;  // per wave: 5 nodes of fanout 64, 9 of fanout 32
+template <int F>
+constexpr int span_nodes() {
+  return kSpanBytes / (node_slots(F) * (int)sizeof(uint64_t));
+}
+
+size_t span_lds_bytes(const TreeView& tv, int levels) {
+  const size_t nl = levels > 0 ? tv.level_start[levels] : 0;
+  return nl * node_slots(tv.fanout) * sizeof(uint64_t) + (size_t)(kThreads / 64) * kSpanBytes;
+}
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint32_t w = (uint32_t)__shfl_xor((int)v, o);
+    v = w < v ? w : v;
+  }
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint32_t w = (uint32_t)__shfl_xor((int)v, o);
+    v = w > v ? w : v;
+  }
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+
 template <int F, int OUT>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) void search_shared_kernel(
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) void search_span_kernel(
     TreeView tv, int lds_levels, const uint64_t* __restrict__ q, const uint32_t* __restrict__ pos,
     uint64_t* __restrict__ out_vals, uint8_t* __restrict__ out_found, uint32_t n, const uint32_t* __restrict__ dn) {
-  search_body<F, 2, 4, OUT, true>(tv, lds_levels, q, pos, out_vals, out_found, n, dn);
+  constexpr int G = 2, Q = 4;
+  constexpr int NS = node_slots(F);
+  constexpr int CH = NS / 2;                   // 16-byte pieces per node
+  constexpr int PL = (CH + 63) / 64;           // pieces per lane of one node
+  constexpr int SPAN = span_nodes<F>();
+  constexpr int SPL = (SPAN * CH + 63) / 64;   // pieces per lane of a full run
+  constexpr uint32_t per_wave = 64 / G * Q;    // 128 consecutive queries per wave and tile
+  constexpr uint32_t tile = kThreads / 64 * per_wave;
+  extern __shared__ __align__(16) uint64_t smem[];
+  stage_top_levels<F>(tv, lds_levels, smem);
+  if (dn) n = *dn;
+  uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  chunk = (chunk + tile - 1) / tile * tile;
+  const uint32_t vb = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t begin = vb * chunk;
+  const uint32_t end = begin + chunk < n ? begin + chunk : n;
+  const int lane = threadIdx.x & 63;
+  const int gl = lane % G;
+  const int gbase = lane - gl;
+  const uint32_t nl = lds_levels > 0 ? tv.level_start[lds_levels] : 0u;
+  uint64_t* wbuf = smem + (size_t)nl * NS + (threadIdx.x >> 6) * (kSpanBytes / sizeof(uint64_t));
+  const uint32_t wq = (threadIdx.x >> 6) * per_wave + lane / G;
+  const uint64_t* __restrict__ inner = tv.inner;
+  const int last = tv.height - 1;
+  for (uint32_t qb = begin; qb < end; qb += tile) {  // uniform across the block
+    uint64_t t[Q], td[Q];
+    uint32_t node[Q], qi[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      qi[i] = qb + wq + i * (64 / G);
+      t[i] = q[qi[i] < end ? qi[i] : end - 1];  // lanes past the end repeat the last query
+      td[i] = t[i] == KEY_MAX ? KEY_MAX - 1 : t[i];
+      node[i] = 0;
+    }
+    int l = 0;
+    for (; l < lds_levels; ++l) {
+#pragma unroll
+      for (int i = 0; i < Q; ++i) node[i] = node_next<F, G>(smem + (size_t)node[i] * NS, td[i], gl, gbase);
+    }
+    for (; l < last; ++l) {  // upper global levels: one node for the wave's queries
+      const uint32_t n0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)node[0]);
+      const ulonglong2* src = reinterpret_cast<const ulonglong2*>(inner + (size_t)n0 * NS);
+      ulonglong2 piece[PL];
+#pragma unroll
+      for (int k = 0; k < PL; ++k) {
+        const int idx = lane + 64 * k;
+        piece[k] = src[idx < CH ? idx : CH - 1];
+      }
+#pragma unroll
+      for (int k = 0; k < PL; ++k)
+        if (lane + 64 * k < CH) reinterpret_cast<ulonglong2*>(wbuf)[lane + 64 * k] = piece[k];
+      wave_lds_sync();
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        if (node[i] == n0)  // uniform inside a group
+          node[i] = node_next<F, G>(wbuf, td[i], gl, gbase);
+        else
+          node[i] = node_next<F, G>(inner + (size_t)node[i] * NS, td[i], gl, gbase);
+      }
+      wave_lds_sync();  // the next level overwrites the buffer
+    }
+    if (l == last) {  // the last inner level: the run of nodes [lo, hi] the wave reaches
+      uint32_t lo = node[0], hi = node[0];
+#pragma unroll
+      for (int i = 1; i < Q; ++i) {
+        lo = node[i] < lo ? node[i] : lo;
+        hi = node[i] > hi ? node[i] : hi;
+      }
+      lo = wave_min_u32(lo);
+      hi = wave_max_u32(hi);
+      if (hi - lo < (uint32_t)SPAN) {  // wave-uniform
+        const int pieces = (int)(hi - lo + 1) * CH;
+        const ulonglong2* src = reinterpret_cast<const ulonglong2*>(inner + (size_t)lo * NS);
+        ulonglong2 piece[SPL];
+#pragma unroll
+        for (int k = 0; k < SPL; ++k) {
+          const int idx = lane + 64 * k;
+          piece[k] = src[idx < pieces ? idx : pieces - 1];  // clamped into nodes lo..hi
+        }
+#pragma unroll
+        for (int k = 0; k < SPL; ++k)
+          if (lane + 64 * k < pieces) reinterpret_cast<ulonglong2*>(wbuf)[lane + 64 * k] = piece[k];
+        wave_lds_sync();
+#pragma unroll
+        for (int i = 0; i < Q; ++i) node[i] = node_next<F, G>(wbuf + (size_t)(node[i] - lo) * NS, td[i], gl, gbase);
+        wave_lds_sync();  // the next tile overwrites the buffer
+      } else {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) node[i] = node_next<F, G>(inner + (size_t)node[i] * NS, td[i], gl, gbase);
+      }
+    }
+    uint32_t leaf[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) leaf[i] = node[i] - tv.n_inner;
+    leaf_step<G, Q, OUT>(tv, t, leaf, qi, end, pos, out_vals, out_found, gl, gbase);
+  }
 }
 
 // Search over PSA-sorted queries, one lane per query, Q per lane: the global levels go through
@@ -504,18 +635,18 @@ hipError_t launch_fgq(const TreeView& tv, const SearchLaunch& sl, const uint64_t
   size_t need = (n + gpb - 1) / gpb;
   size_t nb = need < max_blocks ? need : max_blocks;
   nb = (nb + 7) / 8 * 8;
-  // PSA-sorted input, two lanes per query, four queries per group: the wave shares the upper global
-  // levels' node reads.  Fanout 128 is left out (its kernel spills at five waves per SIMD).
+  // PSA-sorted input, two lanes per query, four queries per group: search_span_kernel.  Fanout 128 is
+  // left out (its kernel spills at five waves per SIMD).
   if constexpr (G == 2 && Q == 4 && F <= 64) {
     if (sl.sorted) {
       int levels = sl.lds_levels;  // the wave buffers come on top of the staged levels: stay within 64 KiB
-      while (levels > 0 && sorted_lds_bytes(tv, levels, Q) > 65536) --levels;
-      const size_t lds = sorted_lds_bytes(tv, levels, Q);
+      while (levels > 0 && span_lds_bytes(tv, levels) > 65536) --levels;
+      const size_t lds = span_lds_bytes(tv, levels);
       if (mode == OUT_DIRECT)
-        hipLaunchKernelGGL((search_shared_kernel<F, OUT_DIRECT>), dim3((unsigned)nb), dim3(kThreads), lds, s, tv,
+        hipLaunchKernelGGL((search_span_kernel<F, OUT_DIRECT>), dim3((unsigned)nb), dim3(kThreads), lds, s, tv,
                            levels, q, pos, out_vals, out_found, (uint32_t)n, dn);
       else
-        hipLaunchKernelGGL((search_shared_kernel<F, OUT_SCATTER>), dim3((unsigned)nb), dim3(kThreads), lds, s, tv,
+        hipLaunchKernelGGL((search_span_kernel<F, OUT_SCATTER>), dim3((unsigned)nb), dim3(kThreads), lds, s, tv,
                            levels, q, pos, out_vals, out_found, (uint32_t)n, dn);
       return hipGetLastError();
     }

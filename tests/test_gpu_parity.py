@@ -128,10 +128,11 @@ def test_search_leaf_window(gpu, F):
 
 
 @pytest.mark.parametrize("F", [32, 64])
-def test_search_shared_levels_deep_tree(gpu, F):
-    """PSA-sorted batches with two lanes x four queries share each wave's upper-level node reads.
-    A tall tree runs that over several levels; few PSA bits leave query slots straddling nodes
-    (the per-group fallback), many bits keep them in one node."""
+def test_search_sorted_span_deep_tree(gpu, F):
+    """PSA-sorted batches with two lanes x four queries: each wave shares its upper-level node reads and
+    loads the run of last-level nodes its 128 queries reach.  A tall tree runs that over several levels;
+    few PSA bits leave waves straddling many nodes (the per-group and direct last-level fallbacks), many
+    bits keep them in one node and a short run."""
     t, keys, vals, sk, sv = make(1 << 21 if F == 32 else 1 << 22, F, 8, seed=31 + F)
     q = queries(sk, 1 << 18, 5 + F)
     ev, ef = oracle.search_sorted(sk, sv, q)
