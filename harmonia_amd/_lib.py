@@ -66,6 +66,10 @@ class HmStats(ctypes.Structure):
         ("epoch", ctypes.c_uint64),
         ("relayouts", ctypes.c_uint64),
         ("respreads", ctypes.c_uint64),
+        ("key_min", ctypes.c_uint64),
+        ("key_max", ctypes.c_uint64),
+        ("psa_window_top", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 

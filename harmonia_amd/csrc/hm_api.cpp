@@ -252,10 +252,14 @@ int enqueue_pipeline(const TreeView& tv, const SearchLaunch& sl, int psa_bits, i
     return HM_OK;
   }
   int which = 0;
+  // Results in input order may come from any bit window: sort by the psa_bits bits below the tree's
+  // psa_top (the bits above it are equal for every key, so they would order nothing but misses outside
+  // the key range).  PSA-order results keep the reference's window [64 - psa_bits, 64).
+  const int top = out_mode == HM_OUT_PSA_ORDER ? 64 : std::max(std::min((int)tv.psa_top, 64), psa_bits);
   if (out_mode == HM_OUT_UNPERMUTE) {
     // keys-only sort recording its passes; search in PSA order (coalesced result writes); then the
     // passes run backwards to put the results in input order
-    HM_TRY(radix_sort_keys_replay(dq, n, 64 - psa_bits, 64, b.k0, b.k1, b.replay, s, &which, dn));
+    HM_TRY(radix_sort_keys_replay(dq, n, top - psa_bits, top, b.k0, b.k1, b.replay, s, &which, dn));
     const uint64_t* sk = which ? b.k1 : b.k0;
     SearchLaunch ss = sl;
     ss.sorted = true;
@@ -263,7 +267,7 @@ int enqueue_pipeline(const TreeView& tv, const SearchLaunch& sl, int psa_bits, i
     HM_TRY(unsort_results(b.rv, b.rf, n, psa_bits, b.replay, b.vt, b.ft, dvals, dfound, s, dn));
     return HM_OK;
   }
-  HM_TRY(radix_sort_pos(dq, n, 64 - psa_bits, 64, b.k0, b.p0, b.k1, b.p1, b.tmp, s, &which, dn));
+  HM_TRY(radix_sort_pos(dq, n, top - psa_bits, top, b.k0, b.p0, b.k1, b.p1, b.tmp, s, &which, dn));
   const uint64_t* sk = which ? b.k1 : b.k0;
   const uint32_t* sp = which ? b.p1 : b.p0;
   SearchLaunch ss = sl;
@@ -570,8 +574,13 @@ int hm_tree_build(const uint64_t* keys, const uint64_t* vals, uint64_t n, const 
     }
   }
   uint32_t flags = 0;
+  uint64_t edge[2] = {~0ull, 0ull};
   BUILD_TRY(validate_sorted(dk, n, d_flags, s));
   BUILD_TRY(hipMemcpyAsync(&flags, d_flags, 4, hipMemcpyDeviceToHost, s));
+  if (n) {
+    BUILD_TRY(hipMemcpyAsync(&edge[0], dk, 8, hipMemcpyDeviceToHost, s));
+    BUILD_TRY(hipMemcpyAsync(&edge[1], dk + n - 1, 8, hipMemcpyDeviceToHost, s));
+  }
   BUILD_TRY(hipStreamSynchronize(s));
   if (flags & 1u) {
     cleanup();
@@ -582,6 +591,7 @@ int hm_tree_build(const uint64_t* keys, const uint64_t* vals, uint64_t n, const 
     return fail(set_err(HM_EINVAL, (p->flags & HM_FLAG_KEYS_SORTED) ? "keys flagged sorted are not strictly ascending"
                                                                      : "duplicate keys in build input"));
   }
+  t->widen_key_range(edge[0], edge[1]);
   BUILD_TRY(hipMalloc(&t->d_leaves, t->leaf_cap * sizeof(Leaf)));
   BUILD_TRY(fill_leaves(dk, dv, n, t->fill, t->d_leaves, t->n_leaves, s));
   {
@@ -1154,7 +1164,11 @@ int publish_prepare(hm_tree* t, hipStream_t s, PublishPlan* plan) {
   HM_TRY(launch_locate(tv, b.uk, n, b.leaf, b.slot, b.found, s, dm));
   HM_TRY(plan_runs(t->d_leaves, (uint32_t)t->n_leaves, t->F, b.leaf, b.found, n, dm, b.summary, b.nlist, s));
   HM_TRY(copy_u32s(t->upd_host, b.summary, 5, s));
+  uint64_t* edge = reinterpret_cast<uint64_t*>(t->upd_host + 8);  // bytes 32..47 of the pinned summary
+  HM_TRY(edge_keys(b.uk, dm, edge, s));
   HM_TRY(hipStreamSynchronize(s));
+  plan->key_lo = edge[0];
+  plan->key_hi = edge[1];
   plan->n_new = t->upd_host[1];
   plan->m = t->upd_host[2];
   // an overflowing leaf whose last-level node has room: that node is re-spread in place at commit
@@ -1369,6 +1383,7 @@ int publish_commit(hm_tree* t, hipStream_t s, const PublishPlan& plan, int* rebu
     t->relayouts++;
   }
   t->n_keys += plan.n_new;
+  t->widen_key_range(plan.key_lo, plan.key_hi);
   t->pend_n = 0;
   HM_TRY(hipEventRecord(t->publish_ev, s));
   t->publish_valid = true;
@@ -1508,7 +1523,7 @@ static int tree_load_impl(const char* path, const hm_params* p, hm_tree** out) {
       fnv1a(leaves.data(), leaves.size() * 8) != h.fnv_leaves)
     return set_err(HM_EINVAL, "%s: leaf section truncated or corrupt", path);
   // the leaf chain must hold the keys in strictly ascending order, padding at the end of each leaf
-  uint64_t n_keys = 0, prev = 0;
+  uint64_t n_keys = 0, prev = 0, first = 0;
   bool any = false;
   for (uint64_t l = 0; l < h.n_leaves; ++l) {
     const uint64_t* k = &leaves[l * 16];
@@ -1519,6 +1534,7 @@ static int tree_load_impl(const char* path, const hm_params* p, hm_tree** out) {
     if (c == 0 && h.n_leaves > 1) return set_err(HM_EINVAL, "%s: empty leaf %llu", path, (unsigned long long)l);
     for (int i = 0; i < c; ++i) {
       if (any && k[i] <= prev) return set_err(HM_EINVAL, "%s: keys out of order in leaf %llu", path, (unsigned long long)l);
+      if (!any) first = k[i];
       prev = k[i];
       any = true;
     }
@@ -1538,6 +1554,7 @@ static int tree_load_impl(const char* path, const hm_params* p, hm_tree** out) {
   t->n_keys = n_keys;
   t->n_leaves = h.n_leaves;
   t->leaf_cap = h.n_leaves;
+  if (any) t->widen_key_range(first, prev);
   auto fail = [&](int code) {
     free_tree(t);
     return code;
@@ -1594,6 +1611,9 @@ int hm_tree_stats(const hm_tree* t, hm_stats* st) {
   st->epoch = t->epoch;
   st->relayouts = t->relayouts;
   st->respreads = t->respreads;
+  st->key_min = t->key_lo;
+  st->key_max = t->key_hi;
+  st->psa_window_top = (uint32_t)t->psa_top;
   return HM_OK;
 }
 

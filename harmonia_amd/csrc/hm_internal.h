@@ -44,6 +44,10 @@ struct TreeView {
   int32_t height;  // inner levels, >= 1
   int32_t fanout;
   uint32_t level_start[MAX_LEVELS + 1];
+  // Host side: one past the highest bit in which the tree's keys differ (64 for fewer than two keys).
+  // Every key agrees with the others above it, so a PSA window that ends there orders queries in the
+  // key range as finely as its width allows.
+  int32_t psa_top;
 };
 
 // ---- launch helpers (host) -------------------------------------------------------------
@@ -216,6 +220,9 @@ hipError_t unique_count(const uint32_t* keep, const uint32_t* keep_scan, size_t 
 // summary[0] |= 2 (a full re-layout is needed).  n sizes the grid; nlist holds 2 n entries.
 hipError_t plan_runs(const Leaf* leaves, uint32_t n_leaves, int F, const uint32_t* leaf, const uint8_t* found, size_t n,
                      const uint32_t* dm, uint32_t* summary, uint32_t* nlist, hipStream_t s);
+// edge[0] = uk[0] and edge[1] = uk[*dm - 1], the smallest and largest of the *dm sorted unique keys
+// (edge = {KEY_MAX, 0} when *dm == 0).  edge may be mapped pinned host memory.
+hipError_t edge_keys(const uint64_t* uk, const uint32_t* dm, uint64_t* edge, hipStream_t s);
 // found[j] |= 2 for the batch entries of the listed nodes (the in-place merge then skips them).
 hipError_t mark_nodes(const uint32_t* nlist, uint32_t n_nodes, int F, const uint32_t* leaf, uint8_t* found,
                       const uint32_t* dm, hipStream_t s);

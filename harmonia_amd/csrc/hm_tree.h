@@ -79,6 +79,9 @@ struct hm_tree {
   hipEvent_t na_ev = nullptr;
   bool na_valid = false;
   uint64_t epoch = 0;
+  // smallest and largest key ever stored (updates only add keys) and the PSA window top they give
+  uint64_t key_lo = ~0ull, key_hi = 0;
+  int psa_top = 64;
   uint64_t relayouts = 0;
   uint64_t respreads = 0;  // last-level nodes re-spread in place by update epochs
   uint64_t* pend_k = nullptr;
@@ -116,7 +119,16 @@ struct hm_tree {
     tv.height = height;
     tv.fanout = F;
     for (int i = 0; i <= hm::MAX_LEVELS; ++i) tv.level_start[i] = level_start[i];
+    tv.psa_top = psa_top;
     return tv;
+  }
+  // Widen the key range [key_lo, key_hi] by [lo, hi] (lo > hi: nothing) and recompute psa_top.
+  void widen_key_range(uint64_t lo, uint64_t hi) {
+    if (lo > hi) return;
+    key_lo = lo < key_lo ? lo : key_lo;
+    key_hi = hi > key_hi ? hi : key_hi;
+    const uint64_t d = key_lo ^ key_hi;
+    psa_top = d ? 64 - __builtin_clzll(d) : 64;
   }
   uint64_t device_bytes() const {
     return (uint64_t)(inner_cap + sb_inner_cap) * (NS * 8 + 4) + (leaf_cap + sb_leaf_cap) * sizeof(hm::Leaf) +
@@ -140,6 +152,7 @@ struct PublishPlan {
   bool relayout = false; // the standby layout holds the new epoch
   uint32_t respread = 0;  // last-level nodes re-spread in place (their list in the update scratch)
   uint64_t new_leaves = 0;
+  uint64_t key_lo = ~0ull, key_hi = 0;  // smallest and largest staged key (lo > hi: none)
   Levels lv;
 };
 // Staging and the two publish phases; the caller holds t->pub_mu across them.  Searches may run

@@ -661,7 +661,21 @@ __global__ void apply_runs_kernel(Leaf* __restrict__ leaves, const uint32_t* __r
   }
 }
 
+__global__ void edge_keys_kernel(const uint64_t* __restrict__ uk, const uint32_t* __restrict__ dm,
+                                 uint64_t* __restrict__ edge) {
+  if (threadIdx.x == 0) {
+    const uint32_t m = *dm;
+    edge[0] = m ? uk[0] : KEY_MAX;
+    edge[1] = m ? uk[m - 1] : 0ull;
+  }
+}
+
 }  // namespace
+
+hipError_t edge_keys(const uint64_t* uk, const uint32_t* dm, uint64_t* edge, hipStream_t s) {
+  hipLaunchKernelGGL(edge_keys_kernel, dim3(1), dim3(64), 0, s, uk, dm, edge);
+  return hipGetLastError();
+}
 
 hipError_t small_sort_dedupe(const uint64_t* pk, const uint64_t* pv, uint32_t n, uint64_t* uk, uint64_t* uv,
                              uint32_t* dm, hipStream_t s) {

@@ -71,6 +71,12 @@ typedef struct hm_stats {
   uint64_t relayouts;      /* epochs that re-laid out every leaf (a last-level node was full) */
   uint64_t respreads;      /* last-level nodes whose keys an epoch re-spread over their own leaves in
                             * place (a leaf overflowed, the node had room) */
+  uint64_t key_min;        /* smallest stored key (HM_KEY_MAX for an empty tree) */
+  uint64_t key_max;        /* largest stored key (0 for an empty tree) */
+  uint32_t psa_window_top; /* input-order PSA searches sort by bits [top - psa_bits, top): one past the
+                            * highest bit in which key_min and key_max differ (64 for fewer than two
+                            * keys); PSA-order results (HM_OUT_PSA_ORDER) always use [64 - psa_bits, 64) */
+  uint32_t reserved;
 } hm_stats;
 
 /* hm_search_opts.output_mode (with PSA on; without PSA results are always in input order) */
@@ -94,7 +100,8 @@ typedef struct hm_search_opts {
 
 /* hm_search_opts.sorted_kernel */
 #define HM_SORTED_AUTO 0    /* the tree's choice (one descent per query) */
-#define HM_SORTED_QUERY 1   /* one descent per query (waves share upper-level node reads) */
+#define HM_SORTED_QUERY 1   /* one descent per query (each wave shares its upper-level node reads and
+                             * loads the run of last-level nodes its queries reach) */
 #define HM_SORTED_WINDOW 2  /* leaf window: each wave streams the leaves its chunk of sorted queries
                              * spans into LDS and ranks the chunk there (chunks spanning more
                              * than 256 leaves fall back to one descent per query) */

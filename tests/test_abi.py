@@ -39,7 +39,7 @@ def test_binding_covers_header():
 def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(_lib.HmParams) == 48
     assert ctypes.sizeof(_lib.HmSearchOpts) == 40
-    assert ctypes.sizeof(_lib.HmStats) == 8 * 3 + 4 * 4 + 8 * 17 + 8 * 7
+    assert ctypes.sizeof(_lib.HmStats) == 8 * 3 + 4 * 4 + 8 * 17 + 8 * 7 + 8 * 2 + 4 * 2
     assert ctypes.sizeof(_lib.HmServerStats) == 8 * 8 + 8 * 5 + 8 * 4 + 8 * 3
 
 

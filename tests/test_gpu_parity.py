@@ -238,6 +238,79 @@ def test_sorted_output_matches_psa_order(gpu):
     assert np.array_equal(f, ef) and np.array_equal(v, ev)
 
 
+@pytest.mark.parametrize("base_bits", [20, 40, 62])
+def test_psa_window_follows_key_range(gpu, base_bits, tmp_path):
+    """Input-order results sort the queries by the psa_bits bits below the highest bit in which the tree's
+    keys differ.  Keys packed into [2^base_bits, 2^base_bits + 2^30) put that window far below bit 64;
+    queries inside, below and above the range, and an update epoch that widens it (keys below and above),
+    must all match the oracle, and PSA-order output keeps the reference's window [64 - bits, 64)."""
+    g = W.rng(base_bits, 7)
+    base = np.uint64(1 << base_bits)
+    off = np.unique(g.integers(0, 1 << 30, size=260000, dtype=np.uint64))[:200000]
+    keys = base + off
+    g.shuffle(keys)
+    vals = W.splitmix64_mix(keys)
+    t = BPlusTree.from_keys(keys, vals, fanout=64, leaf_fill=6)
+    sk = np.sort(keys)
+    sv = W.splitmix64_mix(sk)
+
+    def check_range(tree, cur):
+        st = tree.stats()
+        assert (st.key_min, st.key_max) == (int(cur[0]), int(cur[-1]))
+        assert st.psa_window_top == (int(cur[0]) ^ int(cur[-1])).bit_length()
+
+    check_range(t, sk)
+    assert t.stats().psa_window_top <= 31
+
+    def mixed_queries(sk, seed):
+        h = W.rng(seed, 3)
+        lo, hi = int(sk[0]), int(sk[-1])
+        parts = [sk[h.integers(0, len(sk), size=60000)],                                   # hits
+                 h.integers(lo, hi, size=20000, dtype=np.uint64),                          # in range
+                 h.integers(0, lo, size=5000, dtype=np.uint64),                            # below
+                 h.integers(hi + 1, int(KEY_MAX), size=5000, dtype=np.uint64),             # above
+                 np.array([0, lo, hi, lo - 1, hi + 1, int(KEY_MAX) - 1, int(KEY_MAX)], dtype=np.uint64)]
+        q = np.concatenate(parts)
+        h.shuffle(q)
+        return q
+
+    q = mixed_queries(sk, base_bits)
+    ev, ef = oracle.search_sorted(sk, sv, q)
+    for mode in (1, 2):
+        for bits in (8, 16, 24):
+            v, f = t.search_batch(q, psa_bits=bits, output_mode=mode)
+            assert np.array_equal(f, ef), (mode, bits)
+            assert np.array_equal(v, ev), (mode, bits)
+    pos = np.empty(len(q), dtype=np.uint32)
+    v, f = t.search_batch(q, psa_bits=16, output_mode=3, out_pos=pos)
+    ko, po = oracle.psa_sort(q, 48, 64)
+    assert np.array_equal(pos, po)
+    assert np.array_equal(f, ef[pos]) and np.array_equal(v, ev[pos])
+    # widen the key range on both sides (and touch existing keys), then search the new state
+    new = np.unique(np.concatenate([g.integers(0, int(base), size=3000, dtype=np.uint64),
+                                    g.integers(int(sk[-1]) + 1, 1 << 63, size=3000, dtype=np.uint64)]))
+    new = np.setdiff1d(new, sk)
+    upd = np.concatenate([new, sk[::97]])
+    uv = W.splitmix64_mix(upd ^ np.uint64(5))
+    t.update(upd, uv)
+    t.publish()
+    cur_k, cur_v = oracle.upsert_sorted(sk, sv, upd, uv)
+    check_range(t, cur_k)
+    assert t.stats().psa_window_top == 63
+    path = tmp_path / "w.hmsnap"
+    t.save(path)
+    t2 = BPlusTree.load(path)
+    check_range(t2, cur_k)
+    t2.close()
+    q2 = np.concatenate([mixed_queries(cur_k, base_bits + 1), new])
+    ev2, ef2 = oracle.search_sorted(cur_k, cur_v, q2)
+    for mode in (1, 2):
+        v, f = t.search_batch(q2, psa_bits=16, output_mode=mode)
+        assert np.array_equal(f, ef2), mode
+        assert np.array_equal(v, ev2), mode
+    t.close()
+
+
 # ---------------------------------------------------------------- PSA sort
 @pytest.mark.parametrize("n", [1, 100, 4095, 4096, 4097, 300001])
 @pytest.mark.parametrize("bits", [(48, 64), (0, 64), (56, 64), (40, 56), (60, 64), (3, 9)])
