@@ -215,17 +215,19 @@ __global__ __launch_bounds__(kThreads) void search_wide_kernel(TreeView tv, int 
 // LDS, so the last level costs one global round trip instead of two dependent granule reads per
 // group.  A wave whose run exceeds the buffer (sparse or unsorted input) reads its nodes directly, and
 // a group outside the wave's shared upper-level node reads that level directly: correct for any input
-// order.  Five waves per SIMD (the hint keeps it at 96 VGPRs); with the two staged top levels of the
-// 2^26-key tree a block takes 30.5 KiB of LDS, so five blocks share a CU.
-constexpr int kSpanBytes = 2880;  // per wave: 5 nodes of fanout 64, 9 of fanout 32
+// order.  Five waves per SIMD (the hint keeps it at 96 VGPRs at fanout 32 and 64, 90 at 128); with the
+// two staged top levels of the 2^26-key fanout-64 tree a block takes 30.5 KiB of LDS, so five blocks
+// share a CU.
+// Bytes of a wave's buffer: 9 nodes of fanout 32, 5 of fanout 64, 3 of fanout 128.
+constexpr int span_bytes(int F) { return F >= 128 ? 3 * node_slots(F) * (int)sizeof(uint64_t) : 2880; }
 template <int F>
 constexpr int span_nodes() {
-  return kSpanBytes / (node_slots(F) * (int)sizeof(uint64_t));
+  return span_bytes(F) / (node_slots(F) * (int)sizeof(uint64_t));
 }
 
 size_t span_lds_bytes(const TreeView& tv, int levels) {
   const size_t nl = levels > 0 ? tv.level_start[levels] : 0;
-  return nl * node_slots(tv.fanout) * sizeof(uint64_t) + (size_t)(kThreads / 64) * kSpanBytes;
+  return nl * node_slots(tv.fanout) * sizeof(uint64_t) + (size_t)(kThreads / 64) * span_bytes(tv.fanout);
 }
 
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
@@ -270,7 +272,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) v
   const int gl = lane % G;
   const int gbase = lane - gl;
   const uint32_t nl = lds_levels > 0 ? tv.level_start[lds_levels] : 0u;
-  uint64_t* wbuf = smem + (size_t)nl * NS + (threadIdx.x >> 6) * (kSpanBytes / sizeof(uint64_t));
+  uint64_t* wbuf = smem + (size_t)nl * NS + (threadIdx.x >> 6) * (span_bytes(F) / sizeof(uint64_t));
   const uint32_t wq = (threadIdx.x >> 6) * per_wave + lane / G;
   const uint64_t* __restrict__ inner = tv.inner;
   const int last = tv.height - 1;
@@ -635,9 +637,8 @@ hipError_t launch_fgq(const TreeView& tv, const SearchLaunch& sl, const uint64_t
   size_t need = (n + gpb - 1) / gpb;
   size_t nb = need < max_blocks ? need : max_blocks;
   nb = (nb + 7) / 8 * 8;
-  // PSA-sorted input, two lanes per query, four queries per group: search_span_kernel.  Fanout 128 is
-  // left out (its kernel spills at five waves per SIMD).
-  if constexpr (G == 2 && Q == 4 && F <= 64) {
+  // PSA-sorted input, two lanes per query, four queries per group: search_span_kernel.
+  if constexpr (G == 2 && Q == 4) {
     if (sl.sorted) {
       int levels = sl.lds_levels;  // the wave buffers come on top of the staged levels: stay within 64 KiB
       while (levels > 0 && span_lds_bytes(tv, levels) > 65536) --levels;
