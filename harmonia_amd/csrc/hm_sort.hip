@@ -51,6 +51,9 @@ size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 // Per-tile digit counts, one block per tile.  VEC: the keys are 16-byte aligned and full tiles are
 // read with 16-byte loads (two keys per lane per load; the counts do not depend on which lane counts
 // which key).  (A persistent grid that loads tile t + 1 while counting tile t measured no faster.)
+// In full tiles a wave whose 128 keys of one load share a digit (runs of sorted keys, a hot key
+// repeated, the second pass over a skewed batch) adds 128 once instead of 128 atomics serialized on one
+// LDS bin.
 template <typename K, bool VEC, int NT>
 __global__ __launch_bounds__(NT) void radix_hist_kernel(const K* __restrict__ keys, size_t n, int shift, uint32_t mask,
                                                         uint32_t* __restrict__ counts, uint32_t nb,
@@ -73,9 +76,15 @@ __global__ __launch_bounds__(NT) void radix_hist_kernel(const K* __restrict__ ke
 #pragma unroll
     for (int r = 0; r < kItems / 2; ++r) kk[r] = src[r * 64 + lane];
 #pragma unroll
-    for (int r = 0; r < kItems / 2; ++r) {
-      atomicAdd(&hist[w][(uint32_t)(kk[r].x >> shift) & mask], 1u);
-      atomicAdd(&hist[w][(uint32_t)(kk[r].y >> shift) & mask], 1u);
+    for (int r = 0; r < kItems / 2; ++r) {  // every lane of the wave is active here
+      const uint32_t dx = (uint32_t)(kk[r].x >> shift) & mask, dy = (uint32_t)(kk[r].y >> shift) & mask;
+      const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)dx);
+      if (__ballot(dx == d0 && dy == d0) == ~0ull) {  // wave-uniform: the wave's 128 keys share a digit
+        if (lane == 0) atomicAdd(&hist[w][d0], 128u);
+      } else {
+        atomicAdd(&hist[w][dx], 1u);
+        atomicAdd(&hist[w][dy], 1u);
+      }
     }
   } else {
     const size_t last = n ? n - 1 : 0;

@@ -324,6 +324,47 @@ def test_psa_sort_bit_exact(gpu, n, bits):
     assert np.array_equal(po, ep) and np.array_equal(ko, eo)
 
 
+@pytest.mark.parametrize("kind", ["ascending", "runs", "few_values"])
+@pytest.mark.parametrize("bits", [(48, 64), (56, 64), (0, 64), (40, 56)])
+def test_psa_sort_skewed_inputs(gpu, kind, bits):
+    """Inputs whose waves of keys share a digit (sorted runs, a few repeated values), where the histogram
+    counts a whole wave load at once, against the oracle's stable sort."""
+    n = 300001
+    g = W.rng(n, bits[0] + 7)
+    keys = np.sort(g.integers(0, 1 << 63, n, dtype=np.uint64))
+    if kind == "runs":
+        keys = W.order_queries(keys, "runs4096", 5)
+    elif kind == "few_values":
+        keys = g.choice(np.array([3 << 50, (3 << 50) + 1, 1 << 62], dtype=np.uint64), n)
+    ko, po = psa_sort(keys, *bits)
+    eo, ep = oracle.psa_sort(keys, *bits)
+    assert np.array_equal(po, ep) and np.array_equal(ko, eo)
+
+
+@pytest.mark.parametrize("kind", ["runs", "hot"])
+def test_search_psa_skewed_queries(gpu, kind):
+    """PSA searches over run-sorted and hot-key query batches (the replay sort's histograms take whole
+    wave loads at once) against the oracle, results in input order and in PSA order."""
+    t, keys, vals, sk, sv = make(1 << 20, 64, 8, seed=61)
+    if kind == "runs":
+        q = W.point_queries(sk, 1 << 20, 62, order="runs4096")
+    else:
+        g = W.rng(63, 1)
+        hot = sk[g.integers(0, len(sk), size=8)]
+        q = np.concatenate([np.repeat(hot, 60000), W.point_queries(sk, 1 << 19, 64)])
+        g.shuffle(q)
+    ev, ef = oracle.search_sorted(sk, sv, q)
+    for mode in (1, 2):
+        v, f = t.search_batch(q, psa_bits=16, output_mode=mode)
+        assert np.array_equal(f, ef) and np.array_equal(v, ev), mode
+    pos = np.empty(len(q), dtype=np.uint32)
+    v, f = t.search_batch(q, psa_bits=16, output_mode=3, out_pos=pos)
+    ko, po = oracle.psa_sort(q, 48, 64)
+    assert np.array_equal(pos, po)
+    assert np.array_equal(f, ef[pos]) and np.array_equal(v, ev[pos])
+    t.close()
+
+
 # ---------------------------------------------------------------- range
 @pytest.mark.parametrize("F,fill,psa", [(64, 8, 16), (128, 8, 16), (32, 3, 16), (64, 8, 0), (128, 5, 12)])
 def test_range_parity(gpu, F, fill, psa):
