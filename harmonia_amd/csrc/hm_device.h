@@ -306,14 +306,26 @@ __device__ __forceinline__ void sorted_level(const uint64_t* __restrict__ inner,
   for (int q = 0; q < Q; ++q) node[q] = res[q];
 }
 
-// Stage the top `lds_levels` levels (their nodes, prefix entries included) into dynamic LDS.
+// Stage the top `lds_levels` levels (their nodes, prefix entries included) into dynamic LDS.  Whole
+// block-wide steps of 16-byte pieces go by LDS-direct loads, all in flight at once; the partial last step
+// (which must not write past the staged region) by one ordinary load per thread.  blockDim.x % 64 == 0.
 template <int F>
 __device__ __forceinline__ void stage_top_levels(const TreeView& tv, int lds_levels, uint64_t* smem) {
+  typedef __attribute__((address_space(1))) void global_void_t;
+  typedef __attribute__((address_space(3))) void lds_void_t;
   constexpr int NS = node_slots(F);
   const uint32_t nl = lds_levels > 0 ? tv.level_start[lds_levels] : 0u;
-  const uint32_t words = nl * NS;
-  for (uint32_t i = threadIdx.x; i < words / 2; i += blockDim.x)
-    reinterpret_cast<ulonglong2*>(smem)[i] = reinterpret_cast<const ulonglong2*>(tv.inner)[i];
+  const uint32_t pieces = nl * NS / 2;
+  const uint32_t nt = blockDim.x;
+  const uint32_t full = pieces / nt;
+  const uint32_t w0 = threadIdx.x & ~63u;  // the wave's first thread: its LDS-direct destination base
+  const ulonglong2* src = reinterpret_cast<const ulonglong2*>(tv.inner);
+  for (uint32_t k = 0; k < full; ++k)  // uniform across the block
+    __builtin_amdgcn_global_load_lds((const global_void_t*)(src + k * nt + threadIdx.x),
+                                     (lds_void_t*)(smem + 2 * (k * nt + w0)), 16, 0, 0);
+  const uint32_t i = full * nt + threadIdx.x;
+  if (i < pieces) reinterpret_cast<ulonglong2*>(smem)[i] = src[i];
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's LDS-direct loads have landed
   __syncthreads();
 }
 

@@ -215,19 +215,45 @@ __global__ __launch_bounds__(kThreads) void search_wide_kernel(TreeView tv, int 
 // LDS, so the last level costs one global round trip instead of two dependent granule reads per
 // group.  A wave whose run exceeds the buffer (sparse or unsorted input) reads its nodes directly, and
 // a group outside the wave's shared upper-level node reads that level directly: correct for any input
-// order.  Five waves per SIMD (the hint keeps it at 96 VGPRs at fanout 32 and 64, 90 at 128); with the
-// two staged top levels of the 2^26-key fanout-64 tree a block takes 30.5 KiB of LDS, so five blocks
-// share a CU.
-// Bytes of a wave's buffer: 9 nodes of fanout 32, 5 of fanout 64, 3 of fanout 128.
-constexpr int span_bytes(int F) { return F >= 128 ? 3 * node_slots(F) * (int)sizeof(uint64_t) : 2880; }
+// order.  The node and run loads are LDS-direct (global_load_lds): every load of a run is in flight at
+// once and no registers hold the pieces (register-staged loads, each consumed by a conditional LDS
+// store, were issued one round trip apart).  Five waves per SIMD (90 VGPRs at fanout 32 and 64, 86 at
+// 128); with the two staged top levels of the 2^26-key fanout-64 tree a block takes 30.6 KiB of LDS, so
+// five blocks share a CU.
+typedef __attribute__((address_space(1))) void global_void;
+typedef __attribute__((address_space(3))) void lds_void;
+
+// Nodes of a wave's run held in LDS: 9 of fanout 32, 5 of fanout 64, 3 of fanout 128.  The wave's
+// buffer is whole KiB (one 16-byte piece per lane per KiB), so every lane's LDS-direct load lands inside it.
+constexpr int span_nodes_of(int F) { return F >= 128 ? 3 : (F >= 64 ? 5 : 9); }
 template <int F>
 constexpr int span_nodes() {
-  return span_bytes(F) / (node_slots(F) * (int)sizeof(uint64_t));
+  return span_nodes_of(F);
 }
+constexpr int span_steps(int F) { return (span_nodes_of(F) * node_slots(F) / 2 + 63) / 64; }
+constexpr int node_steps(int F) { return (node_slots(F) / 2 + 63) / 64; }
+constexpr int span_bytes(int F) { return (span_steps(F) > node_steps(F) ? span_steps(F) : node_steps(F)) * 1024; }
 
 size_t span_lds_bytes(const TreeView& tv, int levels) {
   const size_t nl = levels > 0 ? tv.level_start[levels] : 0;
   return nl * node_slots(tv.fanout) * sizeof(uint64_t) + (size_t)(kThreads / 64) * span_bytes(tv.fanout);
+}
+
+// Pieces [0, count) (16 bytes each, count wave-uniform) of src into buf by LDS-direct loads, piece j at
+// byte 16 j: STEPS loads per lane at most, all issued before the wait (lanes past count reload the last
+// piece into the buffer's padding).  The caller has finished every read of buf.
+template <int STEPS>
+__device__ __forceinline__ void wave_stage(const ulonglong2* src, int count, uint64_t* buf, int lane) {
+#pragma unroll
+  for (int k = 0; k < STEPS; ++k) {
+    if (64 * k < count) {  // wave-uniform
+      const int idx = lane + 64 * k;
+      __builtin_amdgcn_global_load_lds((const global_void*)(src + (idx < count ? idx : count - 1)),
+                                       (lds_void*)(buf + 128 * k), 16, 0, 0);
+    }
+  }
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the loads have landed in LDS
+  wave_lds_sync();
 }
 
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
@@ -258,6 +284,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) v
   constexpr int PL = (CH + 63) / 64;           // pieces per lane of one node
   constexpr int SPAN = span_nodes<F>();
   constexpr int SPL = (SPAN * CH + 63) / 64;   // pieces per lane of a full run
+  static_assert(span_bytes(F) >= SPL * 1024 && span_bytes(F) >= PL * 1024, "wave buffer holds every lane's piece");
   constexpr uint32_t per_wave = 64 / G * Q;    // 128 consecutive queries per wave and tile
   constexpr uint32_t tile = kThreads / 64 * per_wave;
   extern __shared__ __align__(16) uint64_t smem[];
@@ -293,17 +320,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) v
     }
     for (; l < last; ++l) {  // upper global levels: one node for the wave's queries
       const uint32_t n0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)node[0]);
-      const ulonglong2* src = reinterpret_cast<const ulonglong2*>(inner + (size_t)n0 * NS);
-      ulonglong2 piece[PL];
-#pragma unroll
-      for (int k = 0; k < PL; ++k) {
-        const int idx = lane + 64 * k;
-        piece[k] = src[idx < CH ? idx : CH - 1];
-      }
-#pragma unroll
-      for (int k = 0; k < PL; ++k)
-        if (lane + 64 * k < CH) reinterpret_cast<ulonglong2*>(wbuf)[lane + 64 * k] = piece[k];
-      wave_lds_sync();
+      wave_stage<PL>(reinterpret_cast<const ulonglong2*>(inner + (size_t)n0 * NS), CH, wbuf, lane);
 #pragma unroll
       for (int i = 0; i < Q; ++i) {
         if (node[i] == n0)  // uniform inside a group
@@ -323,18 +340,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) v
       lo = wave_min_u32(lo);
       hi = wave_max_u32(hi);
       if (hi - lo < (uint32_t)SPAN) {  // wave-uniform
-        const int pieces = (int)(hi - lo + 1) * CH;
-        const ulonglong2* src = reinterpret_cast<const ulonglong2*>(inner + (size_t)lo * NS);
-        ulonglong2 piece[SPL];
-#pragma unroll
-        for (int k = 0; k < SPL; ++k) {
-          const int idx = lane + 64 * k;
-          piece[k] = src[idx < pieces ? idx : pieces - 1];  // clamped into nodes lo..hi
-        }
-#pragma unroll
-        for (int k = 0; k < SPL; ++k)
-          if (lane + 64 * k < pieces) reinterpret_cast<ulonglong2*>(wbuf)[lane + 64 * k] = piece[k];
-        wave_lds_sync();
+        // nodes lo..hi (consecutive in memory), every load in flight at once
+        wave_stage<SPL>(reinterpret_cast<const ulonglong2*>(inner + (size_t)lo * NS), (int)(hi - lo + 1) * CH, wbuf,
+                        lane);
 #pragma unroll
         for (int i = 0; i < Q; ++i) node[i] = node_next<F, G>(wbuf + (size_t)(node[i] - lo) * NS, td[i], gl, gbase);
         wave_lds_sync();  // the next tile overwrites the buffer
