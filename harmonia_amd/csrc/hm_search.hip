@@ -217,9 +217,9 @@ __global__ __launch_bounds__(kThreads) void search_wide_kernel(TreeView tv, int 
 // a group outside the wave's shared upper-level node reads that level directly: correct for any input
 // order.  The node and run loads are LDS-direct (global_load_lds): every load of a run is in flight at
 // once and no registers hold the pieces (register-staged loads, each consumed by a conditional LDS
-// store, were issued one round trip apart).  Five waves per SIMD (90 VGPRs at fanout 32 and 64, 86 at
-// 128); with the two staged top levels of the 2^26-key fanout-64 tree a block takes 30.6 KiB of LDS, so
-// five blocks share a CU.
+// store, were issued one round trip apart).  Five waves per SIMD (94 VGPRs at fanout 32 and 64, 92 at
+// 128, with the key prefetch); with the two staged top levels of the 2^26-key fanout-64 tree a block
+// takes 30.6 KiB of LDS, so five blocks share a CU.
 typedef __attribute__((address_space(1))) void global_void;
 typedef __attribute__((address_space(3))) void lds_void;
 
@@ -247,8 +247,9 @@ __device__ __forceinline__ void wave_stage(const ulonglong2* src, int count, uin
 #pragma unroll
   for (int k = 0; k < STEPS; ++k) {
     if (64 * k < count) {  // wave-uniform
-      const int idx = lane + 64 * k;
-      __builtin_amdgcn_global_load_lds((const global_void*)(src + (idx < count ? idx : count - 1)),
+      // min(lane + 64 k, count - 1) written so that only `lane` stays live across calls
+      const int lim = count - 1 - 64 * k;  // >= 0 here, uniform
+      __builtin_amdgcn_global_load_lds((const global_void*)(src + 64 * k + (lane < lim ? lane : lim)),
                                        (lds_void*)(buf + 128 * k), 16, 0, 0);
     }
   }
@@ -274,7 +275,10 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
 }
 
-template <int F, int OUT>
+// PF: while a tile's leaves are read, LDS-direct loads bring the next tile's 128 keys of the wave into the
+// first KiB of its buffer (four dword loads per lane, no registers held across the leaf step), so a tile
+// starts with its keys in LDS instead of a global round trip.
+template <int F, int OUT, bool PF>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) void search_span_kernel(
     TreeView tv, int lds_levels, const uint64_t* __restrict__ q, const uint32_t* __restrict__ pos,
     uint64_t* __restrict__ out_vals, uint8_t* __restrict__ out_found, uint32_t n, const uint32_t* __restrict__ dn) {
@@ -303,13 +307,42 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) v
   const uint32_t wq = (threadIdx.x >> 6) * per_wave + lane / G;
   const uint64_t* __restrict__ inner = tv.inner;
   const int last = tv.height - 1;
+  // the wave's 128 keys of the tile at b as 256 dwords into wbuf, dword indices clamped to the last one
+  // of the batch (lanes past the end read a copy of it; every key below n arrives whole)
+  // (u32 dword indices: the launcher takes this path only for n <= 2^31 - 1024, so 2 (n + tile) fits)
+  const uint32_t last_dw = 2 * n - 1;  // used only when n > 0
+  auto prefetch = [&](uint32_t b) {
+    const uint32_t w0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)(b + (threadIdx.x >> 6) * per_wave));
+    const uint32_t* qd = reinterpret_cast<const uint32_t*>(q);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t d = 2 * w0 + 64 * k + lane;
+      __builtin_amdgcn_global_load_lds((const global_void*)(qd + (d < last_dw ? d : last_dw)),
+                                       (lds_void*)(reinterpret_cast<uint32_t*>(wbuf) + 64 * k), 4, 0, 0);
+    }
+  };
+  if constexpr (PF) {
+    if (begin < end) prefetch(begin);
+  }
   for (uint32_t qb = begin; qb < end; qb += tile) {  // uniform across the block
     uint64_t t[Q], td[Q];
     uint32_t node[Q], qi[Q];
+    if constexpr (PF) {
+      __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): the tile's keys are in LDS
+      wave_lds_sync();
+#pragma unroll
+      for (int i = 0; i < Q; ++i) t[i] = wbuf[i * (64 / G) + lane / G];
+      wave_lds_sync();  // every lane has its keys before the buffer is reused
+    } else {
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        const uint32_t j = qb + wq + i * (64 / G);
+        t[i] = q[j < end ? j : end - 1];  // lanes past the end repeat the last query
+      }
+    }
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
       qi[i] = qb + wq + i * (64 / G);
-      t[i] = q[qi[i] < end ? qi[i] : end - 1];  // lanes past the end repeat the last query
       td[i] = t[i] == KEY_MAX ? KEY_MAX - 1 : t[i];
       node[i] = 0;
     }
@@ -350,6 +383,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) v
 #pragma unroll
         for (int i = 0; i < Q; ++i) node[i] = node_next<F, G>(inner + (size_t)node[i] * NS, td[i], gl, gbase);
       }
+    }
+    if constexpr (PF) {  // every read of wbuf in this tile is done: bring in the next tile's keys
+      if (qb + tile < end) prefetch(qb + tile);
     }
     uint32_t leaf[Q];
 #pragma unroll
@@ -651,12 +687,19 @@ hipError_t launch_fgq(const TreeView& tv, const SearchLaunch& sl, const uint64_t
       int levels = sl.lds_levels;  // the wave buffers come on top of the staged levels: stay within 64 KiB
       while (levels > 0 && span_lds_bytes(tv, levels) > 65536) --levels;
       const size_t lds = span_lds_bytes(tv, levels);
-      if (mode == OUT_DIRECT)
-        hipLaunchKernelGGL((search_span_kernel<F, OUT_DIRECT>), dim3((unsigned)nb), dim3(kThreads), lds, s, tv,
-                           levels, q, pos, out_vals, out_found, (uint32_t)n, dn);
+      const bool pf = n <= (1u << 31) - 1024;  // the prefetch's u32 dword indices fit
+      if (mode == OUT_DIRECT && pf)
+        hipLaunchKernelGGL((search_span_kernel<F, OUT_DIRECT, true>), dim3((unsigned)nb), dim3(kThreads), lds, s,
+                           tv, levels, q, pos, out_vals, out_found, (uint32_t)n, dn);
+      else if (mode == OUT_DIRECT)
+        hipLaunchKernelGGL((search_span_kernel<F, OUT_DIRECT, false>), dim3((unsigned)nb), dim3(kThreads), lds, s,
+                           tv, levels, q, pos, out_vals, out_found, (uint32_t)n, dn);
+      else if (pf)
+        hipLaunchKernelGGL((search_span_kernel<F, OUT_SCATTER, true>), dim3((unsigned)nb), dim3(kThreads), lds, s,
+                           tv, levels, q, pos, out_vals, out_found, (uint32_t)n, dn);
       else
-        hipLaunchKernelGGL((search_span_kernel<F, OUT_SCATTER>), dim3((unsigned)nb), dim3(kThreads), lds, s, tv,
-                           levels, q, pos, out_vals, out_found, (uint32_t)n, dn);
+        hipLaunchKernelGGL((search_span_kernel<F, OUT_SCATTER, false>), dim3((unsigned)nb), dim3(kThreads), lds, s,
+                           tv, levels, q, pos, out_vals, out_found, (uint32_t)n, dn);
       return hipGetLastError();
     }
   }
