@@ -21,8 +21,10 @@ size_t sorted_lds_bytes(const TreeView& tv, int levels, int Q) {
 }
 
 // Leaf step of a group's Q descents: exact match of t[i] in leaf[i]; the result of query qi[i] goes to
-// (ov, of)[qi[i]] (or to pos[qi[i]] with OUT_SCATTER), skipped for qi[i] >= end.
-template <int G, int Q, int OUT>
+// (ov, of)[qi[i]] (or to pos[qi[i]] with OUT_SCATTER), skipped for qi[i] >= end.  EARLY_POS: the
+// OUT_SCATTER positions are loaded with the leaves (a load inside the writer's branch waits a round trip
+// per query); off where the Q extra registers would spill.
+template <int G, int Q, int OUT, bool EARLY_POS = true>
 __device__ __forceinline__ void leaf_step(const TreeView& tv, const uint64_t (&t)[Q], const uint32_t (&leaf)[Q],
                                           const uint32_t (&qi)[Q], uint32_t end, const uint32_t* __restrict__ pos,
                                           uint64_t* __restrict__ ov, uint8_t* __restrict__ of, int gl, int gbase) {
@@ -31,11 +33,14 @@ __device__ __forceinline__ void leaf_step(const TreeView& tv, const uint64_t (&t
   // line, an L2 hit); G > 1: the group reads keys and values of the leaf line together.
   constexpr bool kValsWithKeys = G > 1;
   uint64_t k[Q][W], v[Q][W];
+  constexpr bool kEarly = OUT != OUT_DIRECT && EARLY_POS;
+  uint32_t o[kEarly ? Q : 1];
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
     const Leaf* lf = tv.leaves + leaf[i];
     load_keys<W>(lf->key + gl * W, k[i]);
     if constexpr (kValsWithKeys) load_keys<W>(lf->val + gl * W, v[i]);
+    if constexpr (kEarly) o[i] = pos[qi[i] < end ? qi[i] : end - 1];
   }
 #pragma unroll
   for (int i = 0; i < Q; ++i) {
@@ -56,9 +61,13 @@ __device__ __forceinline__ void leaf_step(const TreeView& tv, const uint64_t (&t
         ov[qi[i]] = val;
         of[qi[i]] = eq ? 1 : 0;
       } else {
-        const uint32_t o = pos[qi[i]];
-        ov[o] = val;
-        of[o] = eq ? 1 : 0;
+        uint32_t p;
+        if constexpr (kEarly)
+          p = o[i];
+        else
+          p = pos[qi[i]];
+        ov[p] = val;
+        of[p] = eq ? 1 : 0;
       }
     }
   }
@@ -390,7 +399,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(5))) v
     uint32_t leaf[Q];
 #pragma unroll
     for (int i = 0; i < Q; ++i) leaf[i] = node[i] - tv.n_inner;
-    leaf_step<G, Q, OUT>(tv, t, leaf, qi, end, pos, out_vals, out_found, gl, gbase);
+    leaf_step<G, Q, OUT, (F < 128)>(tv, t, leaf, qi, end, pos, out_vals, out_found, gl, gbase);
   }
 }
 
@@ -688,6 +697,8 @@ hipError_t launch_fgq(const TreeView& tv, const SearchLaunch& sl, const uint64_t
       while (levels > 0 && span_lds_bytes(tv, levels) > 65536) --levels;
       const size_t lds = span_lds_bytes(tv, levels);
       const bool pf = n <= (1u << 31) - 1024;  // the prefetch's u32 dword indices fit
+      // (fanout 128 with scattered output has no prefetch form: its registers would spill)
+      constexpr bool kScatterPf = F < 128;
       if (mode == OUT_DIRECT && pf)
         hipLaunchKernelGGL((search_span_kernel<F, OUT_DIRECT, true>), dim3((unsigned)nb), dim3(kThreads), lds, s,
                            tv, levels, q, pos, out_vals, out_found, (uint32_t)n, dn);
@@ -695,8 +706,8 @@ hipError_t launch_fgq(const TreeView& tv, const SearchLaunch& sl, const uint64_t
         hipLaunchKernelGGL((search_span_kernel<F, OUT_DIRECT, false>), dim3((unsigned)nb), dim3(kThreads), lds, s,
                            tv, levels, q, pos, out_vals, out_found, (uint32_t)n, dn);
       else if (pf)
-        hipLaunchKernelGGL((search_span_kernel<F, OUT_SCATTER, true>), dim3((unsigned)nb), dim3(kThreads), lds, s,
-                           tv, levels, q, pos, out_vals, out_found, (uint32_t)n, dn);
+        hipLaunchKernelGGL((search_span_kernel<F, OUT_SCATTER, kScatterPf>), dim3((unsigned)nb), dim3(kThreads), lds,
+                           s, tv, levels, q, pos, out_vals, out_found, (uint32_t)n, dn);
       else
         hipLaunchKernelGGL((search_span_kernel<F, OUT_SCATTER, false>), dim3((unsigned)nb), dim3(kThreads), lds, s,
                            tv, levels, q, pos, out_vals, out_found, (uint32_t)n, dn);
