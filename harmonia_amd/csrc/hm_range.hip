@@ -11,7 +11,10 @@
 // per step (their addresses are known: i, i + 1, ...), so each group keeps several loads in flight
 // instead of one dependent load per leaf; the leaves are then consumed in order exactly as one at a
 // time.  A step loads no more leaves than the range can still use (ceil(rest / 8) + 1: a leaf holds
-// at most 8 keys, and the first one may hold keys below start).
+// at most 8 keys, and the first one may hold keys below start).  The leaves go to LDS by LDS-direct
+// loads, one 16-byte piece per lane (a group's 8 lanes bring its whole 128-B leaf in one instruction,
+// where an 8-byte key load and an 8-byte value load per lane took two), and the lanes read their key
+// and value from there: 8 KiB of LDS per wave.
 // Ranges run in input order: consecutive groups then write consecutive output regions.  (Walking
 // them in PSA order of their starts, as the reference sorts range starts, H/src/range.cu:280-285,
 // was measured: the leaf reads gain locality, the output writes lose it, and the kernel time did
@@ -38,6 +41,12 @@ __global__ __launch_bounds__(kThreads) void range_kernel(TreeView tv, const uint
   const uint32_t gid = threadIdx.x / G;
   constexpr uint32_t gpb = kThreads / G;
   const uint64_t lanes_below = (1ull << gl) - 1ull;
+  typedef __attribute__((address_space(1))) void global_void_t;
+  typedef __attribute__((address_space(3))) void lds_void_t;
+  // the wave's leaves of one step: step u holds its 8 groups' leaves, lane l's 16-byte piece at byte 16 l
+  __shared__ __align__(16) uint64_t lbuf[kThreads / 64][kWalk][128];
+  uint64_t* wb = &lbuf[threadIdx.x >> 6][0][0];
+  const int glane = threadIdx.x & 63;
   for (uint32_t rb = blockIdx.x * gpb; rb < n; rb += gridDim.x * gpb) {  // uniform across the block
     const uint32_t ri = rb + gid;
     const bool valid = ri < n;
@@ -55,10 +64,22 @@ __global__ __launch_bounds__(kThreads) void range_kernel(TreeView tv, const uint
       const uint32_t step = need < (uint32_t)kWalk ? need : (uint32_t)kWalk;
 #pragma unroll
       for (int u = 0; u < kWalk; ++u) {
-        const bool in = (uint32_t)u < step && leaf + u < tv.n_leaves;
-        k[u] = in ? tv.leaves[leaf + u].key[gl] : KEY_MAX;
-        v[u] = in ? tv.leaves[leaf + u].val[gl] : 0ull;
+        if ((uint32_t)u < step) {  // group-uniform; the load writes LDS only
+          const uint32_t lf = leaf + u < tv.n_leaves ? leaf + u : tv.n_leaves - 1;
+          __builtin_amdgcn_global_load_lds((const global_void_t*)(reinterpret_cast<const ulonglong2*>(tv.leaves + lf) + gl),
+                                           (lds_void_t*)(wb + u * 128), 16, 0, 0);
+        }
       }
+      __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's leaves are in LDS
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int u = 0; u < kWalk; ++u) {
+        const bool in = (uint32_t)u < step && leaf + u < tv.n_leaves;
+        const uint64_t* lp = wb + u * 128 + (glane - gl) * 2;  // this group's leaf
+        k[u] = in ? lp[gl] : KEY_MAX;
+        v[u] = in ? lp[8 + gl] : 0ull;
+      }
+      __builtin_amdgcn_wave_barrier();  // the next step's loads overwrite the buffer after these reads
 #pragma unroll
       for (int u = 0; u < kWalk; ++u) {
         if (!done && (uint32_t)u < step) {  // group-uniform
