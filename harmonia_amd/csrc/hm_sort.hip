@@ -132,11 +132,13 @@ __global__ __launch_bounds__(NT) void radix_scatter_kernel(const K* __restrict__
   {
     // digit d starts at the digits below it (scan of the totals after the 256 x nb counts) plus its
     // count in the tiles before this one (threads >= 256 take part in the scans with zeros)
+    // (both loads issued before the scan: the tile's own offset does not depend on it)
     const bool dig = threadIdx.x < 256;
+    const uint32_t tot = dig ? offsets[(size_t)256 * nb + threadIdx.x] : 0u;
+    const uint32_t mine = dig ? offsets[(size_t)threadIdx.x * nb + tile] : 0u;
     uint32_t all;
-    const uint32_t dbase =
-        block_exclusive_scan<uint32_t, NT>(dig ? offsets[(size_t)256 * nb + threadIdx.x] : 0u, scan_sh, all);
-    if (dig) goff[threadIdx.x] = dbase + offsets[(size_t)threadIdx.x * nb + tile];
+    const uint32_t dbase = block_exclusive_scan<uint32_t, NT>(tot, scan_sh, all);
+    if (dig) goff[threadIdx.x] = dbase + mine;
   }
   __syncthreads();
 
@@ -244,7 +246,9 @@ __global__ __launch_bounds__(kUThreads) void radix_unscatter_kernel(const uint64
       const size_t idx = (size_t)d * nb + tile;
       tot = offsets[(size_t)256 * nb + d];
       g = offsets[idx];
-      c = (tile + 1 < nb ? offsets[idx + 1] : tot) - g;
+      // loaded unconditionally so the three loads go out together (idx + 1 <= 256 nb: in bounds)
+      const uint32_t nxt = offsets[idx + 1];
+      c = (tile + 1 < nb ? nxt : tot) - g;
     }
     uint32_t all;
     const uint32_t dbase = block_exclusive_scan<uint32_t, kUThreads>(tot, scan_sh, all);
