@@ -314,6 +314,37 @@ def batched_uneven(dev, n_batches=10000, keys_log2=25, seed=4):
     return res
 
 
+def hbm_streaming(reps=10):
+    """What plain streaming kernels reach on this GPU, beside the published 8 TB/s peak: torch's copy of a
+    1-GiB int64 tensor (1 GiB read + 1 GiB written per launch) and its max-reduction over one (1 GiB read).
+    HIP events on the current stream, where torch launches them; the average over `reps` launches after
+    two warm-up launches."""
+    import torch
+
+    n = (1 << 30) // 8
+    x = torch.arange(n, dtype=torch.int64, device="cuda:0")
+    y = torch.empty_like(x)
+    res = {}
+    for name, fn, nbytes in (("copy", lambda: y.copy_(x), 2 * n * 8), ("read", lambda: x.max(), n * 8)):
+        for _ in range(2):
+            fn()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        b.synchronize()
+        ms = a.elapsed_time(b) / reps
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        res[name] = {"ms": round(ms, 4), "GB_per_s": round(gbs, 1), "frac_of_peak": round(gbs / PEAK_HBM_GBS, 4)}
+    del x, y
+    torch.cuda.empty_cache()
+    res["what"] = ("torch copy_ of 1 GiB (read + write) and max over 1 GiB (read), the practical streaming "
+                   "rates that the search and range kernels' fractions of 8 TB/s compare with")
+    return res
+
+
 def host_streaming(tree, q, reps=3):
     """Queries and results in host memory (the reference's host-resident bucket loop,
     H/regular_bpt/prefix-sort.cu:261-309): hm_search streams a batch above 2^22 queries through the
@@ -790,6 +821,9 @@ def main():
                     "frac": round(e2e_achieved / PEAK_HBM_GBS, 4), "bytes_per_query": BYTES_PER_QUERY_PSA,
                     "what": "whole hm_search step: PSA sort + search + un-sort to input order"}
 
+    hbm = hbm_streaming()
+    log("hbm_streaming", json.dumps(hbm))
+
     hstream = None
     if not args.no_host_stream:
         log("host streaming ...")
@@ -850,6 +884,7 @@ def main():
         "variants": results,
         "build_ms": round(build_ms, 1),
         "tree": {"height": st.height, "n_inner": st.n_inner, "n_leaves": st.n_leaves, "lds_levels": st.lds_levels},
+        "hbm_streaming": hbm,
         "host_streaming": hstream,
         "ablation": abl,
         "skewed_and_ranges": skewed,
